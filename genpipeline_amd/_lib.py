@@ -1,0 +1,133 @@
+"""ctypes binding of libgpsimp.so (the C ABI declared in include/gp_simp.h).
+
+The library is loaded from the package directory. There is no fallback: if
+the shared object is missing or fails to load, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import threading
+from pathlib import Path
+
+_LIB_PATH = Path(__file__).resolve().parent / "libgpsimp.so"
+_lock = threading.Lock()
+_lib = None
+
+GP_OK, GP_ERR_ARG, GP_ERR_OOM, GP_ERR_HIP, GP_BREAKDOWN, GP_NOT_CONVERGED, GP_ERR_NAN, GP_ERR_STATE = range(8)
+STATUS_NAMES = {0: "ok", 1: "bad argument", 2: "out of device memory", 3: "HIP error",
+                4: "breakdown", 5: "not converged", 6: "nan", 7: "bad state"}
+GP_F32, GP_F64, GP_F16 = 0, 1, 2
+GP_LAW_CLAMP, GP_LAW_EMIN = 0, 1
+GP_OC_CPU, GP_OC_GPU = 0, 1
+GP_KE_HEX8, GP_KE_CUSTOM = 0, 1
+
+
+class GpSpec(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+                ("law", C.c_int32), ("oc_variant", C.c_int32), ("ke_kind", C.c_int32),
+                ("E0", C.c_double), ("Emin", C.c_double), ("nu", C.c_double),
+                ("penal", C.c_double), ("rmin", C.c_double), ("move", C.c_double),
+                ("ke", C.POINTER(C.c_double))]
+
+
+class GpPcgOpts(C.Structure):
+    _fields_ = [("precision", C.c_int32), ("max_iter", C.c_int32), ("tol", C.c_double),
+                ("use_x0", C.c_int32), ("refine", C.c_int32), ("inner_tol", C.c_double),
+                ("max_outer", C.c_int32)]
+
+
+class GpPcgStats(C.Structure):
+    _fields_ = [("status", C.c_int32), ("iters", C.c_int32), ("outer", C.c_int32),
+                ("relres", C.c_double)]
+
+
+class GpOcStats(C.Structure):
+    _fields_ = [("lmid", C.c_double), ("steps", C.c_int32)]
+
+
+class GpSimpOpts(C.Structure):
+    _fields_ = [("n_iters", C.c_int32), ("volfrac", C.c_double), ("pcg", GpPcgOpts),
+                ("tol_schedule", C.c_int32), ("early_stop", C.c_int32),
+                ("compliance_each_iter", C.c_int32), ("warm_start", C.c_int32)]
+
+
+class GpSimpStats(C.Structure):
+    _fields_ = [("iters_done", C.c_int32), ("pcg_iters_total", C.c_int32),
+                ("bisect_steps_total", C.c_int32), ("status", C.c_int32),
+                ("compliance", C.c_double), ("seconds", C.c_double)]
+
+
+P = C.c_void_p
+I32, I64, F64, SZ = C.c_int32, C.c_int64, C.c_double, C.c_size_t
+
+# name -> argtypes (restype is int for every entry point except the two below)
+SIGNATURES = {
+    "gp_footprint_bytes": [C.POINTER(GpSpec), I32, C.POINTER(SZ)],
+    "gp_problem_create": [C.POINTER(GpSpec), C.POINTER(P)],
+    "gp_problem_destroy": [P],
+    "gp_problem_set_bcs": [P, P, I64, P],
+    "gp_problem_set_preserved": [P, P],
+    "gp_problem_load": [P, P, P],
+    "gp_stiffness": [P, P, P, P],
+    "gp_ku": [P, P, P, P, I32, I32, P],
+    "gp_diag": [P, P, P, I32, P],
+    "gp_pcg": [P, P, P, P, C.POINTER(GpPcgOpts), C.POINTER(GpPcgStats), P],
+    "gp_sens": [P, P, P, I32, P, P, P],
+    "gp_filter": [P, P, P, I32, P],
+    "gp_oc": [P, P, P, F64, P, P, C.POINTER(GpOcStats), P],
+    "gp_compliance": [P, P, I32, C.POINTER(F64), P],
+    "gp_simp_run": [P, C.POINTER(GpSimpOpts), P, P, P, C.POINTER(GpSimpStats), P],
+    "gp_generic_sens": [P, P, P, P, I64, F64, P, P],
+    "gp_generic_spmv": [P, P, P, P, I64, I64, F64, P, P],
+    "gp_csr_pcg": [P, P, P, P, P, P, P, I64, I64, I32, F64, C.POINTER(GpPcgStats), P],
+}
+EXTRA = {"gp_last_error": ([], C.c_char_p), "gp_version": ([], C.c_int)}
+
+
+class GpError(RuntimeError):
+    """A non-zero status from the C ABI; ``code`` holds the GP_* value."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{STATUS_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+def lib_path() -> Path:
+    return _LIB_PATH
+
+
+def load():
+    """Load libgpsimp.so once; raises if it is missing (no CPU fallback)."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not _LIB_PATH.exists():
+            raise RuntimeError(
+                f"{_LIB_PATH} is missing: build it with `python -m genpipeline_amd.build` "
+                "(the SIMP path has no CPU fallback)")
+        lib = C.CDLL(str(_LIB_PATH))
+        for name, args in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = C.c_int
+        for name, (args, res) in EXTRA.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = lib
+        return lib
+
+
+def check(rc: int):
+    if rc != GP_OK:
+        raise GpError(rc, load().gp_last_error().decode(errors="replace"))
+
+
+def call(name: str, *args):
+    """Call an entry point and raise GpError on a non-zero status."""
+    check(getattr(load(), name)(*args))
+
+
+def exported_symbols() -> list[str]:
+    return sorted(list(SIGNATURES) + list(EXTRA))

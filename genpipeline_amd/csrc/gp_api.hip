@@ -1,0 +1,900 @@
+// C ABI and host-side drivers of the SIMP inner loop (libgpsimp.so).
+//
+// Host code here owns per-problem device state and sequences the kernels in
+// gp_kernels.hip on the caller's stream. PCG runs without a host round trip
+// per iteration: iterations are issued in chunks, each chunk's device-side
+// status is copied to pinned memory, and the next chunk is already queued
+// while the host inspects the previous one (kernels of a finished solve
+// return immediately).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "gp_internal.hpp"
+
+using namespace gp;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                    \
+  do {                                                                                  \
+    hipError_t _e = (expr);                                                             \
+    if (_e != hipSuccess)                                                               \
+      return fail(GP_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e),    \
+                  __FILE__, __LINE__);                                                  \
+  } while (0)
+
+#define LAUNCHCHK()                                                                     \
+  do {                                                                                  \
+    hipError_t _e = hipGetLastError();                                                  \
+    if (_e != hipSuccess)                                                               \
+      return fail(GP_ERR_HIP, "kernel launch failed: %s (%s:%d)", hipGetErrorString(_e), \
+                  __FILE__, __LINE__);                                                  \
+  } while (0)
+
+inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Exact trilinear hex stiffness of a unit cube, 2x2x2 Gauss, isotropic
+// material with E=1 and Poisson ratio nu; engineering shear strains; DOF
+// order 3*corner + component with the C3D8 corner order.
+void hex8_ke(double nu, double* K) {
+  static const int C[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                              {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+  const double lam = nu / ((1 + nu) * (1 - 2 * nu));
+  const double mu = 1.0 / (2 * (1 + nu));
+  double D[6][6] = {};
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) D[a][b] = lam + (a == b ? 2 * mu : 0.0);
+  for (int a = 3; a < 6; ++a) D[a][a] = mu;
+  std::memset(K, 0, sizeof(double) * 576);
+  const double gp1 = 1.0 / std::sqrt(3.0);
+  for (int qa = 0; qa < 2; ++qa)
+    for (int qb = 0; qb < 2; ++qb)
+      for (int qc = 0; qc < 2; ++qc) {
+        const double xi[3] = {qa ? gp1 : -gp1, qb ? gp1 : -gp1, qc ? gp1 : -gp1};
+        double B[6][24] = {};
+        for (int n = 0; n < 8; ++n) {
+          double s[3], dN[3];
+          for (int d = 0; d < 3; ++d) s[d] = 2.0 * C[n][d] - 1.0;
+          // N = prod_d (1 + s_d xi_d) / 8 ; d/dx = 2 d/dxi on the unit cube
+          for (int d = 0; d < 3; ++d) {
+            double v = s[d] / 8.0;
+            for (int o = 0; o < 3; ++o)
+              if (o != d) v *= (1.0 + s[o] * xi[o]);
+            dN[d] = 2.0 * v;
+          }
+          B[0][3 * n] = dN[0];
+          B[1][3 * n + 1] = dN[1];
+          B[2][3 * n + 2] = dN[2];
+          B[3][3 * n] = dN[1];
+          B[3][3 * n + 1] = dN[0];
+          B[4][3 * n + 1] = dN[2];
+          B[4][3 * n + 2] = dN[1];
+          B[5][3 * n] = dN[2];
+          B[5][3 * n + 2] = dN[0];
+        }
+        double DB[6][24];
+        for (int a = 0; a < 6; ++a)
+          for (int c = 0; c < 24; ++c) {
+            double v = 0;
+            for (int b = 0; b < 6; ++b) v += D[a][b] * B[b][c];
+            DB[a][c] = v;
+          }
+        for (int r = 0; r < 24; ++r)
+          for (int c = 0; c < 24; ++c) {
+            double v = 0;
+            for (int a = 0; a < 6; ++a) v += B[a][r] * DB[a][c];
+            K[r * 24 + c] += v * 0.125;  // det J * weights = (1/2)^3
+          }
+      }
+}
+
+}  // namespace
+
+// --------------------------------------------------------------- problem
+struct gp_problem {
+  gp_spec spec{};
+  Grid g{};
+  int device = 0;
+  double ke64_h[576];
+  Stencil st_h{};
+  // device constants
+  double* ke64 = nullptr;
+  float* ke32 = nullptr;
+  Stencil* st = nullptr;
+  double* Hs = nullptr;
+  double* w = nullptr;          // OC volume weights
+  uint8_t* fixmask = nullptr;   // per node, bit c = DOF c fixed
+  uint8_t* pres = nullptr;      // per element or null
+  double n_pres = 0;
+  double* f = nullptr;          // load vector (float64)
+  bool has_bcs = false;
+  // scalars
+  DevScalars* sc = nullptr;     // [0] PCG, [1] OC/dots, [2] refinement
+  DevScalars* h_sc = nullptr;   // pinned mirrors, 3 slots
+  double* partials = nullptr;
+  size_t n_partials = 0;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  // PCG workspace per precision
+  void* ws32 = nullptr;         // r z p Ap minv (float, 5*ndof)
+  void* ws64 = nullptr;         // r z p Ap minv (double, 5*ndof)
+  double* r64 = nullptr;        // refinement residual
+  double* y64 = nullptr;        // refinement K u
+  float* d32 = nullptr;         // refinement correction
+  // SIMP loop workspace
+  float* E = nullptr;
+  double* dc = nullptr;
+  double* dcf = nullptr;
+  double* xnew = nullptr;
+  double* xphys_new = nullptr;
+  double* ce_sum = nullptr;
+};
+
+namespace {
+
+int alloc(void** p, size_t bytes) {
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(GP_ERR_OOM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  }
+  return GP_OK;
+}
+#define ALLOC(ptr, bytes)                                      \
+  do {                                                         \
+    int _r = alloc(reinterpret_cast<void**>(&(ptr)), (bytes)); \
+    if (_r) return _r;                                         \
+  } while (0)
+
+size_t partial_count(const Grid& g) {
+  size_t n = 3 * (size_t)kMaxPartials;
+  n = std::max(n, (size_t)grid_blocks(g.nn));
+  n = std::max(n, (size_t)grid_blocks(g.ne));
+  return n;
+}
+
+int validate_spec(const gp_spec* s) {
+  if (!s) return fail(GP_ERR_ARG, "spec is NULL");
+  if (s->nx < 1 || s->ny < 1 || s->nz < 1) return fail(GP_ERR_ARG, "grid must be >= 1 per axis");
+  const long long nn = (long long)(s->nx + 1) * (s->ny + 1) * (s->nz + 1);
+  if (nn > (1ll << 29)) return fail(GP_ERR_ARG, "grid too large (%lld nodes)", nn);
+  if (s->law != GP_LAW_CLAMP && s->law != GP_LAW_EMIN) return fail(GP_ERR_ARG, "bad law");
+  if (s->oc_variant != GP_OC_CPU && s->oc_variant != GP_OC_GPU) return fail(GP_ERR_ARG, "bad oc_variant");
+  if (s->ke_kind == GP_KE_CUSTOM && s->ke == nullptr) return fail(GP_ERR_ARG, "custom KE is NULL");
+  if (s->ke_kind != GP_KE_HEX8 && s->ke_kind != GP_KE_CUSTOM) return fail(GP_ERR_ARG, "bad ke_kind");
+  if (!(s->rmin > 0) || s->rmin > 3.0) return fail(GP_ERR_ARG, "rmin must be in (0, 3]");
+  if (s->ke_kind == GP_KE_HEX8 && !(s->nu > -1.0 && s->nu < 0.5)) return fail(GP_ERR_ARG, "nu out of range");
+  return GP_OK;
+}
+
+Grid make_grid(const gp_spec* s) {
+  Grid g;
+  g.nx = s->nx;
+  g.ny = s->ny;
+  g.nz = s->nz;
+  g.sJ = s->nz + 1;
+  g.sI = (s->ny + 1) * (s->nz + 1);
+  g.eJ = s->nz;
+  g.eI = s->ny * s->nz;
+  g.nn = (s->nx + 1) * g.sI;
+  g.ne = s->nx * s->ny * s->nz;
+  g.ndof = 3ll * g.nn;
+  return g;
+}
+
+void make_stencil(double rmin, Stencil* st) {
+  const int r = (int)std::ceil(rmin);
+  st->n = 0;
+  for (int di = -r; di <= r; ++di)
+    for (int dj = -r; dj <= r; ++dj)
+      for (int dk = -r; dk <= r; ++dk) {
+        const double dist = std::sqrt((double)(di * di + dj * dj + dk * dk));
+        if (dist > rmin) continue;
+        const double w = rmin - dist;
+        if (!(w > 0.0)) continue;
+        st->di[st->n] = di;
+        st->dj[st->n] = dj;
+        st->dk[st->n] = dk;
+        st->w[st->n] = w;
+        st->n++;
+      }
+}
+
+int sync_read(gp_problem* P, int slot, hipStream_t s) {
+  HIPCHK(hipMemcpyAsync(&P->h_sc[slot], &P->sc[slot], sizeof(DevScalars), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return GP_OK;
+}
+
+int ensure_pcg_ws(gp_problem* P, int prec) {
+  const size_t nd = (size_t)P->g.ndof;
+  if (prec == GP_F64) {
+    if (!P->ws64) ALLOC(P->ws64, 5 * nd * sizeof(double));
+  } else {
+    if (!P->ws32) ALLOC(P->ws32, 5 * nd * sizeof(float));
+  }
+  return GP_OK;
+}
+
+int ensure_refine_ws(gp_problem* P) {
+  const size_t nd = (size_t)P->g.ndof;
+  if (!P->r64) ALLOC(P->r64, nd * sizeof(double));
+  if (!P->y64) ALLOC(P->y64, nd * sizeof(double));
+  if (!P->d32) ALLOC(P->d32, nd * sizeof(float));
+  return GP_OK;
+}
+
+int ensure_simp_ws(gp_problem* P) {
+  const size_t ne = (size_t)P->g.ne;
+  if (!P->E) ALLOC(P->E, ne * sizeof(float));
+  if (!P->dc) ALLOC(P->dc, ne * sizeof(double));
+  if (!P->dcf) ALLOC(P->dcf, ne * sizeof(double));
+  if (!P->xnew) ALLOC(P->xnew, ne * sizeof(double));
+  if (!P->xphys_new) ALLOC(P->xphys_new, ne * sizeof(double));
+  if (!P->ce_sum) ALLOC(P->ce_sum, sizeof(double));
+  return GP_OK;
+}
+
+// Plain PCG in precision T on the free DOFs. u: T*, in x0 / out solution.
+template <typename T>
+int pcg_core(gp_problem* P, const float* E, const double* f, T* u, double tol, int max_iter,
+             bool use_x0, gp_pcg_stats* st, hipStream_t s) {
+  const Grid& g = P->g;
+  const size_t nd = (size_t)g.ndof;
+  T* base = reinterpret_cast<T*>(std::is_same<T, float>::value ? P->ws32 : P->ws64);
+  T* r = base;
+  T* z = base + nd;
+  T* p = base + 2 * nd;
+  T* Ap = base + 3 * nd;
+  T* minv = base + 4 * nd;
+  const T* ke = reinterpret_cast<const T*>(std::is_same<T, float>::value ? (const void*)P->ke32
+                                                                          : (const void*)P->ke64);
+  DevScalars* sc = &P->sc[0];
+  launch_diag_minv<T>(g, ke, E, P->fixmask, minv, true, s);
+  if (use_x0) {
+    launch_mask_fixed<T>(g, u, P->fixmask, s);
+    launch_ku<T>(g, ke, E, u, Ap, P->fixmask, nullptr, nullptr, false, s);
+  }
+  launch_pcg_init<T>(g, f, Ap, use_x0, u, r, z, p, minv, sc, P->partials, tol, max_iter, s);
+  LAUNCHCHK();
+
+  // chunked, speculative issue: one chunk always queued behind the one the
+  // host is inspecting.
+  auto issue = [&](int n_it) {
+    for (int it = 0; it < n_it; ++it) {
+      launch_ku<T>(g, ke, E, p, Ap, P->fixmask, sc, P->partials, true, s);
+      launch_pcg_b<T>(g, u, r, z, p, Ap, minv, sc, P->partials, s);
+      launch_pcg_c<T>(g, p, z, sc, s);
+    }
+  };
+  int chunk = 8;
+  int issued = 0;
+  int slot = 0;
+  // the init status doubles as the first checkpoint
+  HIPCHK(hipMemcpyAsync(&P->h_sc[0], sc, sizeof(DevScalars), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipEventRecord(P->ev[0], s));
+  issue(chunk);
+  issued += chunk;
+  HIPCHK(hipMemcpyAsync(&P->h_sc[1], sc, sizeof(DevScalars), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipEventRecord(P->ev[1], s));
+  LAUNCHCHK();
+  while (true) {
+    HIPCHK(hipEventSynchronize(P->ev[slot]));
+    const DevScalars& h = P->h_sc[slot];
+    if (h.done || issued >= max_iter + 2 * chunk) break;
+    chunk = std::min(chunk * 2, 64);
+    issue(chunk);
+    issued += chunk;
+    HIPCHK(hipMemcpyAsync(&P->h_sc[slot], sc, sizeof(DevScalars), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipEventRecord(P->ev[slot], s));
+    LAUNCHCHK();
+    slot ^= 1;
+  }
+  int rc = sync_read(P, 0, s);
+  if (rc) return rc;
+  const DevScalars& h = P->h_sc[0];
+  st->status = h.status;
+  st->iters += h.iter;
+  st->relres = h.bb > 0 ? std::sqrt(h.rr / h.bb) : 0.0;
+  if (!h.done) st->status = GP_NOT_CONVERGED;
+  return GP_OK;
+}
+
+// FP64 iterative refinement: r = f - K u in FP64, correction solved in FP32.
+int pcg_refine(gp_problem* P, const float* E, const double* f, double* u, const gp_pcg_opts* o,
+               gp_pcg_stats* st, hipStream_t s) {
+  const Grid& g = P->g;
+  int rc = ensure_refine_ws(P);
+  if (rc) return rc;
+  rc = ensure_pcg_ws(P, GP_F32);
+  if (rc) return rc;
+  DevScalars* sc2 = &P->sc[2];
+  // ||f_free||^2
+  HIPCHK(hipMemsetAsync(P->y64, 0, g.ndof * sizeof(double), s));
+  launch_residual64(g, f, P->y64, P->fixmask, P->r64, sc2, P->partials, s);
+  LAUNCHCHK();
+  rc = sync_read(P, 2, s);
+  if (rc) return rc;
+  const double bb = P->h_sc[2].dot;
+  if (!o->use_x0) HIPCHK(hipMemsetAsync(u, 0, g.ndof * sizeof(double), s));
+  launch_mask_fixed<double>(g, u, P->fixmask, s);
+  st->status = GP_OK;
+  if (bb == 0.0) {
+    HIPCHK(hipMemsetAsync(u, 0, g.ndof * sizeof(double), s));
+    st->relres = 0.0;
+    return sync_read(P, 2, s);
+  }
+  const double tol2 = o->tol * o->tol * bb;
+  const int max_outer = o->max_outer > 0 ? o->max_outer : 50;
+  double rr = 0.0;
+  for (int k = 0;; ++k) {
+    launch_ku<double>(g, P->ke64, E, u, P->y64, P->fixmask, nullptr, nullptr, false, s);
+    launch_residual64(g, f, P->y64, P->fixmask, P->r64, sc2, P->partials, s);
+    LAUNCHCHK();
+    rc = sync_read(P, 2, s);
+    if (rc) return rc;
+    rr = P->h_sc[2].dot;
+    st->outer = k;
+    if (rr <= tol2 || !std::isfinite(rr)) break;
+    if (k >= max_outer) {
+      st->status = GP_NOT_CONVERGED;
+      break;
+    }
+    gp_pcg_stats in{};
+    rc = pcg_core<float>(P, E, P->r64, P->d32, o->inner_tol > 0 ? o->inner_tol : 1e-3,
+                         o->max_iter, false, &in, s);
+    if (rc) return rc;
+    st->iters += in.iters;
+    launch_add_correction<float>(g, u, P->d32, s);
+    if (in.status == GP_BREAKDOWN) {
+      st->status = GP_BREAKDOWN;
+      // one more residual for the report
+      launch_ku<double>(g, P->ke64, E, u, P->y64, P->fixmask, nullptr, nullptr, false, s);
+      launch_residual64(g, f, P->y64, P->fixmask, P->r64, sc2, P->partials, s);
+      rc = sync_read(P, 2, s);
+      if (rc) return rc;
+      rr = P->h_sc[2].dot;
+      break;
+    }
+  }
+  st->relres = std::sqrt(rr / bb);
+  if (!std::isfinite(rr)) st->status = GP_ERR_NAN;
+  return GP_OK;
+}
+
+int run_oc(gp_problem* P, const double* x, const double* dc, double volfrac, double* xnew,
+           double* xphys, hipStream_t s) {
+  const Grid& g = P->g;
+  DevScalars* sc = &P->sc[1];
+  launch_oc_init(g, dc, P->spec.oc_variant, volfrac, P->spec.move, sc, P->partials, s);
+  const int max_steps = P->spec.oc_variant == GP_OC_GPU ? 30 : 50;
+  for (int i = 0; i < max_steps; ++i)
+    launch_oc_step(g, x, dc, P->w, P->pres, P->n_pres, sc, P->partials, s);
+  launch_oc_final(g, x, dc, P->pres, sc, xnew, s);
+  launch_filter(g, P->st, P->Hs, xnew, xphys, 0, s);
+  launch_set_preserved(g, P->pres, xphys, s);
+  LAUNCHCHK();
+  return GP_OK;
+}
+
+}  // namespace
+
+// ==================================================================== C ABI
+extern "C" {
+
+const char* gp_last_error(void) { return g_err.c_str(); }
+int gp_version(void) { return 100; }
+
+int gp_footprint_bytes(const gp_spec* spec, int32_t precision, size_t* bytes) {
+  int rc = validate_spec(spec);
+  if (rc) return rc;
+  if (!bytes) return fail(GP_ERR_ARG, "bytes is NULL");
+  const Grid g = make_grid(spec);
+  const size_t nd = (size_t)g.ndof, ne = (size_t)g.ne, nn = (size_t)g.nn;
+  size_t b = 576 * (8 + 4) + sizeof(Stencil) + 2 * ne * 8 /*Hs,w*/ + nn /*fixmask*/ + ne /*pres*/ +
+             nd * 8 /*f*/ + 3 * sizeof(DevScalars) + partial_count(g) * 8;
+  b += ne * (4 + 4 * 8) + 8;                                   // SIMP loop workspace
+  if (precision == GP_F64) b += 5 * nd * 8;
+  else b += 5 * nd * 4 + nd * (8 + 8 + 4);                     // FP32 PCG + refinement
+  *bytes = b;
+  return GP_OK;
+}
+
+int gp_problem_create(const gp_spec* spec, gp_problem** out) {
+  int rc = validate_spec(spec);
+  if (rc) return rc;
+  if (!out) return fail(GP_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  gp_problem* P = new gp_problem();
+  P->spec = *spec;
+  P->spec.ke = nullptr;
+  P->g = make_grid(spec);
+  HIPCHK(hipGetDevice(&P->device));
+  if (spec->ke_kind == GP_KE_HEX8)
+    hex8_ke(spec->nu, P->ke64_h);
+  else
+    std::memcpy(P->ke64_h, spec->ke, sizeof(double) * 576);
+  float ke32[576];
+  for (int i = 0; i < 576; ++i) ke32[i] = (float)P->ke64_h[i];
+  make_stencil(spec->rmin, &P->st_h);
+  const Grid& g = P->g;
+  auto cleanup_fail = [&](int code) {
+    gp_problem_destroy(P);
+    return code;
+  };
+#define CALLOC(ptr, bytes)                                                \
+  do {                                                                    \
+    int _r = alloc(reinterpret_cast<void**>(&(ptr)), (bytes));            \
+    if (_r) return cleanup_fail(_r);                                      \
+  } while (0)
+  CALLOC(P->ke64, 576 * sizeof(double));
+  CALLOC(P->ke32, 576 * sizeof(float));
+  CALLOC(P->st, sizeof(Stencil));
+  CALLOC(P->Hs, g.ne * sizeof(double));
+  CALLOC(P->w, g.ne * sizeof(double));
+  CALLOC(P->fixmask, g.nn);
+  CALLOC(P->f, g.ndof * sizeof(double));
+  CALLOC(P->sc, 3 * sizeof(DevScalars));
+  P->n_partials = partial_count(g);
+  CALLOC(P->partials, P->n_partials * sizeof(double));
+#undef CALLOC
+  if (hipHostMalloc(reinterpret_cast<void**>(&P->h_sc), 3 * sizeof(DevScalars)) != hipSuccess)
+    return cleanup_fail(fail(GP_ERR_OOM, "hipHostMalloc failed"));
+  if (hipEventCreateWithFlags(&P->ev[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&P->ev[1], hipEventDisableTiming) != hipSuccess)
+    return cleanup_fail(fail(GP_ERR_HIP, "hipEventCreate failed"));
+  hipError_t e = hipSuccess;
+  e = e != hipSuccess ? e : hipMemcpy(P->ke64, P->ke64_h, sizeof(double) * 576, hipMemcpyHostToDevice);
+  e = e != hipSuccess ? e : hipMemcpy(P->ke32, ke32, sizeof(float) * 576, hipMemcpyHostToDevice);
+  e = e != hipSuccess ? e : hipMemcpy(P->st, &P->st_h, sizeof(Stencil), hipMemcpyHostToDevice);
+  e = e != hipSuccess ? e : hipMemset(P->sc, 0, 3 * sizeof(DevScalars));
+  e = e != hipSuccess ? e : hipMemset(P->fixmask, 0, g.nn);
+  e = e != hipSuccess ? e : hipMemset(P->f, 0, g.ndof * sizeof(double));
+  if (e != hipSuccess) return cleanup_fail(fail(GP_ERR_HIP, "init copy: %s", hipGetErrorString(e)));
+  launch_hs(g, P->st, P->Hs, nullptr);
+  launch_oc_weights(g, P->st, P->Hs, nullptr, P->w, nullptr);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) return cleanup_fail(fail(GP_ERR_HIP, "init kernels: %s", hipGetErrorString(e)));
+  *out = P;
+  return GP_OK;
+}
+
+int gp_problem_destroy(gp_problem* P) {
+  if (!P) return GP_OK;
+  hipSetDevice(P->device);
+  hipDeviceSynchronize();
+  void* dev[] = {P->ke64, P->ke32, P->st, P->Hs, P->w, P->fixmask, P->pres, P->f,
+                 P->sc, P->partials, P->ws32, P->ws64, P->r64, P->y64, P->d32, P->E,
+                 P->dc, P->dcf, P->xnew, P->xphys_new, P->ce_sum};
+  for (void* p : dev)
+    if (p) hipFree(p);
+  if (P->h_sc) hipHostFree(P->h_sc);
+  for (auto& ev : P->ev)
+    if (ev) hipEventDestroy(ev);
+  delete P;
+  return GP_OK;
+}
+
+int gp_problem_set_bcs(gp_problem* P, const int64_t* fixed, int64_t n_fixed, const double* f_host) {
+  if (!P) return fail(GP_ERR_ARG, "problem is NULL");
+  if (n_fixed < 0 || (n_fixed > 0 && !fixed)) return fail(GP_ERR_ARG, "bad fixed list");
+  if (!f_host) return fail(GP_ERR_ARG, "f is NULL");
+  HIPCHK(hipSetDevice(P->device));
+  const Grid& g = P->g;
+  std::vector<uint8_t> m(g.nn, 0);
+  for (int64_t q = 0; q < n_fixed; ++q) {
+    const int64_t d = fixed[q];
+    if (d < 0 || d >= g.ndof) return fail(GP_ERR_ARG, "fixed dof %lld out of range", (long long)d);
+    m[d / 3] |= (uint8_t)(1u << (d % 3));
+  }
+  HIPCHK(hipMemcpy(P->fixmask, m.data(), g.nn, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(P->f, f_host, g.ndof * sizeof(double), hipMemcpyHostToDevice));
+  P->has_bcs = true;
+  return GP_OK;
+}
+
+int gp_problem_set_preserved(gp_problem* P, const uint8_t* mask) {
+  if (!P) return fail(GP_ERR_ARG, "problem is NULL");
+  HIPCHK(hipSetDevice(P->device));
+  const Grid& g = P->g;
+  if (mask == nullptr) {
+    if (P->pres) HIPCHK(hipFree(P->pres));
+    P->pres = nullptr;
+    P->n_pres = 0;
+  } else {
+    if (!P->pres) {
+      int rc = alloc(reinterpret_cast<void**>(&P->pres), g.ne);
+      if (rc) return rc;
+    }
+    std::vector<uint8_t> m(mask, mask + g.ne);
+    long long c = 0;
+    for (auto& v : m) {
+      v = v ? 1 : 0;
+      c += v;
+    }
+    P->n_pres = (double)c;
+    HIPCHK(hipMemcpy(P->pres, m.data(), g.ne, hipMemcpyHostToDevice));
+  }
+  launch_oc_weights(g, P->st, P->Hs, P->pres, P->w, nullptr);
+  LAUNCHCHK();
+  HIPCHK(hipDeviceSynchronize());
+  return GP_OK;
+}
+
+int gp_problem_load(gp_problem* P, double* f_dev, void* stream) {
+  if (!P || !f_dev) return fail(GP_ERR_ARG, "NULL argument");
+  HIPCHK(hipSetDevice(P->device));
+  HIPCHK(hipMemcpyAsync(f_dev, P->f, P->g.ndof * sizeof(double), hipMemcpyDeviceToDevice, S(stream)));
+  return GP_OK;
+}
+
+int gp_stiffness(gp_problem* P, const double* xPhys, float* E, void* stream) {
+  if (!P || !xPhys || !E) return fail(GP_ERR_ARG, "NULL argument");
+  HIPCHK(hipSetDevice(P->device));
+  launch_stiffness(P->g, xPhys, E, P->spec.law, P->spec.penal, P->spec.E0, P->spec.Emin, S(stream));
+  LAUNCHCHK();
+  return GP_OK;
+}
+
+int gp_ku(gp_problem* P, const float* E, const void* u, void* y, int32_t dtype, int32_t mask,
+          void* stream) {
+  if (!P || !E || !u || !y) return fail(GP_ERR_ARG, "NULL argument");
+  HIPCHK(hipSetDevice(P->device));
+  const uint8_t* fm = mask ? P->fixmask : nullptr;
+  if (dtype == GP_F32)
+    launch_ku<float>(P->g, P->ke32, E, (const float*)u, (float*)y, fm, nullptr, nullptr, false, S(stream));
+  else if (dtype == GP_F64)
+    launch_ku<double>(P->g, P->ke64, E, (const double*)u, (double*)y, fm, nullptr, nullptr, false, S(stream));
+  else
+    return fail(GP_ERR_ARG, "gp_ku: dtype must be GP_F32 or GP_F64");
+  LAUNCHCHK();
+  return GP_OK;
+}
+
+int gp_diag(gp_problem* P, const float* E, void* diag, int32_t dtype, void* stream) {
+  if (!P || !E || !diag) return fail(GP_ERR_ARG, "NULL argument");
+  HIPCHK(hipSetDevice(P->device));
+  if (dtype == GP_F32)
+    launch_diag_minv<float>(P->g, P->ke32, E, P->fixmask, (float*)diag, false, S(stream));
+  else if (dtype == GP_F64)
+    launch_diag_minv<double>(P->g, P->ke64, E, P->fixmask, (double*)diag, false, S(stream));
+  else
+    return fail(GP_ERR_ARG, "gp_diag: dtype must be GP_F32 or GP_F64");
+  LAUNCHCHK();
+  return GP_OK;
+}
+
+int gp_pcg(gp_problem* P, const float* E, const double* f, void* u, const gp_pcg_opts* o,
+           gp_pcg_stats* st, void* stream) {
+  if (!P || !E || !f || !u || !o || !st) return fail(GP_ERR_ARG, "NULL argument");
+  if (o->max_iter < 1 || !(o->tol >= 0)) return fail(GP_ERR_ARG, "bad max_iter/tol");
+  HIPCHK(hipSetDevice(P->device));
+  std::memset(st, 0, sizeof(*st));
+  hipStream_t s = S(stream);
+  int rc;
+  if (o->refine) {
+    if (o->precision == GP_F16) return fail(GP_ERR_ARG, "GP_F16 storage is not available in this build");
+    rc = pcg_refine(P, E, f, (double*)u, o, st, s);
+  } else if (o->precision == GP_F64) {
+    rc = ensure_pcg_ws(P, GP_F64);
+    if (!rc) rc = pcg_core<double>(P, E, f, (double*)u, o->tol, o->max_iter, o->use_x0 != 0, st, s);
+  } else if (o->precision == GP_F32) {
+    rc = ensure_pcg_ws(P, GP_F32);
+    if (!rc) rc = pcg_core<float>(P, E, f, (float*)u, o->tol, o->max_iter, o->use_x0 != 0, st, s);
+  } else {
+    return fail(GP_ERR_ARG, "bad precision %d", o->precision);
+  }
+  if (rc) return rc;
+  if (st->status == GP_BREAKDOWN) fail(GP_BREAKDOWN, "PCG breakdown (p^T A p <= 0)");
+  return GP_OK;
+}
+
+int gp_sens(gp_problem* P, const double* xPhys, const void* u, int32_t dtype, double* dc,
+            double* ce_sum, void* stream) {
+  if (!P || !xPhys || !u || !dc) return fail(GP_ERR_ARG, "NULL argument");
+  HIPCHK(hipSetDevice(P->device));
+  const gp_spec& sp = P->spec;
+  if (dtype == GP_F32)
+    launch_sens<float>(P->g, P->ke64, xPhys, (const float*)u, sp.law, sp.penal, sp.E0, sp.Emin, dc,
+                       ce_sum, &P->sc[1], P->partials, S(stream));
+  else if (dtype == GP_F64)
+    launch_sens<double>(P->g, P->ke64, xPhys, (const double*)u, sp.law, sp.penal, sp.E0, sp.Emin,
+                        dc, ce_sum, &P->sc[1], P->partials, S(stream));
+  else
+    return fail(GP_ERR_ARG, "gp_sens: dtype must be GP_F32 or GP_F64");
+  LAUNCHCHK();
+  return GP_OK;
+}
+
+int gp_filter(gp_problem* P, const double* in, double* out, int32_t mode, void* stream) {
+  if (!P || !in || !out) return fail(GP_ERR_ARG, "NULL argument");
+  if (mode != 0 && mode != 1) return fail(GP_ERR_ARG, "mode must be 0 or 1");
+  HIPCHK(hipSetDevice(P->device));
+  launch_filter(P->g, P->st, P->Hs, in, out, mode, S(stream));
+  LAUNCHCHK();
+  return GP_OK;
+}
+
+int gp_oc(gp_problem* P, const double* x, const double* dc, double volfrac, double* xnew,
+          double* xphys, gp_oc_stats* stats, void* stream) {
+  if (!P || !x || !dc || !xnew || !xphys) return fail(GP_ERR_ARG, "NULL argument");
+  HIPCHK(hipSetDevice(P->device));
+  int rc = run_oc(P, x, dc, volfrac, xnew, xphys, S(stream));
+  if (rc) return rc;
+  rc = sync_read(P, 1, S(stream));
+  if (rc) return rc;
+  if (stats) {
+    stats->lmid = P->h_sc[1].lmid;
+    stats->steps = P->h_sc[1].oc_steps;
+  }
+  return GP_OK;
+}
+
+int gp_compliance(gp_problem* P, const void* u, int32_t dtype, double* out, void* stream) {
+  if (!P || !u || !out) return fail(GP_ERR_ARG, "NULL argument");
+  HIPCHK(hipSetDevice(P->device));
+  if (dtype == GP_F32)
+    launch_dot<double, float>(P->f, (const float*)u, P->g.ndof, &P->sc[1], P->partials, S(stream));
+  else if (dtype == GP_F64)
+    launch_dot<double, double>(P->f, (const double*)u, P->g.ndof, &P->sc[1], P->partials, S(stream));
+  else
+    return fail(GP_ERR_ARG, "gp_compliance: dtype must be GP_F32 or GP_F64");
+  LAUNCHCHK();
+  int rc = sync_read(P, 1, S(stream));
+  if (rc) return rc;
+  *out = P->h_sc[1].dot;
+  return GP_OK;
+}
+
+int gp_simp_run(gp_problem* P, const gp_simp_opts* o, double* x, double* xPhys, double* u,
+                gp_simp_stats* st, void* stream) {
+  if (!P || !o || !x || !xPhys || !u || !st) return fail(GP_ERR_ARG, "NULL argument");
+  if (!P->has_bcs) return fail(GP_ERR_STATE, "boundary conditions not set");
+  HIPCHK(hipSetDevice(P->device));
+  auto t0 = std::chrono::steady_clock::now();
+  std::memset(st, 0, sizeof(*st));
+  hipStream_t s = S(stream);
+  int rc = ensure_simp_ws(P);
+  if (rc) return rc;
+  const Grid& g = P->g;
+  const gp_spec& sp = P->spec;
+  const bool refine = o->pcg.refine != 0 || o->pcg.precision == GP_F64;
+  // u is float64 state; plain FP32 PCG works on a float copy
+  float* u32 = nullptr;
+  if (!refine) {
+    rc = ensure_refine_ws(P);  // d32 doubles as the FP32 iterate
+    if (rc) return rc;
+    u32 = P->d32;
+    launch_cast_f64_to_f32(u, u32, g.ndof, s);
+  }
+  auto solve = [&](double tol, int max_iter, bool warm, gp_pcg_stats* ps) -> int {
+    gp_pcg_opts po = o->pcg;
+    po.tol = tol;
+    po.max_iter = max_iter;
+    po.use_x0 = warm ? 1 : 0;
+    int r;
+    std::memset(ps, 0, sizeof(*ps));
+    if (refine) {
+      if (po.precision == GP_F64 && !po.refine) {
+        r = ensure_pcg_ws(P, GP_F64);
+        if (!r) r = pcg_core<double>(P, P->E, P->f, u, tol, max_iter, warm, ps, s);
+      } else {
+        r = pcg_refine(P, P->E, P->f, u, &po, ps, s);
+      }
+    } else {
+      r = ensure_pcg_ws(P, GP_F32);
+      if (!r) r = pcg_core<float>(P, P->E, P->f, u32, tol, max_iter, warm, ps, s);
+    }
+    if (r) return r;
+    st->pcg_iters_total += ps->iters;
+    if (ps->status > st->status) st->status = ps->status;
+    return GP_OK;
+  };
+  auto compliance = [&](double* c) -> int {
+    if (refine)
+      launch_dot<double, double>(P->f, u, g.ndof, &P->sc[1], P->partials, s);
+    else
+      launch_dot<double, float>(P->f, u32, g.ndof, &P->sc[1], P->partials, s);
+    int r = sync_read(P, 1, s);
+    if (r) return r;
+    *c = P->h_sc[1].dot;
+    return GP_OK;
+  };
+
+  double prev_c = INFINITY, curr_c = 0.0;
+  int small_changes = 0;
+  bool have_u = o->warm_start != 0;  // caller's u is a valid guess only when asked
+  for (int it = 0; it < o->n_iters; ++it) {
+    double tol = o->pcg.tol;
+    int max_iter = o->pcg.max_iter;
+    if (o->tol_schedule) {
+      if (it < 3) { tol = 1e-4; max_iter = 200; }
+      else if (it < 8) { tol = 1e-5; max_iter = 400; }
+      else { tol = 1e-6; max_iter = 800; }
+    }
+    launch_stiffness(g, xPhys, P->E, sp.law, sp.penal, sp.E0, sp.Emin, s);
+    gp_pcg_stats ps;
+    rc = solve(tol, max_iter, o->warm_start && have_u, &ps);
+    if (rc) return rc;
+    have_u = true;
+    launch_reset_flags(&P->sc[1], s);
+    if (refine)
+      launch_sens<double>(g, P->ke64, xPhys, u, sp.law, sp.penal, sp.E0, sp.Emin, P->dc, nullptr,
+                          &P->sc[1], P->partials, s);
+    else
+      launch_sens<float>(g, P->ke64, xPhys, u32, sp.law, sp.penal, sp.E0, sp.Emin, P->dc, nullptr,
+                         &P->sc[1], P->partials, s);
+    launch_filter(g, P->st, P->Hs, P->dc, P->dcf, 1, s);
+    rc = run_oc(P, x, P->dcf, o->volfrac, P->xnew, P->xphys_new, s);
+    if (rc) return rc;
+    rc = sync_read(P, 1, s);
+    if (rc) return rc;
+    if (P->h_sc[1].nan_flag) {
+      st->status = GP_ERR_NAN;
+      break;  // reference aborts before the OC update is applied
+    }
+    st->bisect_steps_total += P->h_sc[1].oc_steps;
+    HIPCHK(hipMemcpyAsync(x, P->xnew, g.ne * sizeof(double), hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(xPhys, P->xphys_new, g.ne * sizeof(double), hipMemcpyDeviceToDevice, s));
+    st->iters_done = it + 1;
+    if (o->compliance_each_iter) {
+      // simp_solver_gpu.py:747-751 -- a fresh solve (zero start, tol 1e-6,
+      // 2000 its); its u becomes the next warm start (simp_solver_gpu.py:593,704)
+      launch_stiffness(g, xPhys, P->E, sp.law, sp.penal, sp.E0, sp.Emin, s);
+      rc = solve(1e-6 > o->pcg.tol ? o->pcg.tol : 1e-6, 2000, false, &ps);
+      if (rc) return rc;
+      rc = compliance(&curr_c);
+      if (rc) return rc;
+      if (o->early_stop) {
+        if (std::isfinite(prev_c)) {
+          const double ch = std::fabs(prev_c - curr_c) / (prev_c + 1e-10);
+          small_changes = ch < 0.01 ? small_changes + 1 : 0;
+          if (small_changes >= 3) break;
+        }
+        prev_c = curr_c;
+      }
+    }
+  }
+  if (o->compliance_each_iter && st->iters_done > 0) {
+    st->compliance = curr_c;
+  } else {
+    // simp_solver.py:78 -- a final solve at the returned density
+    launch_stiffness(g, xPhys, P->E, sp.law, sp.penal, sp.E0, sp.Emin, s);
+    gp_pcg_stats ps;
+    rc = solve(o->pcg.tol, o->pcg.max_iter, o->warm_start != 0, &ps);
+    if (rc) return rc;
+    rc = compliance(&st->compliance);
+    if (rc) return rc;
+  }
+  if (!refine) launch_cast_f32_to_f64(u32, u, g.ndof, s);
+  LAUNCHCHK();
+  HIPCHK(hipStreamSynchronize(s));
+  st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return GP_OK;
+}
+
+// ------------------------------------------------------------ generic API
+int gp_generic_sens(const float* xPhys, const float* u, const float* ke, const int32_t* edof,
+                    int64_t ne, double penal, float* dc, void* stream) {
+  if (!xPhys || !u || !ke || !edof || !dc || ne < 0) return fail(GP_ERR_ARG, "bad argument");
+  if (ne == 0) return GP_OK;
+  launch_generic_sens(xPhys, u, ke, edof, ne, penal, dc, S(stream));
+  LAUNCHCHK();
+  return GP_OK;
+}
+
+int gp_generic_spmv(const float* xPhys, const float* u, const float* ke, const int32_t* edof,
+                    int64_t ne, int64_t ndof, double penal, float* y, void* stream) {
+  if (!xPhys || !u || !ke || !edof || !y || ne < 0 || ndof < 0) return fail(GP_ERR_ARG, "bad argument");
+  HIPCHK(hipMemsetAsync(y, 0, ndof * sizeof(float), S(stream)));
+  if (ne == 0) return GP_OK;
+  launch_generic_spmv(xPhys, u, ke, edof, ne, penal, y, S(stream));
+  LAUNCHCHK();
+  return GP_OK;
+}
+
+// simp_pcg_cuda.cu:66-124 restated: x starts at x0 but r starts at b (the
+// reference never forms b - A x0); z = r / diagonal; ||b|| is the unmasked
+// norm; |p^T A p| < 1e-30 stops the loop. Host-synchronous per iteration,
+// like the reference.
+int gp_csr_pcg(const int64_t* rp, const int64_t* ci, const float* v, const float* diagonal,
+               const float* b, float* x, const int64_t* fixed, int64_t n_fixed, int64_t n,
+               int32_t max_iter, double tol, gp_pcg_stats* st, void* stream) {
+  if (!rp || !ci || !v || !diagonal || !b || !x || !st || n < 0 || n_fixed < 0 ||
+      (n_fixed > 0 && !fixed))
+    return fail(GP_ERR_ARG, "bad argument");
+  std::memset(st, 0, sizeof(*st));
+  if (n == 0) return GP_OK;
+  hipStream_t s = S(stream);
+  float *r = nullptr, *z = nullptr, *p = nullptr, *Ap = nullptr;
+  DevScalars* sc = nullptr;
+  double* part = nullptr;
+  DevScalars* h = nullptr;
+  int rc = GP_OK;
+  auto cleanup = [&]() {
+    for (void* q : {(void*)r, (void*)z, (void*)p, (void*)Ap, (void*)sc, (void*)part})
+      if (q) hipFree(q);
+    if (h) hipHostFree(h);
+  };
+  auto dot = [&](const float* a, const float* c, double* out) -> int {
+    launch_dot<float, float>(a, c, n, sc, part, s);
+    HIPCHK(hipMemcpyAsync(h, sc, sizeof(DevScalars), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    *out = h->dot;
+    return GP_OK;
+  };
+  do {
+    if ((rc = alloc((void**)&r, n * 4)) || (rc = alloc((void**)&z, n * 4)) ||
+        (rc = alloc((void**)&p, n * 4)) || (rc = alloc((void**)&Ap, n * 4)) ||
+        (rc = alloc((void**)&sc, sizeof(DevScalars))) ||
+        (rc = alloc((void**)&part, 3 * kMaxPartials * sizeof(double))))
+      break;
+    if (hipHostMalloc((void**)&h, sizeof(DevScalars)) != hipSuccess) {
+      rc = fail(GP_ERR_OOM, "hipHostMalloc failed");
+      break;
+    }
+    if (hipMemsetAsync(sc, 0, sizeof(DevScalars), s) != hipSuccess ||
+        hipMemcpyAsync(r, b, n * 4, hipMemcpyDeviceToDevice, s) != hipSuccess) {
+      rc = fail(GP_ERR_HIP, "init copy failed");
+      break;
+    }
+    launch_zero_list(r, fixed, n_fixed, s);
+    launch_vec(z, r, diagonal, 0.f, n, 2, s);
+    hipMemcpyAsync(p, z, n * 4, hipMemcpyDeviceToDevice, s);
+    double rz, bn2;
+    if ((rc = dot(r, z, &rz)) || (rc = dot(b, b, &bn2))) break;
+    const double bnorm = std::sqrt(bn2);
+    int it = 0;
+    double rr = 0;
+    for (; it < max_iter; ++it) {
+      launch_csr_spmv(rp, ci, v, p, Ap, n, s);
+      launch_zero_list(Ap, fixed, n_fixed, s);
+      double den;
+      if ((rc = dot(p, Ap, &den))) break;
+      if (std::fabs(den) < 1e-30) break;
+      const float alpha = (float)(rz / den);
+      launch_vec(x, p, nullptr, alpha, n, 1, s);
+      launch_vec(r, Ap, nullptr, -alpha, n, 1, s);
+      launch_zero_list(r, fixed, n_fixed, s);
+      if ((rc = dot(r, r, &rr))) break;
+      if (std::sqrt(rr) < tol * bnorm) {
+        ++it;
+        break;
+      }
+      launch_vec(z, r, diagonal, 0.f, n, 2, s);
+      double rzn;
+      if ((rc = dot(r, z, &rzn))) break;
+      launch_vec(p, z, nullptr, (float)(rzn / rz), n, 0, s);
+      rz = rzn;
+    }
+    st->iters = it;
+    st->relres = bnorm > 0 ? std::sqrt(rr) / bnorm : 0.0;
+    st->status = (it >= max_iter) ? GP_NOT_CONVERGED : GP_OK;
+    if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = fail(GP_ERR_HIP, "sync failed");
+  } while (false);
+  cleanup();
+  return rc;
+}
+
+}  // extern "C"

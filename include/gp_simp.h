@@ -1,0 +1,205 @@
+/*
+ * gp_simp.h -- C ABI of the MI355X SIMP inner-loop library (libgpsimp.so).
+ *
+ * Plain C: every argument is a POD, a pointer or a size. Device pointers are
+ * caller-owned HIP allocations (e.g. torch tensor data_ptr()), contiguous, of
+ * the dtype named in the comment. `stream` is a hipStream_t passed as void*
+ * (NULL = the null stream). Every function returns GP_OK (0) or an error code;
+ * gp_last_error() returns a thread-local message for the last failure.
+ *
+ * Numbering conventions (identical to genpipeline/topology/simp_solver.py):
+ *   element e = i*ny*nz + j*nz + k                      (simp_solver.py:276)
+ *   node    n = i*(ny+1)*(nz+1) + j*(nz+1) + k          (simp_solver.py:277,96)
+ *   dof       = 3*n + c, c in {x,y,z}                   (simp_solver.py:106)
+ *   element corner order (C3D8): (0,0,0) (1,0,0) (1,1,0) (0,1,0)
+ *                                (0,0,1) (1,0,1) (1,1,1) (0,1,1)  (:255-267)
+ */
+#ifndef GP_SIMP_H
+#define GP_SIMP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define GP_OK 0
+#define GP_ERR_ARG 1          /* bad argument / shape                       */
+#define GP_ERR_OOM 2          /* device allocation failed                   */
+#define GP_ERR_HIP 3          /* HIP runtime error                          */
+#define GP_BREAKDOWN 4        /* PCG: p^T A p <= 0 (indefinite operator)    */
+#define GP_NOT_CONVERGED 5    /* PCG: max_iter reached                      */
+#define GP_ERR_NAN 6          /* NaN detected in a result                   */
+#define GP_ERR_STATE 7        /* call out of order (e.g. no BCs set)        */
+
+/* ---- enums ----------------------------------------------------------- */
+#define GP_F32 0              /* PCG vectors float32, FP64 dots              */
+#define GP_F64 1              /* PCG vectors float64                          */
+#define GP_F16 2              /* p / KE coefficients stored fp16, fp32 math,  */
+                              /* FP64 iterative refinement around it          */
+
+#define GP_LAW_CLAMP 0        /* E = max(x,1e-3)^p (simp_solver.py:201-203) */
+#define GP_LAW_EMIN 1         /* E = Emin + x^p (E0-Emin)                   */
+
+#define GP_OC_CPU 0           /* l2=1e9, <=50 steps, tol 1e-4, floor 1e-12  */
+                              /* (simp_solver.py:225-249)                    */
+#define GP_OC_GPU 1           /* l2=2max(max(-dc),1), <=30 steps, tol 1e-5,  */
+                              /* floor 1e-15 (simp_solver_gpu.py:801-841)    */
+
+#define GP_KE_HEX8 0          /* exact 2x2x2-Gauss trilinear hex (default)   */
+#define GP_KE_CUSTOM 1        /* caller-supplied 24x24 table (gp_spec.ke)    */
+
+/* ---- problem specification ------------------------------------------ */
+typedef struct gp_spec {
+  int32_t nx, ny, nz;         /* elements per axis                           */
+  int32_t law;                /* GP_LAW_*                                    */
+  int32_t oc_variant;         /* GP_OC_*                                     */
+  int32_t ke_kind;            /* GP_KE_*                                     */
+  double E0, Emin, nu, penal, rmin, move;
+  const double* ke;           /* host 24x24 row-major if GP_KE_CUSTOM        */
+} gp_spec;
+
+typedef struct gp_problem gp_problem;   /* opaque */
+
+typedef struct gp_pcg_opts {
+  int32_t precision;          /* GP_F32 / GP_F64 / GP_F16                    */
+  int32_t max_iter;           /* inner PCG iteration cap                     */
+  double tol;                 /* ||r|| <= tol ||f_free||  (outer when refine) */
+  int32_t use_x0;             /* 1: u holds the initial guess               */
+  int32_t refine;             /* 1: FP64 iterative refinement around the     */
+                              /*    low-precision PCG (u must be float64)    */
+  double inner_tol;           /* inner PCG tol when refine=1                 */
+  int32_t max_outer;          /* refinement sweeps cap                       */
+} gp_pcg_opts;
+
+typedef struct gp_pcg_stats {
+  int32_t status;             /* GP_OK / GP_BREAKDOWN / GP_NOT_CONVERGED     */
+  int32_t iters;              /* total inner PCG iterations                  */
+  int32_t outer;              /* refinement sweeps (0 without refine)        */
+  double relres;              /* final ||r||/||f|| (FP64 true residual when  */
+                              /* refine=1, recursive residual otherwise)     */
+} gp_pcg_stats;
+
+typedef struct gp_oc_stats {
+  double lmid;                /* final Lagrange multiplier                   */
+  int32_t steps;              /* bisection steps taken                       */
+} gp_oc_stats;
+
+typedef struct gp_simp_opts {
+  int32_t n_iters;            /* OC iterations                               */
+  double volfrac;
+  gp_pcg_opts pcg;
+  int32_t tol_schedule;       /* 1: SIMPSolverGPU schedule 1e-4/1e-5/1e-6 and */
+                              /*    200/400/800 its (simp_solver_gpu.py:95-101)*/
+  int32_t early_stop;         /* 1: stop after 3 consecutive dC<1%           */
+                              /*    (simp_solver_gpu.py:118-125)             */
+  int32_t compliance_each_iter; /* 1: compliance solve every OC iteration     */
+  int32_t warm_start;         /* 1: PCG starts from the previous u           */
+} gp_simp_opts;
+
+typedef struct gp_simp_stats {
+  int32_t iters_done;         /* OC iterations executed                      */
+  int32_t pcg_iters_total;
+  int32_t bisect_steps_total;
+  int32_t status;             /* worst PCG status seen                       */
+  double compliance;          /* final f.u (reference: last_compliance)      */
+  double seconds;             /* wall time of the call                       */
+} gp_simp_stats;
+
+/* ---- library --------------------------------------------------------- */
+const char* gp_last_error(void);
+int gp_version(void);                                 /* 100*major + minor */
+
+/* Device bytes a problem of this spec allocates (admission by footprint). */
+int gp_footprint_bytes(const gp_spec* spec, int32_t precision, size_t* bytes);
+
+/* ---- problem lifecycle ----------------------------------------------- *
+ * Replaces the per-solver set-up of SIMPSolver.__init__ / SIMPSolverGPU.__init__
+ * (simp_solver.py:24-54, simp_solver_gpu.py:16-69): KE, edof (derived
+ * arithmetically in-kernel, no table), filter stencil and Hs.              */
+int gp_problem_create(const gp_spec* spec, gp_problem** out);
+int gp_problem_destroy(gp_problem* p);
+
+/* Fixed DOFs (host int64 list, any order, duplicates allowed) and load
+ * vector f (host float64, n_dof). Replaces _get_bcs (simp_solver.py:81-157,
+ * simp_solver_gpu.py:138-248); the caller computes the lists.             */
+int gp_problem_set_bcs(gp_problem* p, const int64_t* fixed_dofs, int64_t n_fixed,
+                       const double* f_host);
+/* Preserved (non-design) elements: host uint8 per element or NULL.         */
+int gp_problem_set_preserved(gp_problem* p, const uint8_t* mask_host);
+/* Copy of the device load vector into a float64 device buffer (n_dof).      */
+int gp_problem_load(gp_problem* p, double* f_dev, void* stream);
+
+/* ---- operator pieces (stream-ordered, no host sync) ------------------ */
+/* E[e] = law(xPhys[e]) as float32. xPhys float64 (n_elem).                   */
+int gp_stiffness(gp_problem* p, const double* xPhys, float* E, void* stream);
+/* y = K(E) u with fixed DOFs of y zeroed when mask != 0. u,y dtype =
+ * GP_F32 (float) or GP_F64 (double). Replaces the assembled K SpMV
+ * (simp_solver.py:201-213, simp_solver_gpu.py:313-326,341) and fused_spmv
+ * (cuda_kernels/__init__.py:329-368) on the structured grid.              */
+int gp_ku(gp_problem* p, const float* E, const void* u, void* y, int32_t dtype,
+          int32_t mask, void* stream);
+/* diag(K(E)) (dtype as gp_ku), fixed DOFs report their raw diagonal.        */
+int gp_diag(gp_problem* p, const float* E, void* diag, int32_t dtype, void* stream);
+
+/* ---- solve ------------------------------------------------------------ *
+ * Jacobi-PCG on the free DOFs, u_fixed = 0. Replaces SIMPSolverGPU._solve /
+ * _pcg / _pcg_ic (simp_solver_gpu.py:328-475,571-705) and the SuperLU solve
+ * of SIMPSolver (simp_solver.py:213). f float64 device (n_dof). u dtype:
+ * float64 when opts->refine or precision==GP_F64, else float32. Blocks the
+ * host until the solve finishes (the iteration count is a host value).     */
+int gp_pcg(gp_problem* p, const float* E, const double* f, void* u,
+           const gp_pcg_opts* opts, gp_pcg_stats* stats, void* stream);
+
+/* ---- SIMP pieces ------------------------------------------------------ */
+/* dc[e] = -dE/dx(xPhys[e]) * u_e^T KE u_e (float64 out); ce_sum (device
+ * double scalar or NULL) receives sum_e E_e u_e^T KE u_e.
+ * Replaces SIMPSolver._sensitivity's energy part (simp_solver.py:215-218)
+ * and simp_sensitivity (cuda_kernels/__init__.py:286-326).                 */
+int gp_sens(gp_problem* p, const double* xPhys, const void* u, int32_t dtype,
+            double* dc, double* ce_sum, void* stream);
+/* mode 0: out = (H in)/Hs (density filter, simp_solver.py:240)
+ * mode 1: out = H^T (in/Hs) (sensitivity chain rule, simp_solver.py:221-223) */
+int gp_filter(gp_problem* p, const double* in, double* out, int32_t mode, void* stream);
+/* OC bisection update (simp_solver.py:225-249 / simp_solver_gpu.py:801-841).
+ * x, dc: float64 in; xnew, xPhys_new: float64 out. Blocks for stats.       */
+int gp_oc(gp_problem* p, const double* x, const double* dc, double volfrac,
+          double* xnew, double* xPhys_new, gp_oc_stats* stats, void* stream);
+/* out_host = f . u  (FP64 sum). Blocks.                                     */
+int gp_compliance(gp_problem* p, const void* u, int32_t dtype, double* out_host,
+                  void* stream);
+
+/* ---- whole SIMP loop ---------------------------------------------------- *
+ * SIMPSolver.run / SIMPSolverGPU.run (simp_solver.py:56-79,
+ * simp_solver_gpu.py:71-136). x, xPhys: float64 device (n_elem) in/out;
+ * u: float64 device (n_dof) warm-start state in/out.                       */
+int gp_simp_run(gp_problem* p, const gp_simp_opts* opts, double* x, double* xPhys,
+                double* u, gp_simp_stats* stats, void* stream);
+
+/* ---- generic-table kernels (reference kernel API) --------------------- *
+ * Explicit KE (24x24 float32 row-major) and edof (int32 n_elem x 24) as the
+ * reference kernels take them; edof may be arbitrary (no grid assumption).
+ * simp_sensitivity: genpipeline/cuda_kernels/__init__.py:286-326,
+ *   kernel simp_sensitivity_ptx.cu:163-270 -> dc = -p max(x,1e-3)^(p-1) u_e^T KE u_e
+ * fused_spmv:       __init__.py:329-368, kernel simp_sensitivity_ptx.cu:301-373
+ *   -> y = sum_e max(x,1e-3)^p KE u_e scattered (y zero-initialised here)   */
+int gp_generic_sens(const float* xPhys, const float* u, const float* ke,
+                    const int32_t* edof, int64_t n_elem, double penal, float* dc,
+                    void* stream);
+int gp_generic_spmv(const float* xPhys, const float* u, const float* ke,
+                    const int32_t* edof, int64_t n_elem, int64_t n_dof, double penal,
+                    float* y, void* stream);
+/* Jacobi-PCG on a CSR matrix (float32 values, int64 row_ptr/col_idx), fixed
+ * DOFs (int64 list) held at zero. Replaces pcg_solve_cuda
+ * (__init__.py:443-473; simp_pcg_cuda.cu:66-124). x: in x0, out solution.  */
+int gp_csr_pcg(const int64_t* row_ptr, const int64_t* col_idx, const float* values,
+               const float* diagonal, const float* b, float* x, const int64_t* fixed,
+               int64_t n_fixed, int64_t n, int32_t max_iter, double tol,
+               gp_pcg_stats* stats, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GP_SIMP_H */
