@@ -1,0 +1,187 @@
+"""SIMP solves/s on one MI355X, with the K.u roofline and a CPU baseline.
+
+Workload (default): one 64x64x64 SIMP problem, hex8 elements, E0=1,
+Emin=1e-9, nu=0.3, p=3, volfrac 0.3, rmin 1.5, clamp at x=0, 4 seeded unit
+point loads on face x=max (seed 0), FP32 Jacobi-PCG (FP64 dots) at relative
+tolerance 1e-6 with warm starts, 100 OC iterations plus the final compliance
+solve. One step = one complete SIMP solve from the uniform initial density.
+
+Roofline: the dominant kernel is the fused PCG K.u. Its algorithmic bytes
+per launch are 24*n_nodes + 4*n_elems (read p, write Ap: 12 B/node each;
+read E_e: 4 B/element); its average duration comes from HIP events that
+bracket sampled K.u launches on the solver's stream during the timed steps.
+
+CPU baseline: the oracle's multi-core FP64 C restatement (oracle/simp_cpu.c,
+matrix-free Jacobi-PCG, OpenMP over all host cores) on a bounded sample of
+the same problem: the first OC iterations, timed; the per-PCG-iteration cost
+is scaled by the PCG iteration count of the GPU's full solve.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "SIMP solves/sec & K·u GB/s vs HBM peak; batched p99 latency @1e-6 compliance"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--grid", type=int, nargs=3, default=[64, 64, 64])
+    ap.add_argument("--oc-iters", type=int, default=100)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-oc-iters", type=int, default=2)
+    ap.add_argument("--traffic-file", default=str(ROOT / "profiles" / "ku_traffic.json"))
+    return ap.parse_args()
+
+
+def cpu_baseline(w, gpu_pcg_iters, gpu_oc_iters, n_sample):
+    import numpy as np
+    from oracle import cpu_ref, simp_oracle as O
+
+    threads = len(os.sched_getaffinity(0))
+    KE = O.ke_hex8(w.spec.nu)
+    grid = (w.spec.nx, w.spec.ny, w.spec.nz)
+    t0 = time.perf_counter()
+    x, xp, u, comp, its = cpu_ref.simp_run(grid, KE, w.f, w.fixed, n_sample, w.spec.volfrac, w.tol,
+                                           law=w.spec.law, threads=threads)
+    dt = time.perf_counter() - t0
+    # the sample ran n_sample OC iterations and one final solve: n_sample+1 solves
+    per_pcg_iter = dt / max(its, 1)
+    est_solve_s = per_pcg_iter * gpu_pcg_iters
+    return {
+        "value": 1.0 / est_solve_s,
+        "unit": "solves/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"oracle/simp_cpu.c FP64 matrix-free Jacobi-PCG, OpenMP x{threads}, first "
+                   f"{n_sample} OC iterations + final solve of the same {grid[0]}x{grid[1]}x{grid[2]} "
+                   f"problem ({its} PCG iterations, {dt:.2f} s); per-PCG-iteration time scaled by the "
+                   f"GPU solve's {gpu_pcg_iters} PCG iterations over {gpu_oc_iters} OC iterations"),
+        "sample_seconds": dt,
+        "sample_pcg_iters": its,
+        "cpu_model": _cpu_model(),
+        "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS"),
+    }
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); independent SIMP "
+              "problems shard one per GPU, and no multi-GPU figure is measured in this build.",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from genpipeline_amd import Problem
+    from genpipeline_amd.problems import point_loads
+
+    nx, ny, nz = a.grid
+    w = point_loads(nx, ny, nz, seed=a.seed, n_loads=4, volfrac=0.3, n_iters=a.oc_iters, tol=1e-6)
+    dev = torch.device("cuda", 0)
+    P = Problem(w.spec, w.fixed, w.f, device=dev)
+    run_kw = dict(tol=w.tol, max_iter=20000, precision="fp32", warm_start=True)
+    for k in range(a.warmup):
+        rw = P.run(w.n_iters, **run_kw)
+        print(f"[bench] warmup {k}: {rw.seconds:.2f} s, {rw.pcg_iters} PCG its, "
+              f"C={rw.compliance:.6g}", file=sys.stderr, flush=True)
+    P.set_profiling(4)
+    P.ku_profile(reset=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    results = []
+    for k in range(a.steps):
+        results.append(P.run(w.n_iters, **run_kw))
+        print(f"[bench] step {k}: {results[-1].seconds:.2f} s", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ku_ms, ku_samples = P.ku_profile(reset=True)
+    P.set_profiling(0)
+    r = results[-1]
+    sp = w.spec
+    ku_bytes = 24 * sp.n_nodes + 4 * sp.n_elem
+    achieved = ku_bytes / (ku_ms * 1e-3) / 1e9
+    traffic = None
+    tf = Path(a.traffic_file)
+    if tf.exists():
+        try:
+            t = json.loads(tf.read_text())
+            if t.get("grid") == list(a.grid):
+                traffic = t.get("bytes_per_launch")
+        except (ValueError, OSError):
+            traffic = None
+    out = {
+        "metric": METRIC,
+        "value": a.steps / dt,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": dt / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"simp_{nx}x{ny}x{nz}_point_loads4_seed{a.seed}",
+            "grid": [nx, ny, nz],
+            "oc_iters": w.n_iters,
+            "pcg": "jacobi, fp32 vectors, fp64 dots, rel tol 1e-6, warm start",
+            "material": "hex8 E0=1 Emin=1e-9 nu=0.3 p=3 volfrac=0.3 rmin=1.5",
+            "parallelism": "single problem, one GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_ku<float,true> (fused PCG K.u + p.Ap)",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": ku_bytes,
+            "avg_launch_us": ku_ms * 1e3,
+            "event_samples": ku_samples,
+        },
+        "extra": {
+            "compliance": r.compliance,
+            "pcg_iters_per_solve": r.pcg_iters,
+            "bisect_steps_per_solve": r.bisect_steps,
+            "status": r.status,
+            "solve_seconds_lib": [x.seconds for x in results],
+        },
+    }
+    if not a.no_cpu:
+        print("[bench] cpu baseline sample", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(w, r.pcg_iters, w.n_iters, a.cpu_oc_iters)
+    else:
+        out["cpu_baseline"] = None
+    P.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -68,6 +68,10 @@ SIGNATURES = {
     "gp_problem_set_bcs": [P, P, I64, P],
     "gp_problem_set_preserved": [P, P],
     "gp_problem_load": [P, P, P],
+    "gp_problem_set_profiling": [P, I32],
+    "gp_problem_get_profile": [P, C.POINTER(F64), C.POINTER(I64), I32],
+    "gp_problem_set_ku_tile": [P, I32, I32, I32, I32],
+    "gp_problem_get_ku_tile": [P, I32, C.POINTER(I32), C.POINTER(I32), C.POINTER(I32), C.POINTER(I32)],
     "gp_stiffness": [P, P, P, P],
     "gp_ku": [P, P, P, P, I32, I32, P],
     "gp_diag": [P, P, P, I32, P],

@@ -16,6 +16,7 @@
 #include <cstring>
 #include <string>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "gp_internal.hpp"
@@ -132,8 +133,17 @@ struct gp_problem {
   // scalars
   DevScalars* sc = nullptr;     // [0] PCG, [1] OC/dots, [2] refinement
   DevScalars* h_sc = nullptr;   // pinned mirrors, 3 slots
-  double* partials = nullptr;
+  double* partials = nullptr;   // backing store of `parts`
   size_t n_partials = 0;
+  Parts parts{};
+  // modal K.u: coefficients projected from KE, valid when the table has the
+  // isotropic hex8 modal structure (always for GP_KE_HEX8)
+  bool modal_ok = false;
+  ModalCoef<float> mc32{};
+  ModalCoef<double> mc64{};
+  KuTile tile32{8, 32, 4};
+  KuTile tile64{16, 16, 4};
+  bool use_modal32 = true, use_modal64 = true;
   hipEvent_t ev[2] = {nullptr, nullptr};
   // PCG workspace per precision
   void* ws32 = nullptr;         // r z p Ap minv (float, 5*ndof)
@@ -148,6 +158,17 @@ struct gp_problem {
   double* xnew = nullptr;
   double* xphys_new = nullptr;
   double* ce_sum = nullptr;
+  // sampled K.u timing: events bracket every `prof_every`-th fused K.u launch
+  int prof_every = 0;
+  long long prof_counter = 0;
+  double prof_ms = 0.0;
+  long long prof_n = 0;
+  std::vector<hipEvent_t> prof_pool;
+  struct ProfSample {
+    hipEvent_t a, b;
+    int launch;  // index of the K.u launch within its solve
+  };
+  std::vector<ProfSample> prof_pending;
 };
 
 namespace {
@@ -166,12 +187,96 @@ int alloc(void** p, size_t bytes) {
     if (_r) return _r;                                         \
   } while (0)
 
-size_t partial_count(const Grid& g) {
-  size_t n = 3 * (size_t)kMaxPartials;
+const KuTile kTiles[] = {{8, 32, 1}, {16, 16, 1}, {14, 18, 1}, {6, 34, 1},
+                         {14, 34, 1}, {4, 64, 1}, {10, 26, 1}};
+
+// entries per partials region: enough for every reducing launch shape
+size_t partial_cap(const Grid& g) {
+  size_t n = (size_t)grid_blocks(g.ndof);
   n = std::max(n, (size_t)grid_blocks(g.nn));
   n = std::max(n, (size_t)grid_blocks(g.ne));
-  return n;
+  for (const KuTile& t : kTiles) n = std::max(n, (size_t)modal_blocks(g, t));
+  return n + 16;
 }
+constexpr int kRegions = 9;
+
+// Khat entry (mode mA comp qA, mode mB comp qB) of a 24x24 table, /64.
+double khat(const double* ke, int mA, int qA, int mB, int qB) {
+  static const int C[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                              {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+  auto W = [&](int c, int m) {
+    double w = 1.0;
+    for (int d = 0; d < 3; ++d)
+      if ((m >> (2 - d)) & 1) w *= (2.0 * C[c][d] - 1.0);
+    return w;
+  };
+  double s = 0.0;
+  for (int a = 0; a < 8; ++a)
+    for (int b = 0; b < 8; ++b) s += W(a, mA) * ke[(3 * a + qA) * 24 + 3 * b + qB] * W(b, mB);
+  return s / 64.0;
+}
+
+// Modes as bit masks x=4, y=2, z=1 (bit of mi, mj, mk).
+enum { MX = 4, MY = 2, MZ = 1, MXY = 6, MYZ = 3, MXZ = 5, MXYZ = 7 };
+
+// v = Khat b / 64 with the 8-coefficient isotropic form, exactly as the
+// modal kernel applies it (b, v indexed [mode mask][component]).
+void modal_apply(const ModalCoef<double>& k, const double (&b)[8][3], double (&v)[8][3]) {
+  for (auto& r : v) r[0] = r[1] = r[2] = 0.0;
+  const double tr = b[MX][0] + b[MY][1] + b[MZ][2];
+  v[MX][0] = k.An * b[MX][0] + k.Ln * tr;
+  v[MY][1] = k.An * b[MY][1] + k.Ln * tr;
+  v[MZ][2] = k.An * b[MZ][2] + k.Ln * tr;
+  v[MX][1] = v[MY][0] = k.S * (b[MX][1] + b[MY][0]);
+  v[MX][2] = v[MZ][0] = k.S * (b[MX][2] + b[MZ][0]);
+  v[MY][2] = v[MZ][1] = k.S * (b[MY][2] + b[MZ][1]);
+  v[MXY][0] = k.B1 * b[MXY][0] + k.B2 * b[MYZ][2];
+  v[MYZ][2] = k.B2 * b[MXY][0] + k.B1 * b[MYZ][2];
+  v[MXY][1] = k.B1 * b[MXY][1] + k.B2 * b[MXZ][2];
+  v[MXZ][2] = k.B2 * b[MXY][1] + k.B1 * b[MXZ][2];
+  v[MYZ][1] = k.B1 * b[MYZ][1] + k.B2 * b[MXZ][0];
+  v[MXZ][0] = k.B2 * b[MYZ][1] + k.B1 * b[MXZ][0];
+  const double s3 = b[MXY][2] + b[MYZ][0] + b[MXZ][1];
+  v[MXY][2] = k.T1 * b[MXY][2] + k.T2 * s3;
+  v[MYZ][0] = k.T1 * b[MYZ][0] + k.T2 * s3;
+  v[MXZ][1] = k.T1 * b[MXZ][1] + k.T2 * s3;
+  for (int q = 0; q < 3; ++q) v[MXYZ][q] = k.X * b[MXYZ][q];
+}
+
+// Projects KE onto the modal basis; true when the 8-coefficient form
+// reproduces every Khat entry to 1e-10 relative.
+bool modal_coefs(const double* ke, ModalCoef<double>* out) {
+  ModalCoef<double> k;
+  k.Ln = khat(ke, MX, 0, MY, 1);
+  k.An = khat(ke, MX, 0, MX, 0) - k.Ln;
+  k.S = khat(ke, MX, 1, MX, 1);
+  k.B1 = khat(ke, MXY, 0, MXY, 0);
+  k.B2 = khat(ke, MXY, 0, MYZ, 2);
+  k.T2 = khat(ke, MXY, 2, MYZ, 0);
+  k.T1 = khat(ke, MXY, 2, MXY, 2) - k.T2;
+  k.X = khat(ke, MXYZ, 0, MXYZ, 0);
+  double mx = 0.0, err = 0.0;
+  for (int mB = 0; mB < 8; ++mB)
+    for (int qB = 0; qB < 3; ++qB) {
+      double b[8][3] = {}, v[8][3];
+      b[mB][qB] = 1.0;
+      modal_apply(k, b, v);
+      for (int mA = 0; mA < 8; ++mA)
+        for (int qA = 0; qA < 3; ++qA) {
+          const double ref = khat(ke, mA, qA, mB, qB);
+          mx = std::max(mx, std::fabs(ref));
+          err = std::max(err, std::fabs(ref - v[mA][qA]));
+        }
+    }
+  *out = k;
+  return mx > 0 && err <= 1e-10 * mx;
+}
+
+// K.u on the problem's grid in precision T; returns the number of blocks
+// (= p.Ap partials written when fuse).
+template <typename T>
+int ku_dispatch(gp_problem* P, const float* E, const T* u, T* y, bool mask, bool fuse,
+                hipStream_t s);
 
 int validate_spec(const gp_spec* s) {
   if (!s) return fail(GP_ERR_ARG, "spec is NULL");
@@ -255,6 +360,23 @@ int ensure_simp_ws(gp_problem* P) {
   return GP_OK;
 }
 
+template <typename T>
+int ku_dispatch(gp_problem* P, const float* E, const T* u, T* y, bool mask, bool fuse,
+                hipStream_t s) {
+  const bool f32 = std::is_same<T, float>::value;
+  const uint8_t* fm = mask ? P->fixmask : nullptr;
+  const DevScalars* sc = &P->sc[0];
+  if (P->modal_ok && (f32 ? P->use_modal32 : P->use_modal64)) {
+    const KuTile& t = f32 ? P->tile32 : P->tile64;
+    const void* kc = f32 ? (const void*)&P->mc32 : (const void*)&P->mc64;
+    if (launch_ku_modal(P->g, t, kc, f32 ? GP_F32 : GP_F64, E, u, y, fm, sc, P->parts.A, fuse, s))
+      return modal_blocks(P->g, t);
+  }
+  const T* ke = reinterpret_cast<const T*>(f32 ? (const void*)P->ke32 : (const void*)P->ke64);
+  launch_ku<T>(P->g, ke, E, u, y, fm, sc, P->parts.A, fuse, s);
+  return gather_blocks(P->g);
+}
+
 // Plain PCG in precision T on the free DOFs. u: T*, in x0 / out solution.
 template <typename T>
 int pcg_core(gp_problem* P, const float* E, const double* f, T* u, double tol, int max_iter,
@@ -273,18 +395,40 @@ int pcg_core(gp_problem* P, const float* E, const double* f, T* u, double tol, i
   launch_diag_minv<T>(g, ke, E, P->fixmask, minv, true, s);
   if (use_x0) {
     launch_mask_fixed<T>(g, u, P->fixmask, s);
-    launch_ku<T>(g, ke, E, u, Ap, P->fixmask, nullptr, nullptr, false, s);
+    ku_dispatch<T>(P, E, u, Ap, true, false, s);
   }
-  launch_pcg_init<T>(g, f, Ap, use_x0, u, r, z, p, minv, sc, P->partials, tol, max_iter, s);
+  launch_pcg_init<T>(g, f, Ap, use_x0, u, r, z, p, minv, P->parts, sc, tol, max_iter, s);
   LAUNCHCHK();
 
   // chunked, speculative issue: one chunk always queued behind the one the
   // host is inspecting.
+  auto take_event = [&]() {
+    hipEvent_t e = nullptr;
+    if (!P->prof_pool.empty()) {
+      e = P->prof_pool.back();
+      P->prof_pool.pop_back();
+    } else if (hipEventCreate(&e) != hipSuccess) {
+      e = nullptr;
+    }
+    return e;
+  };
+  int launch_idx = 0;
   auto issue = [&](int n_it) {
-    for (int it = 0; it < n_it; ++it) {
-      launch_ku<T>(g, ke, E, p, Ap, P->fixmask, sc, P->partials, true, s);
-      launch_pcg_b<T>(g, u, r, z, p, Ap, minv, sc, P->partials, s);
-      launch_pcg_c<T>(g, p, z, sc, s);
+    for (int it = 0; it < n_it; ++it, ++launch_idx) {
+      const bool sample = P->prof_every > 0 && (P->prof_counter++ % P->prof_every) == 0;
+      hipEvent_t ea = sample ? take_event() : nullptr;
+      hipEvent_t eb = sample ? take_event() : nullptr;
+      if (ea && eb) hipEventRecord(ea, s);
+      const int nA = ku_dispatch<T>(P, E, p, Ap, true, true, s);
+      if (ea && eb) {
+        hipEventRecord(eb, s);
+        P->prof_pending.push_back({ea, eb, launch_idx});
+      } else {
+        if (ea) P->prof_pool.push_back(ea);
+        if (eb) P->prof_pool.push_back(eb);
+      }
+      launch_pcg_b<T>(g, launch_idx, nA, u, r, z, p, Ap, minv, P->parts, sc, s);
+      launch_pcg_c<T>(g, launch_idx, p, z, P->parts, sc, s);
     }
   };
   int chunk = 8;
@@ -313,6 +457,18 @@ int pcg_core(gp_problem* P, const float* E, const double* f, T* u, double tol, i
   int rc = sync_read(P, 0, s);
   if (rc) return rc;
   const DevScalars& h = P->h_sc[0];
+  // Launch m ran the kernel body iff m < iterations completed (later launches
+  // found the solve done and returned at once); only those samples count.
+  for (auto& pr : P->prof_pending) {
+    float ms = 0.f;
+    if (pr.launch < h.iter && hipEventElapsedTime(&ms, pr.a, pr.b) == hipSuccess) {
+      P->prof_ms += ms;
+      P->prof_n += 1;
+    }
+    P->prof_pool.push_back(pr.a);
+    P->prof_pool.push_back(pr.b);
+  }
+  P->prof_pending.clear();
   st->status = h.status;
   st->iters += h.iter;
   st->relres = h.bb > 0 ? std::sqrt(h.rr / h.bb) : 0.0;
@@ -331,11 +487,11 @@ int pcg_refine(gp_problem* P, const float* E, const double* f, double* u, const 
   DevScalars* sc2 = &P->sc[2];
   // ||f_free||^2
   HIPCHK(hipMemsetAsync(P->y64, 0, g.ndof * sizeof(double), s));
-  launch_residual64(g, f, P->y64, P->fixmask, P->r64, sc2, P->partials, s);
+  launch_residual64(g, f, P->y64, P->fixmask, P->r64, P->parts, sc2, s);
   LAUNCHCHK();
   rc = sync_read(P, 2, s);
   if (rc) return rc;
-  const double bb = P->h_sc[2].dot;
+  const double bb = P->h_sc[2].sum;
   if (!o->use_x0) HIPCHK(hipMemsetAsync(u, 0, g.ndof * sizeof(double), s));
   launch_mask_fixed<double>(g, u, P->fixmask, s);
   st->status = GP_OK;
@@ -348,12 +504,12 @@ int pcg_refine(gp_problem* P, const float* E, const double* f, double* u, const 
   const int max_outer = o->max_outer > 0 ? o->max_outer : 50;
   double rr = 0.0;
   for (int k = 0;; ++k) {
-    launch_ku<double>(g, P->ke64, E, u, P->y64, P->fixmask, nullptr, nullptr, false, s);
-    launch_residual64(g, f, P->y64, P->fixmask, P->r64, sc2, P->partials, s);
+    ku_dispatch<double>(P, E, u, P->y64, true, false, s);
+    launch_residual64(g, f, P->y64, P->fixmask, P->r64, P->parts, sc2, s);
     LAUNCHCHK();
     rc = sync_read(P, 2, s);
     if (rc) return rc;
-    rr = P->h_sc[2].dot;
+    rr = P->h_sc[2].sum;
     st->outer = k;
     if (rr <= tol2 || !std::isfinite(rr)) break;
     if (k >= max_outer) {
@@ -369,11 +525,11 @@ int pcg_refine(gp_problem* P, const float* E, const double* f, double* u, const 
     if (in.status == GP_BREAKDOWN) {
       st->status = GP_BREAKDOWN;
       // one more residual for the report
-      launch_ku<double>(g, P->ke64, E, u, P->y64, P->fixmask, nullptr, nullptr, false, s);
-      launch_residual64(g, f, P->y64, P->fixmask, P->r64, sc2, P->partials, s);
+      ku_dispatch<double>(P, E, u, P->y64, true, false, s);
+      launch_residual64(g, f, P->y64, P->fixmask, P->r64, P->parts, sc2, s);
       rc = sync_read(P, 2, s);
       if (rc) return rc;
-      rr = P->h_sc[2].dot;
+      rr = P->h_sc[2].sum;
       break;
     }
   }
@@ -386,10 +542,10 @@ int run_oc(gp_problem* P, const double* x, const double* dc, double volfrac, dou
            double* xphys, hipStream_t s) {
   const Grid& g = P->g;
   DevScalars* sc = &P->sc[1];
-  launch_oc_init(g, dc, P->spec.oc_variant, volfrac, P->spec.move, sc, P->partials, s);
+  launch_oc_init(g, dc, P->spec.oc_variant, volfrac, P->spec.move, P->parts, sc, s);
   const int max_steps = P->spec.oc_variant == GP_OC_GPU ? 30 : 50;
   for (int i = 0; i < max_steps; ++i)
-    launch_oc_step(g, x, dc, P->w, P->pres, P->n_pres, sc, P->partials, s);
+    launch_oc_step(g, i, x, dc, P->w, P->pres, P->n_pres, P->parts, sc, s);
   launch_oc_final(g, x, dc, P->pres, sc, xnew, s);
   launch_filter(g, P->st, P->Hs, xnew, xphys, 0, s);
   launch_set_preserved(g, P->pres, xphys, s);
@@ -412,7 +568,7 @@ int gp_footprint_bytes(const gp_spec* spec, int32_t precision, size_t* bytes) {
   const Grid g = make_grid(spec);
   const size_t nd = (size_t)g.ndof, ne = (size_t)g.ne, nn = (size_t)g.nn;
   size_t b = 576 * (8 + 4) + sizeof(Stencil) + 2 * ne * 8 /*Hs,w*/ + nn /*fixmask*/ + ne /*pres*/ +
-             nd * 8 /*f*/ + 3 * sizeof(DevScalars) + partial_count(g) * 8;
+             nd * 8 /*f*/ + 3 * sizeof(DevScalars) + kRegions * partial_cap(g) * 8;
   b += ne * (4 + 4 * 8) + 8;                                   // SIMP loop workspace
   if (precision == GP_F64) b += 5 * nd * 8;
   else b += 5 * nd * 4 + nd * (8 + 8 + 4);                     // FP32 PCG + refinement
@@ -455,8 +611,34 @@ int gp_problem_create(const gp_spec* spec, gp_problem** out) {
   CALLOC(P->fixmask, g.nn);
   CALLOC(P->f, g.ndof * sizeof(double));
   CALLOC(P->sc, 3 * sizeof(DevScalars));
-  P->n_partials = partial_count(g);
-  CALLOC(P->partials, P->n_partials * sizeof(double));
+  {
+    const size_t cap = partial_cap(g);
+    P->n_partials = kRegions * cap;
+    CALLOC(P->partials, P->n_partials * sizeof(double));
+    double* b = P->partials;
+    P->parts.cap = (int)cap;
+    P->parts.A = b;
+    P->parts.RZ[0] = b + cap;
+    P->parts.RZ[1] = b + 2 * cap;
+    P->parts.RR = b + 3 * cap;
+    P->parts.BB = b + 4 * cap;
+    P->parts.O[0] = b + 5 * cap;
+    P->parts.O[1] = b + 6 * cap;
+    P->parts.M = b + 7 * cap;
+    P->parts.S = b + 8 * cap;
+  }
+  {
+    ModalCoef<double> mc;
+    P->modal_ok = modal_coefs(P->ke64_h, &mc);
+    P->mc64 = mc;
+    P->mc32 = {(float)mc.An, (float)mc.Ln, (float)mc.S, (float)mc.B1,
+               (float)mc.B2, (float)mc.T1, (float)mc.T2, (float)mc.X};
+    // default sweep depth: enough workgroups to cover the device a few times
+    const int ntj = (g.ny + 1 + 6) / 7, ntk = (g.nz + 1 + 30) / 31;
+    int ni = (ntj * ntk * (g.nx + 1)) / 1024;
+    P->tile32.ni = std::max(2, std::min(16, ni));
+    P->tile64.ni = P->tile32.ni;
+  }
 #undef CALLOC
   if (hipHostMalloc(reinterpret_cast<void**>(&P->h_sc), 3 * sizeof(DevScalars)) != hipSuccess)
     return cleanup_fail(fail(GP_ERR_OOM, "hipHostMalloc failed"));
@@ -470,6 +652,7 @@ int gp_problem_create(const gp_spec* spec, gp_problem** out) {
   e = e != hipSuccess ? e : hipMemset(P->sc, 0, 3 * sizeof(DevScalars));
   e = e != hipSuccess ? e : hipMemset(P->fixmask, 0, g.nn);
   e = e != hipSuccess ? e : hipMemset(P->f, 0, g.ndof * sizeof(double));
+  e = e != hipSuccess ? e : hipMemset(P->partials, 0, P->n_partials * sizeof(double));
   if (e != hipSuccess) return cleanup_fail(fail(GP_ERR_HIP, "init copy: %s", hipGetErrorString(e)));
   launch_hs(g, P->st, P->Hs, nullptr);
   launch_oc_weights(g, P->st, P->Hs, nullptr, P->w, nullptr);
@@ -492,6 +675,11 @@ int gp_problem_destroy(gp_problem* P) {
   if (P->h_sc) hipHostFree(P->h_sc);
   for (auto& ev : P->ev)
     if (ev) hipEventDestroy(ev);
+  for (auto& pr : P->prof_pending) {
+    hipEventDestroy(pr.a);
+    hipEventDestroy(pr.b);
+  }
+  for (auto e : P->prof_pool) hipEventDestroy(e);
   delete P;
   return GP_OK;
 }
@@ -542,6 +730,55 @@ int gp_problem_set_preserved(gp_problem* P, const uint8_t* mask) {
   return GP_OK;
 }
 
+int gp_problem_set_profiling(gp_problem* P, int32_t every) {
+  if (!P || every < 0) return fail(GP_ERR_ARG, "bad argument");
+  P->prof_every = every;
+  P->prof_counter = 0;
+  return GP_OK;
+}
+
+int gp_problem_get_profile(gp_problem* P, double* ku_ms_sum, int64_t* ku_samples, int32_t reset) {
+  if (!P || !ku_ms_sum || !ku_samples) return fail(GP_ERR_ARG, "NULL argument");
+  *ku_ms_sum = P->prof_ms;
+  *ku_samples = P->prof_n;
+  if (reset) {
+    P->prof_ms = 0.0;
+    P->prof_n = 0;
+  }
+  return GP_OK;
+}
+
+int gp_problem_set_ku_tile(gp_problem* P, int32_t precision, int32_t tj, int32_t tk, int32_t ni) {
+  if (!P) return fail(GP_ERR_ARG, "problem is NULL");
+  if (precision != GP_F32 && precision != GP_F64) return fail(GP_ERR_ARG, "bad precision");
+  bool& use = precision == GP_F32 ? P->use_modal32 : P->use_modal64;
+  KuTile& t = precision == GP_F32 ? P->tile32 : P->tile64;
+  if (tj == 0) {
+    use = false;  // dense gather kernel
+    return GP_OK;
+  }
+  bool known = false;
+  for (const KuTile& k : kTiles) known = known || (k.tj == tj && k.tk == tk);
+  if (!known) return fail(GP_ERR_ARG, "unsupported K.u tile %dx%d", tj, tk);
+  if (ni < 1 || ni > 256) return fail(GP_ERR_ARG, "ni must be in [1, 256]");
+  t = {tj, tk, ni};
+  use = true;
+  return GP_OK;
+}
+
+int gp_problem_get_ku_tile(gp_problem* P, int32_t precision, int32_t* tj, int32_t* tk, int32_t* ni,
+                           int32_t* modal) {
+  if (!P || !tj || !tk || !ni || !modal) return fail(GP_ERR_ARG, "NULL argument");
+  const bool f32 = precision == GP_F32;
+  const KuTile& t = f32 ? P->tile32 : P->tile64;
+  const bool use = P->modal_ok && (f32 ? P->use_modal32 : P->use_modal64);
+  *tj = use ? t.tj : 0;
+  *tk = use ? t.tk : 0;
+  *ni = use ? t.ni : 0;
+  *modal = P->modal_ok ? 1 : 0;
+  return GP_OK;
+}
+
 int gp_problem_load(gp_problem* P, double* f_dev, void* stream) {
   if (!P || !f_dev) return fail(GP_ERR_ARG, "NULL argument");
   HIPCHK(hipSetDevice(P->device));
@@ -561,11 +798,10 @@ int gp_ku(gp_problem* P, const float* E, const void* u, void* y, int32_t dtype, 
           void* stream) {
   if (!P || !E || !u || !y) return fail(GP_ERR_ARG, "NULL argument");
   HIPCHK(hipSetDevice(P->device));
-  const uint8_t* fm = mask ? P->fixmask : nullptr;
   if (dtype == GP_F32)
-    launch_ku<float>(P->g, P->ke32, E, (const float*)u, (float*)y, fm, nullptr, nullptr, false, S(stream));
+    ku_dispatch<float>(P, E, (const float*)u, (float*)y, mask != 0, false, S(stream));
   else if (dtype == GP_F64)
-    launch_ku<double>(P->g, P->ke64, E, (const double*)u, (double*)y, fm, nullptr, nullptr, false, S(stream));
+    ku_dispatch<double>(P, E, (const double*)u, (double*)y, mask != 0, false, S(stream));
   else
     return fail(GP_ERR_ARG, "gp_ku: dtype must be GP_F32 or GP_F64");
   LAUNCHCHK();
@@ -617,10 +853,10 @@ int gp_sens(gp_problem* P, const double* xPhys, const void* u, int32_t dtype, do
   const gp_spec& sp = P->spec;
   if (dtype == GP_F32)
     launch_sens<float>(P->g, P->ke64, xPhys, (const float*)u, sp.law, sp.penal, sp.E0, sp.Emin, dc,
-                       ce_sum, &P->sc[1], P->partials, S(stream));
+                       ce_sum, P->parts, &P->sc[1], S(stream));
   else if (dtype == GP_F64)
     launch_sens<double>(P->g, P->ke64, xPhys, (const double*)u, sp.law, sp.penal, sp.E0, sp.Emin,
-                        dc, ce_sum, &P->sc[1], P->partials, S(stream));
+                        dc, ce_sum, P->parts, &P->sc[1], S(stream));
   else
     return fail(GP_ERR_ARG, "gp_sens: dtype must be GP_F32 or GP_F64");
   LAUNCHCHK();
@@ -655,15 +891,15 @@ int gp_compliance(gp_problem* P, const void* u, int32_t dtype, double* out, void
   if (!P || !u || !out) return fail(GP_ERR_ARG, "NULL argument");
   HIPCHK(hipSetDevice(P->device));
   if (dtype == GP_F32)
-    launch_dot<double, float>(P->f, (const float*)u, P->g.ndof, &P->sc[1], P->partials, S(stream));
+    launch_dot<double, float>(P->f, (const float*)u, P->g.ndof, P->parts, &P->sc[1], S(stream));
   else if (dtype == GP_F64)
-    launch_dot<double, double>(P->f, (const double*)u, P->g.ndof, &P->sc[1], P->partials, S(stream));
+    launch_dot<double, double>(P->f, (const double*)u, P->g.ndof, P->parts, &P->sc[1], S(stream));
   else
     return fail(GP_ERR_ARG, "gp_compliance: dtype must be GP_F32 or GP_F64");
   LAUNCHCHK();
   int rc = sync_read(P, 1, S(stream));
   if (rc) return rc;
-  *out = P->h_sc[1].dot;
+  *out = P->h_sc[1].sum;
   return GP_OK;
 }
 
@@ -713,12 +949,12 @@ int gp_simp_run(gp_problem* P, const gp_simp_opts* o, double* x, double* xPhys, 
   };
   auto compliance = [&](double* c) -> int {
     if (refine)
-      launch_dot<double, double>(P->f, u, g.ndof, &P->sc[1], P->partials, s);
+      launch_dot<double, double>(P->f, u, g.ndof, P->parts, &P->sc[1], s);
     else
-      launch_dot<double, float>(P->f, u32, g.ndof, &P->sc[1], P->partials, s);
+      launch_dot<double, float>(P->f, u32, g.ndof, P->parts, &P->sc[1], s);
     int r = sync_read(P, 1, s);
     if (r) return r;
-    *c = P->h_sc[1].dot;
+    *c = P->h_sc[1].sum;
     return GP_OK;
   };
 
@@ -741,10 +977,10 @@ int gp_simp_run(gp_problem* P, const gp_simp_opts* o, double* x, double* xPhys, 
     launch_reset_flags(&P->sc[1], s);
     if (refine)
       launch_sens<double>(g, P->ke64, xPhys, u, sp.law, sp.penal, sp.E0, sp.Emin, P->dc, nullptr,
-                          &P->sc[1], P->partials, s);
+                          P->parts, &P->sc[1], s);
     else
       launch_sens<float>(g, P->ke64, xPhys, u32, sp.law, sp.penal, sp.E0, sp.Emin, P->dc, nullptr,
-                         &P->sc[1], P->partials, s);
+                         P->parts, &P->sc[1], s);
     launch_filter(g, P->st, P->Hs, P->dc, P->dcf, 1, s);
     rc = run_oc(P, x, P->dcf, o->volfrac, P->xnew, P->xphys_new, s);
     if (rc) return rc;
@@ -838,17 +1074,19 @@ int gp_csr_pcg(const int64_t* rp, const int64_t* ci, const float* v, const float
     if (h) hipHostFree(h);
   };
   auto dot = [&](const float* a, const float* c, double* out) -> int {
-    launch_dot<float, float>(a, c, n, sc, part, s);
+    Parts pp{};
+    pp.S = part;
+    launch_dot<float, float>(a, c, n, pp, sc, s);
     HIPCHK(hipMemcpyAsync(h, sc, sizeof(DevScalars), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
-    *out = h->dot;
+    *out = h->sum;
     return GP_OK;
   };
   do {
     if ((rc = alloc((void**)&r, n * 4)) || (rc = alloc((void**)&z, n * 4)) ||
         (rc = alloc((void**)&p, n * 4)) || (rc = alloc((void**)&Ap, n * 4)) ||
         (rc = alloc((void**)&sc, sizeof(DevScalars))) ||
-        (rc = alloc((void**)&part, 3 * kMaxPartials * sizeof(double))))
+        (rc = alloc((void**)&part, kVecBlocks * sizeof(double))))
       break;
     if (hipHostMalloc((void**)&h, sizeof(DevScalars)) != hipSuccess) {
       rc = fail(GP_ERR_OOM, "hipHostMalloc failed");

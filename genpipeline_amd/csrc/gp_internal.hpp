@@ -1,15 +1,14 @@
 // Internal declarations shared by the kernel and API translation units.
 #pragma once
 #include <hip/hip_runtime.h>
-#include <hip/hip_fp16.h>
 #include <cstdint>
 #include <cstddef>
 #include "../../include/gp_simp.h"
 
 namespace gp {
 
-constexpr int kBlock = 256;          // threads per block for all 1-D kernels
-constexpr int kMaxPartials = 2048;   // max blocks of a reducing grid-stride kernel
+constexpr int kBlock = 256;          // threads per block of the 1-D kernels
+constexpr int kVecBlocks = 512;      // cap on blocks of reducing vector kernels
 constexpr int kMaxOffsets = 343;     // filter stencil cap: (2*3+1)^3, rmin <= 3
 
 // Structured hex8 grid. Node strides: sJ = nz+1, sI = (ny+1)(nz+1).
@@ -29,42 +28,83 @@ struct Stencil {
   double w[kMaxOffsets];
 };
 
-// Scalars of one PCG solve / one OC update, resident in device memory.
-struct DevScalars {
-  double rz, pAp, rr, bb, alpha, beta, tol2, rr0;
-  double l1, l2, lmid, oc_eps, oc_tol, oc_floor, volfrac, move;
-  double dot;          // generic reduction result
-  int iter, max_iter, done, status;
-  int oc_steps, oc_max, oc_done, nan_flag;
-  unsigned int cnt[8]; // last-block counters, one per reducing kernel kind
+// Modal coefficients of an isotropic hex8 KE (Khat/64, see gp_ku_modal.hip).
+template <typename T>
+struct ModalCoef {
+  T An, Ln, S, B1, B2, T1, T2, X;
 };
 
-enum CounterSlot { CNT_A = 0, CNT_B, CNT_INIT, CNT_OC, CNT_DOT, CNT_MAX, CNT_SENS, CNT_RES };
+// Workgroup shape of the modal K.u sweep: TJ x TK element columns, NI node
+// planes per workgroup.
+struct KuTile {
+  int tj, tk, ni;
+};
+
+// FP64 partial sums, one region per reduction kind. Every reduction is
+// finished by its consumers: each consuming block sums the producer's
+// partials in the same fixed order, so all blocks agree bit for bit and no
+// kernel needs device-scope fences or atomics.
+struct Parts {
+  double* A;       // p.Ap per K.u block
+  double* RZ[2];   // r.z per vector block, by iteration parity
+  double* RR;      // r.r per vector block
+  double* BB;      // b.b per vector block
+  double* O[2];    // OC volume sums per block, by bisection-step parity
+  double* M;       // OC max(-dc) per block
+  double* S;       // one-shot sums (energy, dot, residual)
+  int cap;         // entries per region
+};
+
+// Scalars of one PCG solve / OC update, resident in device memory. Fields
+// are written by block 0 of a kernel and read by later kernels only.
+struct DevScalars {
+  double bb, tol2, rr, rz, pAp;
+  double ocl1[2], ocl2[2];   // bisection interval at the start of a step, by parity
+  double lmid;               // final multiplier
+  double volfrac, move, oc_eps, oc_tol, oc_floor;
+  double sum;                // one-shot reduction result
+  int iter, max_iter, done, status;
+  int oc_steps, oc_max, oc_done, nan_flag;
+};
+
+__device__ inline double wave_sum64(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
 
 // ---------------------------------------------------------------- launchers
 // All launchers are asynchronous on `st`.
 template <typename T>
 void launch_ku(const Grid& g, const T* ke, const float* E, const T* u, T* y,
-               const uint8_t* fixmask, DevScalars* sc, double* partials, bool fuse_pap,
+               const uint8_t* fixmask, const DevScalars* sc, double* partA, bool fuse_pap,
                hipStream_t st);
+int gather_blocks(const Grid& g);
+bool launch_ku_modal(const Grid& g, const KuTile& t, const void* kc, int prec, const float* E,
+                     const void* u, void* y, const uint8_t* fixmask, const DevScalars* sc,
+                     double* partA, bool fuse, hipStream_t st);
+int modal_blocks(const Grid& g, const KuTile& t);
+
 template <typename T>
 void launch_diag_minv(const Grid& g, const T* ke, const float* E, const uint8_t* fixmask,
                       T* diag_or_minv, bool invert, hipStream_t st);
 template <typename T>
 void launch_mask_fixed(const Grid& g, T* x, const uint8_t* fixmask, hipStream_t st);
+int vec_blocks(const Grid& g);
 template <typename T>
 void launch_pcg_init(const Grid& g, const double* f, const T* Ax, bool use_x0, T* x, T* r,
-                     T* z, T* p, const T* minv, DevScalars* sc, double* partials, double tol,
+                     T* z, T* p, const T* minv, Parts parts, DevScalars* sc, double tol,
                      int max_iter, hipStream_t st);
 template <typename T>
-void launch_pcg_b(const Grid& g, T* x, T* r, T* z, const T* p, const T* Ap, const T* minv,
-                  DevScalars* sc, double* partials, hipStream_t st);
+void launch_pcg_b(const Grid& g, int it, int nA, T* x, T* r, T* z, const T* p, const T* Ap,
+                  const T* minv, Parts parts, DevScalars* sc, hipStream_t st);
 template <typename T>
-void launch_pcg_c(const Grid& g, T* p, const T* z, DevScalars* sc, hipStream_t st);
+void launch_pcg_c(const Grid& g, int it, T* p, const T* z, Parts parts, DevScalars* sc,
+                  hipStream_t st);
 
 // FP64 refinement helpers
 void launch_residual64(const Grid& g, const double* f, const double* Ku, const uint8_t* fixmask,
-                       double* r, DevScalars* sc, double* partials, hipStream_t st);
+                       double* r, Parts parts, DevScalars* sc, hipStream_t st);
 template <typename T>
 void launch_add_correction(const Grid& g, double* u, const T* d, hipStream_t st);
 void launch_cast_f64_to_f32(const double* a, float* b, long long n, hipStream_t st);
@@ -75,23 +115,24 @@ void launch_stiffness(const Grid& g, const double* xPhys, float* E, int law, dou
 template <typename T>
 void launch_sens(const Grid& g, const double* ke64, const double* xPhys, const T* u, int law,
                  double penal, double E0, double Emin, double* dc, double* ce_sum,
-                 DevScalars* sc, double* partials, hipStream_t st);
+                 Parts parts, DevScalars* sc, hipStream_t st);
 // `s` points to a Stencil in device memory.
 void launch_filter(const Grid& g, const Stencil* s, const double* Hs, const double* in,
                    double* out, int mode, hipStream_t st);
 void launch_hs(const Grid& g, const Stencil* s, double* Hs, hipStream_t st);
 void launch_oc_weights(const Grid& g, const Stencil* s, const double* Hs, const uint8_t* pres,
                        double* w, hipStream_t st);
+int oc_blocks(const Grid& g);
 void launch_oc_init(const Grid& g, const double* dc, int variant, double volfrac, double move,
-                    DevScalars* sc, double* partials, hipStream_t st);
-void launch_oc_step(const Grid& g, const double* x, const double* dc, const double* w,
-                    const uint8_t* pres, double n_pres, DevScalars* sc, double* partials,
+                    Parts parts, DevScalars* sc, hipStream_t st);
+void launch_oc_step(const Grid& g, int s, const double* x, const double* dc, const double* w,
+                    const uint8_t* pres, double n_pres, Parts parts, DevScalars* sc,
                     hipStream_t st);
 void launch_oc_final(const Grid& g, const double* x, const double* dc, const uint8_t* pres,
-                     DevScalars* sc, double* xnew, hipStream_t st);
+                     const DevScalars* sc, double* xnew, hipStream_t st);
 void launch_set_preserved(const Grid& g, const uint8_t* pres, double* xPhys, hipStream_t st);
 template <typename TA, typename T>
-void launch_dot(const TA* a, const T* b, long long n, DevScalars* sc, double* partials,
+void launch_dot(const TA* a, const T* b, long long n, Parts parts, DevScalars* sc,
                 hipStream_t st);
 void launch_reset_flags(DevScalars* sc, hipStream_t st);
 void launch_zero_list(float* v, const int64_t* idx, long long n, hipStream_t st);
@@ -110,10 +151,6 @@ inline int grid_blocks(long long n, int per_thread = 1) {
   long long b = (n + (long long)kBlock * per_thread - 1) / ((long long)kBlock * per_thread);
   if (b < 1) b = 1;
   return (int)b;
-}
-inline int reduce_blocks(long long n) {
-  int b = grid_blocks(n);
-  return b > kMaxPartials ? kMaxPartials : b;
 }
 
 }  // namespace gp

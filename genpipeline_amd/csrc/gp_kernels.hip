@@ -1,12 +1,16 @@
-// HIP kernels of the SIMP inner loop for gfx950 (CDNA4, wave64).
+// HIP kernels of the SIMP inner loop for gfx950 (CDNA4, wave64): the dense
+// gather K.u (any 24x24 table), Jacobi-PCG vector kernels, FP64 refinement,
+// material law, sensitivity, density filter, OC bisection and the
+// generic-table kernels of the reference kernel API.
 //
 // Grid numbering follows genpipeline/topology/simp_solver.py:251-282 exactly;
 // nothing is gathered through an edof table on the structured path -- every
 // index is derived arithmetically from (i,j,k).
 //
-// Reductions are deterministic: each block reduces in a fixed butterfly order,
-// writes one FP64 partial, and the last block to finish (device-scope counter)
-// sums the partials in index order. No FP64 atomics touch a result.
+// Reductions: a producing kernel writes one FP64 partial per block (fixed
+// butterfly order inside the block); each block of the consuming kernel sums
+// the producer's partials in index order. All blocks therefore agree bit for
+// bit on every scalar, and no kernel uses device-scope fences or atomics.
 #include "gp_internal.hpp"
 
 namespace gp {
@@ -19,67 +23,57 @@ __device__ constexpr int CZ[8] = {0, 0, 0, 0, 1, 1, 1, 1};
 __device__ constexpr int LOC[8] = {0, 4, 3, 7, 1, 5, 2, 6};
 
 // ------------------------------------------------------------ reductions
-__device__ inline double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
 __device__ inline double wave_max(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
   return v;
 }
 
-// Sums NV values over the block; the result is valid in thread 0.
-template <int NV>
-__device__ inline void block_sum(double (&v)[NV]) {
-  __shared__ double sh[NV][kBlock / 64];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-  for (int q = 0; q < NV; ++q) {
-    double s = wave_sum(v[q]);
-    if (lane == 0) sh[q][w] = s;
-  }
+// Block sum (blockDim == kBlock); result in thread 0.
+__device__ inline double block_sum0(double v) {
+  __shared__ double sh[kBlock / 64];
+  v = wave_sum64(v);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
   __syncthreads();
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int q = 0; q < NV; ++q) {
-      double s = 0.0;
-      for (int i = 0; i < kBlock / 64; ++i) s += sh[q][i];
-      v[q] = s;
-    }
-  }
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < kBlock / 64; ++i) s += sh[i];
+  __syncthreads();
+  return s;
 }
 
-// Thread 0 of every block has written its partials; returns true in every
-// thread of the last block to arrive. Release/acquire through device-scope
-// fences so the last block sees all partials.
-__device__ inline bool last_block(unsigned int* counter) {
-  __shared__ int s_last;
-  if (threadIdx.x == 0) {
-    __threadfence();
-    unsigned int prev = atomicAdd(counter, 1u);
-    s_last = (prev == gridDim.x - 1);
-  }
+// Sum of p[0..n) in a fixed order, returned to every thread of the block.
+__device__ inline double reduce_all(const double* __restrict__ p, int n) {
+  __shared__ double sh[kBlock / 64];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += kBlock) s += p[i];
+  s = wave_sum64(s);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
   __syncthreads();
-  const bool last = s_last != 0;
-  if (last) __threadfence();
-  return last;
+  double t = 0.0;
+  for (int i = 0; i < kBlock / 64; ++i) t += sh[i];
+  __syncthreads();
+  return t;
 }
 
-// Sums `n` partials (NV interleaved streams, stride kMaxPartials) in index
-// order; result valid in thread 0.
-template <int NV>
-__device__ inline void sum_partials(const double* partials, int n, double (&out)[NV]) {
-  // Each thread sums a fixed strided subset, then the block tree combines:
-  // the order depends only on n, never on timing.
-#pragma unroll
-  for (int q = 0; q < NV; ++q) {
-    double s = 0.0;
-    for (int i = threadIdx.x; i < n; i += kBlock) s += partials[q * kMaxPartials + i];
-    out[q] = s;
-  }
-  block_sum<NV>(out);
+__device__ inline double reduce_max_all(const double* __restrict__ p, int n) {
+  __shared__ double sh[kBlock / 64];
+  double s = -INFINITY;
+  for (int i = threadIdx.x; i < n; i += kBlock) s = fmax(s, p[i]);
+  s = wave_max(s);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  double t = -INFINITY;
+  for (int i = 0; i < kBlock / 64; ++i) t = fmax(t, sh[i]);
+  __syncthreads();
+  return t;
+}
+
+// One block: *out = sum of p[0..n).
+__global__ __launch_bounds__(kBlock) void k_sum_fin(const double* __restrict__ p, int n,
+                                                   double* out) {
+  const double s = reduce_all(p, n);
+  if (threadIdx.x == 0) *out = s;
 }
 
 // --------------------------------------------------------------- helpers
@@ -100,19 +94,16 @@ __device__ inline void st3(T* __restrict__ a, long long n, T x, T y, T z) {
   a[3 * n + 2] = z;
 }
 
-// --------------------------------------------------------------- K . u
-// One thread per node. The node's 27-node neighbourhood is held in
-// registers; each of its (up to) 8 elements contributes E_e * KE[3L+r, :] u_e,
-// with L the node's corner index in that element. KE is read with uniform
-// (scalar-cache) loads because every index is a compile-time constant.
-// FUSE: the PCG form -- returns early once the solve is done and reduces
-// p.Ap into alpha = rz / pAp through the last block.
+// ------------------------------------------------------- K . u (gather)
+// Any 24x24 table: one thread per node, the 27-node neighbourhood in
+// registers, each of the (up to) 8 elements adding E_e * KE[3L+r, :] u_e,
+// with L the node's corner index in that element.
 template <typename T, bool FUSE>
 __global__ __launch_bounds__(kBlock) void k_ku(Grid g, const T* __restrict__ ke,
                                               const float* __restrict__ E,
                                               const T* __restrict__ u, T* __restrict__ y,
                                               const uint8_t* __restrict__ fixmask,
-                                              DevScalars* sc, double* partials) {
+                                              const DevScalars* sc, double* partA) {
   if (FUSE && sc->done) return;
   const int n = blockIdx.x * kBlock + threadIdx.x;
   const bool valid = n < g.nn;
@@ -175,42 +166,28 @@ __global__ __launch_bounds__(kBlock) void k_ku(Grid g, const T* __restrict__ ke,
     if (fm & 4) acc2 = 0;
   }
   if (valid) st3(y, n, acc0, acc1, acc2);
-
   if (FUSE) {
-    double v[1] = {valid ? (double)un[13][0] * (double)acc0 + (double)un[13][1] * (double)acc1 +
-                               (double)un[13][2] * (double)acc2
-                         : 0.0};
-    block_sum<1>(v);
-    if (threadIdx.x == 0) partials[blockIdx.x] = v[0];
-    if (last_block(&sc->cnt[CNT_A])) {
-      double s[1];
-      sum_partials<1>(partials, gridDim.x, s);
-      if (threadIdx.x == 0) {
-        const double pAp = s[0];
-        sc->pAp = pAp;
-        if (!(pAp > 0.0)) {
-          sc->status = GP_BREAKDOWN;
-          sc->done = 1;
-        } else {
-          sc->alpha = sc->rz / pAp;
-        }
-        sc->cnt[CNT_A] = 0;
-      }
-    }
+    const double d = valid ? (double)un[13][0] * (double)acc0 + (double)un[13][1] * (double)acc1 +
+                                 (double)un[13][2] * (double)acc2
+                           : 0.0;
+    const double s = block_sum0(d);
+    if (threadIdx.x == 0) partA[blockIdx.x] = s;
   }
 }
 
+int gather_blocks(const Grid& g) { return grid_blocks(g.nn); }
+
 template <typename T>
 void launch_ku(const Grid& g, const T* ke, const float* E, const T* u, T* y,
-               const uint8_t* fixmask, DevScalars* sc, double* partials, bool fuse_pap,
+               const uint8_t* fixmask, const DevScalars* sc, double* partA, bool fuse_pap,
                hipStream_t st) {
-  const int nb = grid_blocks(g.nn);
+  const int nb = gather_blocks(g);
   if (fuse_pap)
     hipLaunchKernelGGL((k_ku<T, true>), dim3(nb), dim3(kBlock), 0, st, g, ke, E, u, y, fixmask,
-                       sc, partials);
+                       sc, partA);
   else
     hipLaunchKernelGGL((k_ku<T, false>), dim3(nb), dim3(kBlock), 0, st, g, ke, E, u, y, fixmask,
-                       sc, partials);
+                       sc, partA);
 }
 
 // diag(K) per DOF; with invert: Jacobi M^-1 (0 on fixed DOFs and on DOFs
@@ -274,76 +251,98 @@ void launch_mask_fixed(const Grid& g, T* x, const uint8_t* fixmask, hipStream_t 
 // ------------------------------------------------------------ PCG vectors
 // Fixed DOFs are exactly the DOFs with M^-1 == 0 (set by k_diag), so the
 // vector kernels need no separate mask: r = 0 there, hence z = p = 0 there.
+// Vector kernels run a fixed grid of vec_blocks() blocks, grid-striding.
+int vec_blocks(const Grid& g) {
+  const int b = grid_blocks(3ll * g.nn);
+  return b < kVecBlocks ? b : kVecBlocks;
+}
 
-// r = b - A x0 (or b), z = M^-1 r, p = z; partials rz, rr, bb.
+// r = b - A x0 (or b), z = M^-1 r, p = z; partials rz -> RZ[0], rr, bb.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_pcg_init(int nn, const double* __restrict__ f,
                                                     const T* __restrict__ Ax, int use_x0,
                                                     T* __restrict__ x, T* __restrict__ r,
                                                     T* __restrict__ z, T* __restrict__ p,
-                                                    const T* __restrict__ minv, DevScalars* sc,
-                                                    double* partials, double tol,
-                                                    int max_iter) {
-  double acc[3] = {0.0, 0.0, 0.0};  // rz, rr, bb
+                                                    const T* __restrict__ minv, Parts parts) {
+  double rz = 0.0, rr = 0.0, bb = 0.0;
   for (long long d = blockIdx.x * (long long)kBlock + threadIdx.x; d < 3ll * nn;
        d += (long long)gridDim.x * kBlock) {
     const T mi = minv[d];
     const bool freed = mi != T(0);
     const double b = freed ? f[d] : 0.0;
-    T xd = (use_x0 && freed) ? x[d] : T(0);
+    const T xd = (use_x0 && freed) ? x[d] : T(0);
     x[d] = xd;
     const T rd = freed ? (T)(b - (use_x0 ? (double)Ax[d] : 0.0)) : T(0);
     const T zd = mi * rd;
     r[d] = rd;
     z[d] = zd;
     p[d] = zd;
-    acc[0] += (double)rd * (double)zd;
-    acc[1] += (double)rd * (double)rd;
-    acc[2] += b * b;
+    rz += (double)rd * (double)zd;
+    rr += (double)rd * (double)rd;
+    bb += b * b;
   }
-  block_sum<3>(acc);
+  rz = block_sum0(rz);
+  rr = block_sum0(rr);
+  bb = block_sum0(bb);
   if (threadIdx.x == 0) {
-    partials[blockIdx.x] = acc[0];
-    partials[kMaxPartials + blockIdx.x] = acc[1];
-    partials[2 * kMaxPartials + blockIdx.x] = acc[2];
+    parts.RZ[0][blockIdx.x] = rz;
+    parts.RR[blockIdx.x] = rr;
+    parts.BB[blockIdx.x] = bb;
   }
-  if (last_block(&sc->cnt[CNT_INIT])) {
-    double s[3];
-    sum_partials<3>(partials, gridDim.x, s);
-    if (threadIdx.x == 0) {
-      sc->rz = s[0];
-      sc->rr = s[1];
-      sc->rr0 = s[1];
-      sc->bb = s[2];
-      sc->tol2 = tol * tol * s[2];
-      sc->max_iter = max_iter;
-      sc->iter = 0;
-      sc->status = GP_OK;
-      sc->done = (s[2] == 0.0 || s[1] <= sc->tol2) ? 1 : 0;
-      sc->cnt[CNT_INIT] = 0;
-    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_pcg_init_fin(Parts parts, int nV, double tol,
+                                                        int max_iter, DevScalars* sc) {
+  const double bb = reduce_all(parts.BB, nV);
+  const double rr = reduce_all(parts.RR, nV);
+  const double rz = reduce_all(parts.RZ[0], nV);
+  if (threadIdx.x == 0) {
+    sc->bb = bb;
+    sc->rr = rr;
+    sc->rz = rz;
+    sc->tol2 = tol * tol * bb;
+    sc->max_iter = max_iter;
+    sc->iter = 0;
+    sc->status = GP_OK;
+    sc->done = (bb == 0.0 || rr <= tol * tol * bb) ? 1 : 0;
   }
 }
 
 template <typename T>
 void launch_pcg_init(const Grid& g, const double* f, const T* Ax, bool use_x0, T* x, T* r, T* z,
-                     T* p, const T* minv, DevScalars* sc, double* partials, double tol,
-                     int max_iter, hipStream_t st) {
-  hipLaunchKernelGGL((k_pcg_init<T>), dim3(reduce_blocks(3ll * g.nn)), dim3(kBlock), 0, st, g.nn,
-                     f, Ax, use_x0 ? 1 : 0, x, r, z, p, minv, sc, partials, tol, max_iter);
+                     T* p, const T* minv, Parts parts, DevScalars* sc, double tol, int max_iter,
+                     hipStream_t st) {
+  const int nV = vec_blocks(g);
+  hipLaunchKernelGGL((k_pcg_init<T>), dim3(nV), dim3(kBlock), 0, st, g.nn, f, Ax, use_x0 ? 1 : 0,
+                     x, r, z, p, minv, parts);
+  hipLaunchKernelGGL(k_pcg_init_fin, dim3(1), dim3(kBlock), 0, st, parts, nV, tol, max_iter, sc);
 }
 
-// x += a p; r -= a Ap; z = M^-1 r; then beta = rz_new / rz and the
-// convergence test ||r||^2 <= tol^2 ||b||^2 in the last block.
+// Iteration `it`: alpha = rz / pAp from the partials of the previous
+// kernels; x += a p; r -= a Ap; z = M^-1 r; partials rz' -> RZ[(it+1)&1], rr.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_pcg_b(int nn, T* __restrict__ x, T* __restrict__ r,
-                                                 T* __restrict__ z, const T* __restrict__ p,
+__global__ __launch_bounds__(kBlock) void k_pcg_b(int it, int nn, int nA, T* __restrict__ x,
+                                                 T* __restrict__ r, T* __restrict__ z,
+                                                 const T* __restrict__ p,
                                                  const T* __restrict__ Ap,
-                                                 const T* __restrict__ minv, DevScalars* sc,
-                                                 double* partials) {
+                                                 const T* __restrict__ minv, Parts parts,
+                                                 DevScalars* sc) {
   if (sc->done) return;
-  const T alpha = (T)sc->alpha;
-  double acc[2] = {0.0, 0.0};  // rz, rr
+  const int q = it & 1;
+  const int nV = gridDim.x;
+  const double pAp = reduce_all(parts.A, nA);
+  const double rz = reduce_all(parts.RZ[q], nV);
+  if (!(pAp > 0.0)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      sc->pAp = pAp;
+      sc->status = GP_BREAKDOWN;
+      sc->iter = it;
+      sc->done = 1;
+    }
+    return;
+  }
+  const T alpha = (T)(rz / pAp);
+  double rzn = 0.0, rr = 0.0;
   for (long long d = blockIdx.x * (long long)kBlock + threadIdx.x; d < 3ll * nn;
        d += (long long)gridDim.x * kBlock) {
     const T pd = p[d];
@@ -352,86 +351,82 @@ __global__ __launch_bounds__(kBlock) void k_pcg_b(int nn, T* __restrict__ x, T* 
     r[d] = rd;
     const T zd = minv[d] * rd;
     z[d] = zd;
-    acc[0] += (double)rd * (double)zd;
-    acc[1] += (double)rd * (double)rd;
+    rzn += (double)rd * (double)zd;
+    rr += (double)rd * (double)rd;
   }
-  block_sum<2>(acc);
+  rzn = block_sum0(rzn);
+  rr = block_sum0(rr);
   if (threadIdx.x == 0) {
-    partials[blockIdx.x] = acc[0];
-    partials[kMaxPartials + blockIdx.x] = acc[1];
+    parts.RZ[q ^ 1][blockIdx.x] = rzn;
+    parts.RR[blockIdx.x] = rr;
   }
-  if (last_block(&sc->cnt[CNT_B])) {
-    double s[2];
-    sum_partials<2>(partials, gridDim.x, s);
-    if (threadIdx.x == 0) {
-      const int it = sc->iter + 1;
-      sc->iter = it;
-      sc->rr = s[1];
-      if (s[1] <= sc->tol2) {
-        sc->done = 1;
-      } else if (it >= sc->max_iter) {
-        sc->done = 1;
-        sc->status = GP_NOT_CONVERGED;
-      } else {
-        sc->beta = s[0] / sc->rz;
-        sc->rz = s[0];
-      }
-      sc->cnt[CNT_B] = 0;
+}
+
+template <typename T>
+void launch_pcg_b(const Grid& g, int it, int nA, T* x, T* r, T* z, const T* p, const T* Ap,
+                  const T* minv, Parts parts, DevScalars* sc, hipStream_t st) {
+  hipLaunchKernelGGL((k_pcg_b<T>), dim3(vec_blocks(g)), dim3(kBlock), 0, st, it, g.nn, nA, x, r,
+                     z, p, Ap, minv, parts, sc);
+}
+
+// Iteration `it` epilogue: convergence test ||r||^2 <= tol^2 ||b||^2 and the
+// iteration cap, then p = z + beta p with beta = rz' / rz.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pcg_c(int it, int nn, T* __restrict__ p,
+                                                 const T* __restrict__ z, Parts parts,
+                                                 DevScalars* sc) {
+  if (sc->done) return;
+  const int q = it & 1;
+  const int nV = gridDim.x;
+  const double rzn = reduce_all(parts.RZ[q ^ 1], nV);
+  const double rr = reduce_all(parts.RR, nV);
+  const double rz = reduce_all(parts.RZ[q], nV);
+  const bool conv = rr <= sc->tol2;
+  const bool cap = it + 1 >= sc->max_iter;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->iter = it + 1;
+    sc->rr = rr;
+    sc->rz = rzn;
+    if (conv || cap) {
+      sc->status = conv ? GP_OK : GP_NOT_CONVERGED;
+      sc->done = 1;
     }
   }
-}
-
-template <typename T>
-void launch_pcg_b(const Grid& g, T* x, T* r, T* z, const T* p, const T* Ap, const T* minv,
-                  DevScalars* sc, double* partials, hipStream_t st) {
-  hipLaunchKernelGGL((k_pcg_b<T>), dim3(reduce_blocks(3ll * g.nn)), dim3(kBlock), 0, st, g.nn, x,
-                     r, z, p, Ap, minv, sc, partials);
-}
-
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_pcg_c(int nn, T* __restrict__ p,
-                                                 const T* __restrict__ z, DevScalars* sc) {
-  if (sc->done) return;
-  const T beta = (T)sc->beta;
+  if (conv || cap) return;
+  const T beta = (T)(rzn / rz);
   for (long long d = blockIdx.x * (long long)kBlock + threadIdx.x; d < 3ll * nn;
        d += (long long)gridDim.x * kBlock)
     p[d] = z[d] + beta * p[d];
 }
 template <typename T>
-void launch_pcg_c(const Grid& g, T* p, const T* z, DevScalars* sc, hipStream_t st) {
-  hipLaunchKernelGGL((k_pcg_c<T>), dim3(grid_blocks(3ll * g.nn)), dim3(kBlock), 0, st, g.nn, p,
-                     z, sc);
+void launch_pcg_c(const Grid& g, int it, T* p, const T* z, Parts parts, DevScalars* sc,
+                  hipStream_t st) {
+  hipLaunchKernelGGL((k_pcg_c<T>), dim3(vec_blocks(g)), dim3(kBlock), 0, st, it, g.nn, p, z, parts,
+                     sc);
 }
 
 // ------------------------------------------------------- FP64 refinement
 __global__ __launch_bounds__(kBlock) void k_residual64(int nn, const double* __restrict__ f,
                                                       const double* __restrict__ Ku,
                                                       const uint8_t* __restrict__ fixmask,
-                                                      double* __restrict__ r, DevScalars* sc,
-                                                      double* partials) {
-  double acc[1] = {0.0};
+                                                      double* __restrict__ r, double* partS) {
+  double acc = 0.0;
   for (long long d = blockIdx.x * (long long)kBlock + threadIdx.x; d < 3ll * nn;
        d += (long long)gridDim.x * kBlock) {
     const bool fixed = (fixmask[d / 3] >> (d % 3)) & 1;
     const double rd = fixed ? 0.0 : f[d] - Ku[d];
     r[d] = rd;
-    acc[0] += rd * rd;
+    acc += rd * rd;
   }
-  block_sum<1>(acc);
-  if (threadIdx.x == 0) partials[blockIdx.x] = acc[0];
-  if (last_block(&sc->cnt[CNT_RES])) {
-    double s[1];
-    sum_partials<1>(partials, gridDim.x, s);
-    if (threadIdx.x == 0) {
-      sc->dot = s[0];
-      sc->cnt[CNT_RES] = 0;
-    }
-  }
+  acc = block_sum0(acc);
+  if (threadIdx.x == 0) partS[blockIdx.x] = acc;
 }
 void launch_residual64(const Grid& g, const double* f, const double* Ku, const uint8_t* fixmask,
-                       double* r, DevScalars* sc, double* partials, hipStream_t st) {
-  hipLaunchKernelGGL(k_residual64, dim3(reduce_blocks(3ll * g.nn)), dim3(kBlock), 0, st, g.nn, f,
-                     Ku, fixmask, r, sc, partials);
+                       double* r, Parts parts, DevScalars* sc, hipStream_t st) {
+  const int nb = vec_blocks(g);
+  hipLaunchKernelGGL(k_residual64, dim3(nb), dim3(kBlock), 0, st, g.nn, f, Ku, fixmask, r,
+                     parts.S);
+  hipLaunchKernelGGL(k_sum_fin, dim3(1), dim3(kBlock), 0, st, parts.S, nb, &sc->sum);
 }
 
 template <typename T>
@@ -488,8 +483,7 @@ __global__ __launch_bounds__(kBlock) void k_sens(Grid g, const double* __restric
                                                 const double* __restrict__ xPhys,
                                                 const T* __restrict__ u, int law, double p,
                                                 double E0, double Emin, double* __restrict__ dc,
-                                                double* ce_sum, DevScalars* sc,
-                                                double* partials) {
+                                                int want_sum, double* partS, DevScalars* sc) {
   const int e = blockIdx.x * kBlock + threadIdx.x;
   const bool valid = e < g.ne;
   const int ec = valid ? e : 0;
@@ -518,27 +512,20 @@ __global__ __launch_bounds__(kBlock) void k_sens(Grid g, const double* __restric
     dc[e] = dcv;
     if (dcv != dcv) sc->nan_flag = 1;
   }
-  if (ce_sum != nullptr) {
-    double v[1] = {valid ? law_E(x, law, p, E0, Emin) * ce : 0.0};
-    block_sum<1>(v);
-    if (threadIdx.x == 0) partials[blockIdx.x] = v[0];
-    if (last_block(&sc->cnt[CNT_SENS])) {
-      double s[1];
-      sum_partials<1>(partials, gridDim.x, s);
-      if (threadIdx.x == 0) {
-        *ce_sum = s[0];
-        sc->cnt[CNT_SENS] = 0;
-      }
-    }
+  if (want_sum) {
+    const double s = block_sum0(valid ? law_E(x, law, p, E0, Emin) * ce : 0.0);
+    if (threadIdx.x == 0) partS[blockIdx.x] = s;
   }
 }
 template <typename T>
 void launch_sens(const Grid& g, const double* ke64, const double* xPhys, const T* u, int law,
-                 double penal, double E0, double Emin, double* dc, double* ce_sum,
-                 DevScalars* sc, double* partials, hipStream_t st) {
-  const int nb = grid_blocks(g.ne);  // partials buffer holds >= nb entries
+                 double penal, double E0, double Emin, double* dc, double* ce_sum, Parts parts,
+                 DevScalars* sc, hipStream_t st) {
+  const int nb = grid_blocks(g.ne);  // parts.cap >= nb
   hipLaunchKernelGGL((k_sens<T>), dim3(nb), dim3(kBlock), 0, st, g, ke64, xPhys, u, law, penal,
-                     E0, Emin, dc, ce_sum, sc, partials);
+                     E0, Emin, dc, ce_sum != nullptr ? 1 : 0, parts.S, sc);
+  if (ce_sum != nullptr)
+    hipLaunchKernelGGL(k_sum_fin, dim3(1), dim3(kBlock), 0, st, parts.S, nb, ce_sum);
 }
 
 // ---------------------------------------------------------------- filter
@@ -618,44 +605,52 @@ void launch_oc_weights(const Grid& g, const Stencil* s, const double* Hs, const 
 }
 
 // ------------------------------------------------------------------- OC
-// l1 = 0; l2 = 1e9 (cpu) or 2 max(max(-dc), 1) (gpu variant).
-__global__ __launch_bounds__(kBlock) void k_oc_init(int ne, const double* __restrict__ dc,
-                                                   int variant, double volfrac, double move,
-                                                   DevScalars* sc, double* partials) {
+int oc_blocks(const Grid& g) {
+  const int b = grid_blocks(g.ne);
+  return b < kVecBlocks ? b : kVecBlocks;
+}
+
+// per-block max(-dc) (gpu variant: l2 = 2 max(max(-dc), 1))
+__global__ __launch_bounds__(kBlock) void k_oc_max(int ne, const double* __restrict__ dc,
+                                                  double* partM) {
   double m = -INFINITY;
-  if (variant == GP_OC_GPU)
-    for (int e = blockIdx.x * kBlock + threadIdx.x; e < ne; e += gridDim.x * kBlock)
-      m = fmax(m, -dc[e]);
+  for (int e = blockIdx.x * kBlock + threadIdx.x; e < ne; e += gridDim.x * kBlock)
+    m = fmax(m, -dc[e]);
   m = wave_max(m);
   __shared__ double sh[kBlock / 64];
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int i = 0; i < kBlock / 64; ++i) m = fmax(m, sh[i]);
-    partials[blockIdx.x] = m;
-  }
-  if (last_block(&sc->cnt[CNT_MAX])) {
-    if (threadIdx.x == 0) {
-      double mm = -INFINITY;
-      for (int b = 0; b < (int)gridDim.x; ++b) mm = fmax(mm, partials[b]);
-      sc->l1 = 0.0;
-      sc->l2 = (variant == GP_OC_GPU) ? fmax(mm, 1.0) * 2.0 : 1e9;
-      sc->oc_eps = (variant == GP_OC_GPU) ? 1e-15 : 1e-12;
-      sc->oc_tol = (variant == GP_OC_GPU) ? 1e-5 : 1e-4;
-      sc->oc_floor = (variant == GP_OC_GPU) ? 1e-15 : 1e-12;
-      sc->oc_max = (variant == GP_OC_GPU) ? 30 : 50;
-      sc->volfrac = volfrac;
-      sc->move = move;
-      sc->oc_steps = 0;
-      sc->oc_done = 0;
-      sc->cnt[CNT_MAX] = 0;
-    }
+    partM[blockIdx.x] = m;
   }
 }
+
+__global__ __launch_bounds__(kBlock) void k_oc_init_fin(Parts parts, int nb, int variant,
+                                                       double volfrac, double move,
+                                                       DevScalars* sc) {
+  const double mm = variant == GP_OC_GPU ? reduce_max_all(parts.M, nb) : 0.0;
+  if (threadIdx.x == 0) {
+    sc->ocl1[0] = 0.0;
+    sc->ocl2[0] = (variant == GP_OC_GPU) ? fmax(mm, 1.0) * 2.0 : 1e9;
+    sc->oc_eps = (variant == GP_OC_GPU) ? 1e-15 : 1e-12;
+    sc->oc_tol = (variant == GP_OC_GPU) ? 1e-5 : 1e-4;
+    sc->oc_floor = (variant == GP_OC_GPU) ? 1e-15 : 1e-12;
+    sc->oc_max = (variant == GP_OC_GPU) ? 30 : 50;
+    sc->volfrac = volfrac;
+    sc->move = move;
+    sc->oc_steps = 0;
+    sc->oc_done = 0;
+  }
+}
+
 void launch_oc_init(const Grid& g, const double* dc, int variant, double volfrac, double move,
-                    DevScalars* sc, double* partials, hipStream_t st) {
-  hipLaunchKernelGGL(k_oc_init, dim3(reduce_blocks(g.ne)), dim3(kBlock), 0, st, g.ne, dc, variant,
-                     volfrac, move, sc, partials);
+                    Parts parts, DevScalars* sc, hipStream_t st) {
+  const int nb = oc_blocks(g);
+  if (variant == GP_OC_GPU)
+    hipLaunchKernelGGL(k_oc_max, dim3(nb), dim3(kBlock), 0, st, g.ne, dc, parts.M);
+  hipLaunchKernelGGL(k_oc_init_fin, dim3(1), dim3(kBlock), 0, st, parts, nb, variant, volfrac,
+                     move, sc);
 }
 
 __device__ inline double oc_xnew(double x, double dc, double lmid, double floor_, double move) {
@@ -664,55 +659,66 @@ __device__ inline double oc_xnew(double x, double dc, double lmid, double floor_
   return fmin(fmax(v, x - move), x + move);
 }
 
-// One bisection step (simp_solver.py:230-248): the stop test on (l1,l2)
-// precedes the volume branch, exactly as in the reference loop body.
-__global__ __launch_bounds__(kBlock) void k_oc_step(int ne, const double* __restrict__ x,
+// Bisection step s (simp_solver.py:230-248). The interval at the start of
+// step s follows from step s-1's interval and its volume sum; the stop test
+// on (l1,l2) precedes the volume evaluation, as in the reference loop body.
+__global__ __launch_bounds__(kBlock) void k_oc_step(int s, int ne, const double* __restrict__ x,
                                                    const double* __restrict__ dc,
                                                    const double* __restrict__ w,
                                                    const uint8_t* __restrict__ pres,
-                                                   double n_pres, DevScalars* sc,
-                                                   double* partials) {
+                                                   double n_pres, Parts parts, DevScalars* sc) {
   if (sc->oc_done) return;
-  const double l1 = sc->l1, l2 = sc->l2;
+  const int nb = gridDim.x;
+  double l1, l2;
+  if (s == 0) {
+    l1 = sc->ocl1[0];
+    l2 = sc->ocl2[0];
+  } else {
+    const int pq = (s - 1) & 1;
+    const double pl1 = sc->ocl1[pq], pl2 = sc->ocl2[pq];
+    const double plmid = 0.5 * (pl2 + pl1);
+    const double mean = (reduce_all(parts.O[pq], nb) + n_pres) / (double)ne;
+    if (mean > sc->volfrac) {
+      l1 = plmid;
+      l2 = pl2;
+    } else {
+      l1 = pl1;
+      l2 = plmid;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      sc->ocl1[s & 1] = l1;
+      sc->ocl2[s & 1] = l2;
+    }
+  }
   const double lmid = 0.5 * (l2 + l1);
   if ((l2 - l1) / (l1 + l2 + sc->oc_eps) < sc->oc_tol) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       sc->lmid = lmid;
-      sc->oc_steps += 1;
+      sc->oc_steps = s + 1;
       sc->oc_done = 1;
     }
     return;
   }
+  if (s + 1 >= sc->oc_max && blockIdx.x == 0 && threadIdx.x == 0) {
+    // last permitted step: its lmid is final (the branch after it is unused)
+    sc->lmid = lmid;
+    sc->oc_steps = s + 1;
+    sc->oc_done = 1;
+  }
   const double fl = sc->oc_floor, mv = sc->move;
-  double acc[1] = {0.0};
+  double acc = 0.0;
   for (int e = blockIdx.x * kBlock + threadIdx.x; e < ne; e += gridDim.x * kBlock) {
     const double xn = (pres != nullptr && pres[e]) ? 1.0 : oc_xnew(x[e], dc[e], lmid, fl, mv);
-    acc[0] += w[e] * xn;
+    acc += w[e] * xn;
   }
-  block_sum<1>(acc);
-  if (threadIdx.x == 0) partials[blockIdx.x] = acc[0];
-  if (last_block(&sc->cnt[CNT_OC])) {
-    double s[1];
-    sum_partials<1>(partials, gridDim.x, s);
-    if (threadIdx.x == 0) {
-      const double mean = (s[0] + n_pres) / (double)ne;
-      if (mean > sc->volfrac)
-        sc->l1 = lmid;
-      else
-        sc->l2 = lmid;
-      sc->lmid = lmid;
-      const int st = sc->oc_steps + 1;
-      sc->oc_steps = st;
-      if (st >= sc->oc_max) sc->oc_done = 1;
-      sc->cnt[CNT_OC] = 0;
-    }
-  }
+  acc = block_sum0(acc);
+  if (threadIdx.x == 0) parts.O[s & 1][blockIdx.x] = acc;
 }
-void launch_oc_step(const Grid& g, const double* x, const double* dc, const double* w,
-                    const uint8_t* pres, double n_pres, DevScalars* sc, double* partials,
+void launch_oc_step(const Grid& g, int s, const double* x, const double* dc, const double* w,
+                    const uint8_t* pres, double n_pres, Parts parts, DevScalars* sc,
                     hipStream_t st) {
-  hipLaunchKernelGGL(k_oc_step, dim3(reduce_blocks(g.ne)), dim3(kBlock), 0, st, g.ne, x, dc, w,
-                     pres, n_pres, sc, partials);
+  hipLaunchKernelGGL(k_oc_step, dim3(oc_blocks(g)), dim3(kBlock), 0, st, s, g.ne, x, dc, w, pres,
+                     n_pres, parts, sc);
 }
 
 __global__ void k_oc_final(int ne, const double* __restrict__ x, const double* __restrict__ dc,
@@ -720,10 +726,11 @@ __global__ void k_oc_final(int ne, const double* __restrict__ x, const double* _
                            double* __restrict__ xnew) {
   const int e = blockIdx.x * kBlock + threadIdx.x;
   if (e >= ne) return;
-  xnew[e] = (pres != nullptr && pres[e]) ? 1.0 : oc_xnew(x[e], dc[e], sc->lmid, sc->oc_floor, sc->move);
+  xnew[e] = (pres != nullptr && pres[e]) ? 1.0
+                                         : oc_xnew(x[e], dc[e], sc->lmid, sc->oc_floor, sc->move);
 }
 void launch_oc_final(const Grid& g, const double* x, const double* dc, const uint8_t* pres,
-                     DevScalars* sc, double* xnew, hipStream_t st) {
+                     const DevScalars* sc, double* xnew, hipStream_t st) {
   hipLaunchKernelGGL(k_oc_final, dim3(grid_blocks(g.ne)), dim3(kBlock), 0, st, g.ne, x, dc, pres,
                      sc, xnew);
 }
@@ -741,28 +748,26 @@ void launch_set_preserved(const Grid& g, const uint8_t* pres, double* xPhys, hip
 // ------------------------------------------------------------ dot (FP64)
 template <typename TA, typename T>
 __global__ __launch_bounds__(kBlock) void k_dot(long long n, const TA* __restrict__ a,
-                                               const T* __restrict__ b, DevScalars* sc,
-                                               double* partials) {
-  double acc[1] = {0.0};
+                                               const T* __restrict__ b, double* partS) {
+  double acc = 0.0;
   for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n;
        i += (long long)gridDim.x * kBlock)
-    acc[0] += (double)a[i] * (double)b[i];
-  block_sum<1>(acc);
-  if (threadIdx.x == 0) partials[blockIdx.x] = acc[0];
-  if (last_block(&sc->cnt[CNT_DOT])) {
-    double s[1];
-    sum_partials<1>(partials, gridDim.x, s);
-    if (threadIdx.x == 0) {
-      sc->dot = s[0];
-      sc->cnt[CNT_DOT] = 0;
-    }
-  }
+    acc += (double)a[i] * (double)b[i];
+  acc = block_sum0(acc);
+  if (threadIdx.x == 0) partS[blockIdx.x] = acc;
 }
 template <typename TA, typename T>
-void launch_dot(const TA* a, const T* b, long long n, DevScalars* sc, double* partials,
+void launch_dot(const TA* a, const T* b, long long n, Parts parts, DevScalars* sc,
                 hipStream_t st) {
-  hipLaunchKernelGGL((k_dot<TA, T>), dim3(reduce_blocks(n)), dim3(kBlock), 0, st, n, a, b, sc,
-                     partials);
+  int nb = grid_blocks(n);
+  nb = nb < kVecBlocks ? nb : kVecBlocks;
+  hipLaunchKernelGGL((k_dot<TA, T>), dim3(nb), dim3(kBlock), 0, st, n, a, b, parts.S);
+  hipLaunchKernelGGL(k_sum_fin, dim3(1), dim3(kBlock), 0, st, parts.S, nb, &sc->sum);
+}
+
+__global__ void k_reset_flags(DevScalars* sc) { sc->nan_flag = 0; }
+void launch_reset_flags(DevScalars* sc, hipStream_t st) {
+  hipLaunchKernelGGL(k_reset_flags, dim3(1), dim3(1), 0, st, sc);
 }
 
 // ------------------------------------------------- generic-table kernels
@@ -835,11 +840,6 @@ void launch_csr_spmv(const int64_t* rp, const int64_t* ci, const float* v, const
   hipLaunchKernelGGL(k_csr_spmv, dim3(grid_blocks(n)), dim3(kBlock), 0, st, rp, ci, v, x, y, n);
 }
 
-__global__ void k_reset_flags(DevScalars* sc) { sc->nan_flag = 0; }
-void launch_reset_flags(DevScalars* sc, hipStream_t st) {
-  hipLaunchKernelGGL(k_reset_flags, dim3(1), dim3(1), 0, st, sc);
-}
-
 // ---------------------------------------------------- CSR PCG helpers
 __global__ void k_zero_list(float* v, const int64_t* __restrict__ idx, long long n) {
   const long long i = blockIdx.x * (long long)kBlock + threadIdx.x;
@@ -865,9 +865,9 @@ void launch_vec(float* y, const float* x, const float* d, float a, long long n, 
 
 // ------------------------------------------------ explicit instantiations
 template void launch_ku<float>(const Grid&, const float*, const float*, const float*, float*,
-                               const uint8_t*, DevScalars*, double*, bool, hipStream_t);
+                               const uint8_t*, const DevScalars*, double*, bool, hipStream_t);
 template void launch_ku<double>(const Grid&, const double*, const float*, const double*, double*,
-                                const uint8_t*, DevScalars*, double*, bool, hipStream_t);
+                                const uint8_t*, const DevScalars*, double*, bool, hipStream_t);
 template void launch_diag_minv<float>(const Grid&, const float*, const float*, const uint8_t*,
                                       float*, bool, hipStream_t);
 template void launch_diag_minv<double>(const Grid&, const double*, const float*, const uint8_t*,
@@ -875,31 +875,33 @@ template void launch_diag_minv<double>(const Grid&, const double*, const float*,
 template void launch_mask_fixed<float>(const Grid&, float*, const uint8_t*, hipStream_t);
 template void launch_mask_fixed<double>(const Grid&, double*, const uint8_t*, hipStream_t);
 template void launch_pcg_init<float>(const Grid&, const double*, const float*, bool, float*,
-                                     float*, float*, float*, const float*, DevScalars*, double*,
+                                     float*, float*, float*, const float*, Parts, DevScalars*,
                                      double, int, hipStream_t);
 template void launch_pcg_init<double>(const Grid&, const double*, const double*, bool, double*,
-                                      double*, double*, double*, const double*, DevScalars*,
-                                      double*, double, int, hipStream_t);
-template void launch_pcg_b<float>(const Grid&, float*, float*, float*, const float*, const float*,
-                                  const float*, DevScalars*, double*, hipStream_t);
-template void launch_pcg_b<double>(const Grid&, double*, double*, double*, const double*,
-                                   const double*, const double*, DevScalars*, double*,
+                                      double*, double*, double*, const double*, Parts,
+                                      DevScalars*, double, int, hipStream_t);
+template void launch_pcg_b<float>(const Grid&, int, int, float*, float*, float*, const float*,
+                                  const float*, const float*, Parts, DevScalars*, hipStream_t);
+template void launch_pcg_b<double>(const Grid&, int, int, double*, double*, double*,
+                                   const double*, const double*, const double*, Parts,
+                                   DevScalars*, hipStream_t);
+template void launch_pcg_c<float>(const Grid&, int, float*, const float*, Parts, DevScalars*,
+                                  hipStream_t);
+template void launch_pcg_c<double>(const Grid&, int, double*, const double*, Parts, DevScalars*,
                                    hipStream_t);
-template void launch_pcg_c<float>(const Grid&, float*, const float*, DevScalars*, hipStream_t);
-template void launch_pcg_c<double>(const Grid&, double*, const double*, DevScalars*, hipStream_t);
 template void launch_add_correction<float>(const Grid&, double*, const float*, hipStream_t);
 template void launch_add_correction<double>(const Grid&, double*, const double*, hipStream_t);
 template void launch_sens<float>(const Grid&, const double*, const double*, const float*, int,
-                                 double, double, double, double*, double*, DevScalars*, double*,
+                                 double, double, double, double*, double*, Parts, DevScalars*,
                                  hipStream_t);
 template void launch_sens<double>(const Grid&, const double*, const double*, const double*, int,
-                                  double, double, double, double*, double*, DevScalars*, double*,
+                                  double, double, double, double*, double*, Parts, DevScalars*,
                                   hipStream_t);
-template void launch_dot<double, float>(const double*, const float*, long long, DevScalars*,
-                                       double*, hipStream_t);
-template void launch_dot<double, double>(const double*, const double*, long long, DevScalars*,
-                                        double*, hipStream_t);
-template void launch_dot<float, float>(const float*, const float*, long long, DevScalars*, double*,
+template void launch_dot<double, float>(const double*, const float*, long long, Parts,
+                                        DevScalars*, hipStream_t);
+template void launch_dot<double, double>(const double*, const double*, long long, Parts,
+                                         DevScalars*, hipStream_t);
+template void launch_dot<float, float>(const float*, const float*, long long, Parts, DevScalars*,
                                       hipStream_t);
 
 }  // namespace gp

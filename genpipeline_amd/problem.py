@@ -179,6 +179,30 @@ class Problem:
         with torch.cuda.device(self.device):
             L.call("gp_problem_set_preserved", self._h, arg)
 
+    def set_profiling(self, every: int):
+        """Bracket every `every`-th fused K.u launch of PCG with HIP events."""
+        L.call("gp_problem_set_profiling", self._h, int(every))
+
+    def ku_profile(self, reset=True):
+        """(mean K.u launch milliseconds, samples) since the last reset."""
+        ms = C.c_double(0)
+        n = C.c_int64(0)
+        L.call("gp_problem_get_profile", self._h, C.byref(ms), C.byref(n), 1 if reset else 0)
+        return (ms.value / n.value if n.value else float("nan")), int(n.value)
+
+    KU_TILES = [(8, 32), (16, 16), (14, 18), (6, 34), (14, 34), (4, 64), (10, 26)]
+
+    def set_ku_tile(self, tj: int, tk: int = 0, ni: int = 1, precision: str = "fp32"):
+        """Modal K.u workgroup shape (tj x tk element columns, ni planes);
+        tj = 0 selects the dense gather kernel."""
+        L.call("gp_problem_set_ku_tile", self._h, PRECISIONS[precision], int(tj), int(tk), int(ni))
+
+    def ku_tile(self, precision: str = "fp32"):
+        """(tj, tk, ni, modal_ok) currently used for K.u."""
+        o = [C.c_int32() for _ in range(4)]
+        L.call("gp_problem_get_ku_tile", self._h, PRECISIONS[precision], *[C.byref(x) for x in o])
+        return tuple(x.value for x in o)
+
     # ------------------------------------------------------------ helpers
     def _f64(self, a, n):
         t = torch.as_tensor(a, dtype=torch.float64, device=self.device).contiguous()

@@ -129,6 +129,24 @@ int gp_problem_set_bcs(gp_problem* p, const int64_t* fixed_dofs, int64_t n_fixed
                        const double* f_host);
 /* Preserved (non-design) elements: host uint8 per element or NULL.         */
 int gp_problem_set_preserved(gp_problem* p, const uint8_t* mask_host);
+/* Sampled kernel timing: with every > 0, HIP events bracket every every-th
+ * fused K.u launch of the PCG loop on the caller's stream (0 = off).
+ * get_profile returns the summed milliseconds and the sample count of the
+ * launches that executed (early exits after convergence are dropped).
+ * No counterpart in the reference; measurement hook for benchmarks.        */
+int gp_problem_set_profiling(gp_problem* p, int32_t every);
+int gp_problem_get_profile(gp_problem* p, double* ku_ms_sum, int64_t* ku_samples,
+                           int32_t reset);
+/* Workgroup shape of the modal K.u sweep for one precision (GP_F32/GP_F64):
+ * tj x tk element columns (one of 8x32 16x16 14x18 6x34 14x34 4x64 10x26),
+ * ni node planes per workgroup; tj = 0 selects the dense gather kernel.
+ * Used by the autotuner. No counterpart in the reference.                */
+int gp_problem_set_ku_tile(gp_problem* p, int32_t precision, int32_t tj, int32_t tk,
+                           int32_t ni);
+/* Current shape (tj = 0: gather kernel) and whether KE has the isotropic
+ * hex8 modal structure the sweep kernel needs (modal = 1).                 */
+int gp_problem_get_ku_tile(gp_problem* p, int32_t precision, int32_t* tj, int32_t* tk,
+                           int32_t* ni, int32_t* modal);
 /* Copy of the device load vector into a float64 device buffer (n_dof).      */
 int gp_problem_load(gp_problem* p, double* f_dev, void* stream);
 

@@ -58,6 +58,34 @@ def test_ku_fp32_matches_oracle(gpu, grid):
     assert np.abs(y - ref).max() <= 2e-6 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("tile", [(8, 32, 1), (16, 16, 3), (14, 18, 2), (6, 34, 5), (14, 34, 4),
+                                  (4, 64, 7), (10, 26, 16), (0, 0, 0)])
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_ku_every_tile_matches_oracle(gpu, tile, prec):
+    """Modal sweep for each workgroup shape (tj=0: dense gather) on ragged grids."""
+    KE = O.ke_hex8(0.3)
+    for grid in [(5, 7, 6), (13, 4, 17), (2, 33, 9)]:
+        spec, fixed, f, x, rng = _problem(*grid, seed=21)
+        edof = O.edof_matrix(*grid)
+        E = O.stiffness(x, "emin", 3.0).astype(np.float32)
+        dt = np.float32 if prec == "fp32" else np.float64
+        u = rng.standard_normal(spec.n_dof).astype(dt)
+        ref = O.ku_matfree(KE, E.astype(np.float64), u.astype(np.float64), edof, fixed)
+        with Problem(spec, fixed, f, device=gpu) as P:
+            P.set_ku_tile(*tile, precision=prec)
+            tj, tk, ni, modal = P.ku_tile(prec)
+            assert modal == 1 and (tj, tk, ni) == tile
+            y = P.ku(torch.from_numpy(E), torch.from_numpy(u)).cpu().numpy().astype(np.float64)
+        tol = 2e-6 if prec == "fp32" else 1e-12
+        assert np.abs(y - ref).max() <= tol * np.abs(ref).max(), (grid, tile)
+
+
+def test_custom_table_uses_gather(gpu):
+    spec, fixed, f, x, rng = _problem(4, 3, 2, seed=2)
+    with Problem(spec, fixed, f, ke=rng.standard_normal((24, 24)), device=gpu) as P:
+        assert P.ku_tile()[3] == 0 and P.ku_tile()[0] == 0
+
+
 def test_ku_custom_table(gpu):
     """A caller-supplied (asymmetric) table is applied as y_e = KE u_e."""
     grid = (4, 3, 2)
