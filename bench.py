@@ -108,6 +108,8 @@ def main():
         rw = P.run(w.n_iters, **run_kw)
         print(f"[bench] warmup {k}: {rw.seconds:.2f} s, {rw.pcg_iters} PCG its, "
               f"C={rw.compliance:.6g}", file=sys.stderr, flush=True)
+    ku_tile = P.ku_tile("fp32")
+    print(f"[bench] K.u tile (tj, tk, ni, modal) = {ku_tile}", file=sys.stderr, flush=True)
     P.set_profiling(4)
     P.ku_profile(reset=True)
     torch.cuda.synchronize()
@@ -156,7 +158,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_ku<float,true> (fused PCG K.u + p.Ap)",
+            "kernel": "k_cg_modal<float> (one Jacobi-PCG iteration: K.z + vector updates + dots)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -172,6 +174,7 @@ def main():
             "bisect_steps_per_solve": r.bisect_steps,
             "status": r.status,
             "solve_seconds_lib": [x.seconds for x in results],
+            "ku_tile_fp32": list(ku_tile),
         },
     }
     if not a.no_cpu:

@@ -63,6 +63,7 @@ I32, I64, F64, SZ = C.c_int32, C.c_int64, C.c_double, C.c_size_t
 # name -> argtypes (restype is int for every entry point except the two below)
 SIGNATURES = {
     "gp_footprint_bytes": [C.POINTER(GpSpec), I32, C.POINTER(SZ)],
+    "gp_hex8_ke": [F64, P],
     "gp_problem_create": [C.POINTER(GpSpec), C.POINTER(P)],
     "gp_problem_destroy": [P],
     "gp_problem_set_bcs": [P, P, I64, P],
@@ -81,6 +82,9 @@ SIGNATURES = {
     "gp_oc": [P, P, P, F64, P, P, C.POINTER(GpOcStats), P],
     "gp_compliance": [P, P, I32, C.POINTER(F64), P],
     "gp_simp_run": [P, C.POINTER(GpSimpOpts), P, P, P, C.POINTER(GpSimpStats), P],
+    "gp_batch_create": [P, I32, I32, I32, C.POINTER(P)],
+    "gp_batch_destroy": [P],
+    "gp_batch_simp_run": [P, C.POINTER(GpSimpOpts), P, P, P, P, P, P],
     "gp_generic_sens": [P, P, P, P, I64, F64, P, P],
     "gp_generic_spmv": [P, P, P, P, I64, I64, F64, P, P],
     "gp_csr_pcg": [P, P, P, P, P, P, P, I64, I64, I32, F64, C.POINTER(GpPcgStats), P],
@@ -131,6 +135,14 @@ def check(rc: int):
 def call(name: str, *args):
     """Call an entry point and raise GpError on a non-zero status."""
     check(getattr(load(), name)(*args))
+
+
+def hex8_ke(nu: float = 0.3):
+    """Exact hex8 element stiffness (24x24 float64 numpy array, E = 1)."""
+    import numpy as np
+    out = np.zeros((24, 24))
+    call("gp_hex8_ke", C.c_double(nu), out.ctypes.data_as(C.c_void_p))
+    return out
 
 
 def exported_symbols() -> list[str]:

@@ -146,8 +146,8 @@ struct gp_problem {
   bool use_modal32 = true, use_modal64 = true;
   hipEvent_t ev[2] = {nullptr, nullptr};
   // PCG workspace per precision
-  void* ws32 = nullptr;         // r z p Ap minv (float, 5*ndof)
-  void* ws64 = nullptr;         // r z p Ap minv (double, 5*ndof)
+  void* ws32 = nullptr;         // PCG vectors (float, 8*ndof), layout in pcg_core
+  void* ws64 = nullptr;         // PCG vectors (double, 8*ndof)
   double* r64 = nullptr;        // refinement residual
   double* y64 = nullptr;        // refinement K u
   float* d32 = nullptr;         // refinement correction
@@ -198,7 +198,7 @@ size_t partial_cap(const Grid& g) {
   for (const KuTile& t : kTiles) n = std::max(n, (size_t)modal_blocks(g, t));
   return n + 16;
 }
-constexpr int kRegions = 9;
+constexpr int kRegions = 15;
 
 // Khat entry (mode mA comp qA, mode mB comp qB) of a 24x24 table, /64.
 double khat(const double* ke, int mA, int qA, int mB, int qB) {
@@ -334,9 +334,9 @@ int sync_read(gp_problem* P, int slot, hipStream_t s) {
 int ensure_pcg_ws(gp_problem* P, int prec) {
   const size_t nd = (size_t)P->g.ndof;
   if (prec == GP_F64) {
-    if (!P->ws64) ALLOC(P->ws64, 5 * nd * sizeof(double));
+    if (!P->ws64) ALLOC(P->ws64, 8 * nd * sizeof(double));
   } else {
-    if (!P->ws32) ALLOC(P->ws32, 5 * nd * sizeof(float));
+    if (!P->ws32) ALLOC(P->ws32, 8 * nd * sizeof(float));
   }
   return GP_OK;
 }
@@ -366,10 +366,11 @@ int ku_dispatch(gp_problem* P, const float* E, const T* u, T* y, bool mask, bool
   const bool f32 = std::is_same<T, float>::value;
   const uint8_t* fm = mask ? P->fixmask : nullptr;
   const DevScalars* sc = &P->sc[0];
-  if (P->modal_ok && (f32 ? P->use_modal32 : P->use_modal64)) {
+  if (!fuse && P->modal_ok && (f32 ? P->use_modal32 : P->use_modal64)) {
     const KuTile& t = f32 ? P->tile32 : P->tile64;
     const void* kc = f32 ? (const void*)&P->mc32 : (const void*)&P->mc64;
-    if (launch_ku_modal(P->g, t, kc, f32 ? GP_F32 : GP_F64, E, u, y, fm, sc, P->parts.A, fuse, s))
+    if (launch_ku_modal(P->g, t, kc, f32 ? GP_F32 : GP_F64, E, u, nullptr, y, nullptr, fm,
+                        &P->sc[0], P->parts, 0, 0, 0, s))
       return modal_blocks(P->g, t);
   }
   const T* ke = reinterpret_cast<const T*>(f32 ? (const void*)P->ke32 : (const void*)P->ke64);
@@ -389,15 +390,29 @@ int pcg_core(gp_problem* P, const float* E, const double* f, T* u, double tol, i
   T* p = base + 2 * nd;
   T* Ap = base + 3 * nd;
   T* minv = base + 4 * nd;
+  const bool modal = P->modal_ok && (std::is_same<T, float>::value ? P->use_modal32 : P->use_modal64);
+  const KuTile tile = std::is_same<T, float>::value ? P->tile32 : P->tile64;
+  const void* kc = std::is_same<T, float>::value ? (const void*)&P->mc32 : (const void*)&P->mc64;
+  const int prec = std::is_same<T, float>::value ? GP_F32 : GP_F64;
   const T* ke = reinterpret_cast<const T*>(std::is_same<T, float>::value ? (const void*)P->ke32
                                                                           : (const void*)P->ke64);
   DevScalars* sc = &P->sc[0];
+  // single-kernel (Chronopoulos-Gear) iteration buffers on the modal path:
+  // r, w, s ping-pong by parity; p in place
+  T* rb[2] = {base, base + nd};
+  T* wb[2] = {base + 2 * nd, base + 3 * nd};
+  T* sb[2] = {base + 5 * nd, base + 6 * nd};
+  T* pc = base + 7 * nd;
   launch_diag_minv<T>(g, ke, E, P->fixmask, minv, true, s);
   if (use_x0) {
     launch_mask_fixed<T>(g, u, P->fixmask, s);
-    ku_dispatch<T>(P, E, u, Ap, true, false, s);
+    ku_dispatch<T>(P, E, u, modal ? wb[0] : Ap, true, false, s);
   }
-  launch_pcg_init<T>(g, f, Ap, use_x0, u, r, z, p, minv, P->parts, sc, tol, max_iter, s);
+  if (modal)
+    launch_cg_init<T>(g, f, wb[0], use_x0, u, rb[1], sb[0], wb[1], pc, minv, P->parts, sc, tol,
+                      max_iter, s);
+  else
+    launch_pcg_init<T>(g, f, Ap, use_x0, u, r, z, p, minv, P->parts, sc, tol, max_iter, s);
   LAUNCHCHK();
 
   // chunked, speculative issue: one chunk always queued behind the one the
@@ -419,7 +434,15 @@ int pcg_core(gp_problem* P, const float* E, const double* f, T* u, double tol, i
       hipEvent_t ea = sample ? take_event() : nullptr;
       hipEvent_t eb = sample ? take_event() : nullptr;
       if (ea && eb) hipEventRecord(ea, s);
-      const int nA = ku_dispatch<T>(P, E, p, Ap, true, true, s);
+      if (modal) {
+        const int k = launch_idx, c = k & 1, o = c ^ 1;
+        launch_cg_modal(g, tile, kc, prec, E, rb[o], wb[o], sb[c], minv, rb[c], wb[c], sb[o], u,
+                        pc, P->fixmask, sc, P->parts, k, s);
+      } else {
+        const int nA = ku_dispatch<T>(P, E, p, Ap, true, true, s);
+        launch_pcg_b<T>(g, launch_idx, nA, u, r, z, p, Ap, minv, P->parts, sc, s);
+        launch_pcg_c<T>(g, launch_idx, p, z, P->parts, sc, s);
+      }
       if (ea && eb) {
         hipEventRecord(eb, s);
         P->prof_pending.push_back({ea, eb, launch_idx});
@@ -427,8 +450,6 @@ int pcg_core(gp_problem* P, const float* E, const double* f, T* u, double tol, i
         if (ea) P->prof_pool.push_back(ea);
         if (eb) P->prof_pool.push_back(eb);
       }
-      launch_pcg_b<T>(g, launch_idx, nA, u, r, z, p, Ap, minv, P->parts, sc, s);
-      launch_pcg_c<T>(g, launch_idx, p, z, P->parts, sc, s);
     }
   };
   int chunk = 8;
@@ -461,7 +482,8 @@ int pcg_core(gp_problem* P, const float* E, const double* f, T* u, double tol, i
   // found the solve done and returned at once); only those samples count.
   for (auto& pr : P->prof_pending) {
     float ms = 0.f;
-    if (pr.launch < h.iter && hipEventElapsedTime(&ms, pr.a, pr.b) == hipSuccess) {
+    const bool ran = modal ? pr.launch <= h.iter : pr.launch < h.iter;
+    if (ran && hipEventElapsedTime(&ms, pr.a, pr.b) == hipSuccess) {
       P->prof_ms += ms;
       P->prof_n += 1;
     }
@@ -559,6 +581,13 @@ int run_oc(gp_problem* P, const double* x, const double* dc, double volfrac, dou
 extern "C" {
 
 const char* gp_last_error(void) { return g_err.c_str(); }
+
+int gp_hex8_ke(double nu, double* ke_out) {
+  if (!ke_out) return fail(GP_ERR_ARG, "ke_out is NULL");
+  if (!(nu > -1.0 && nu < 0.5)) return fail(GP_ERR_ARG, "nu out of range");
+  hex8_ke(nu, ke_out);
+  return GP_OK;
+}
 int gp_version(void) { return 100; }
 
 int gp_footprint_bytes(const gp_spec* spec, int32_t precision, size_t* bytes) {
@@ -570,8 +599,8 @@ int gp_footprint_bytes(const gp_spec* spec, int32_t precision, size_t* bytes) {
   size_t b = 576 * (8 + 4) + sizeof(Stencil) + 2 * ne * 8 /*Hs,w*/ + nn /*fixmask*/ + ne /*pres*/ +
              nd * 8 /*f*/ + 3 * sizeof(DevScalars) + kRegions * partial_cap(g) * 8;
   b += ne * (4 + 4 * 8) + 8;                                   // SIMP loop workspace
-  if (precision == GP_F64) b += 5 * nd * 8;
-  else b += 5 * nd * 4 + nd * (8 + 8 + 4);                     // FP32 PCG + refinement
+  if (precision == GP_F64) b += 8 * nd * 8;
+  else b += 8 * nd * 4 + nd * (8 + 8 + 4);                     // FP32 PCG + refinement
   *bytes = b;
   return GP_OK;
 }
@@ -626,6 +655,11 @@ int gp_problem_create(const gp_spec* spec, gp_problem** out) {
     P->parts.O[1] = b + 6 * cap;
     P->parts.M = b + 7 * cap;
     P->parts.S = b + 8 * cap;
+    for (int q = 0; q < 2; ++q) {
+      P->parts.G[q] = b + (9 + q) * cap;
+      P->parts.D[q] = b + (11 + q) * cap;
+      P->parts.R[q] = b + (13 + q) * cap;
+    }
   }
   {
     ModalCoef<double> mc;
@@ -1133,6 +1167,253 @@ int gp_csr_pcg(const int64_t* rp, const int64_t* ci, const float* v, const float
   } while (false);
   cleanup();
   return rc;
+}
+
+}  // extern "C"
+
+// ================================================================ batches
+struct gp_batch {
+  std::vector<gp_problem*> probs;
+  int device = 0;
+  KuTile tile{8, 32, 1};
+  std::vector<BatchProb> h_probs;
+  BatchProb* d_probs = nullptr;
+  int* d_off = nullptr;
+  BatchLaunch bl{};
+  BatchStatus* d_status = nullptr;
+  BatchStatus* h_status = nullptr;  // pinned, 2 slots of n
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  int oc_max_steps = 50;
+};
+
+namespace {
+
+int batch_wait_all_done(gp_batch* B, hipStream_t s, int max_iter, int* iters_out) {
+  const int n = (int)B->probs.size();
+  int chunk = 8, issued = 0, slot = 0, k = 0;
+  auto issue = [&](int cnt) {
+    for (int i = 0; i < cnt; ++i, ++k) batch_cg(B->bl, k, s);
+  };
+  auto snap = [&](int sl) -> int {
+    batch_status(B->bl, B->d_status + sl * n, s);
+    HIPCHK(hipMemcpyAsync(B->h_status + sl * n, B->d_status + sl * n, n * sizeof(BatchStatus),
+                          hipMemcpyDeviceToHost, s));
+    HIPCHK(hipEventRecord(B->ev[sl], s));
+    return GP_OK;
+  };
+  int rc = snap(0);
+  if (rc) return rc;
+  issue(chunk);
+  issued += chunk;
+  rc = snap(1);
+  if (rc) return rc;
+  LAUNCHCHK();
+  while (true) {
+    HIPCHK(hipEventSynchronize(B->ev[slot]));
+    bool all = true;
+    for (int b = 0; b < n; ++b) all = all && B->h_status[slot * n + b].done;
+    if (all || issued >= max_iter + 2 * chunk + 2) break;
+    chunk = std::min(chunk * 2, 64);
+    issue(chunk);
+    issued += chunk;
+    rc = snap(slot);
+    if (rc) return rc;
+    LAUNCHCHK();
+    slot ^= 1;
+  }
+  HIPCHK(hipStreamSynchronize(s));
+  rc = snap(0);
+  if (rc) return rc;
+  HIPCHK(hipEventSynchronize(B->ev[0]));
+  for (int b = 0; b < n; ++b) iters_out[b] = B->h_status[b].iter;
+  return GP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gp_batch_create(gp_problem* const* probs, int32_t n, int32_t tj, int32_t tk, gp_batch** out) {
+  if (!probs || n < 1 || !out) return fail(GP_ERR_ARG, "bad argument");
+  *out = nullptr;
+  bool known = false;
+  for (const KuTile& t : kTiles) known = known || (t.tj == tj && t.tk == tk);
+  if (!known) return fail(GP_ERR_ARG, "unsupported K.u tile %dx%d", tj, tk);
+  for (int b = 0; b < n; ++b) {
+    if (!probs[b]) return fail(GP_ERR_ARG, "problem %d is NULL", b);
+    if (!probs[b]->modal_ok) return fail(GP_ERR_ARG, "problem %d: KE is not an isotropic hex8 table", b);
+    if (!probs[b]->has_bcs) return fail(GP_ERR_STATE, "problem %d: boundary conditions not set", b);
+    if (probs[b]->device != probs[0]->device) return fail(GP_ERR_ARG, "problems on different devices");
+  }
+  HIPCHK(hipSetDevice(probs[0]->device));
+  gp_batch* B = new gp_batch();
+  B->probs.assign(probs, probs + n);
+  B->device = probs[0]->device;
+  B->tile = {tj, tk, 1};
+  auto bail = [&](int code) {
+    gp_batch_destroy(B);
+    return code;
+  };
+  std::vector<int> off(5 * (n + 1), 0);
+  B->h_probs.resize(n);
+  bool any_cpu_oc = false;
+  for (int b = 0; b < n; ++b) {
+    gp_problem* P = probs[b];
+    int rc = ensure_simp_ws(P);
+    if (!rc) rc = ensure_refine_ws(P);
+    if (!rc) rc = ensure_pcg_ws(P, GP_F32);
+    if (rc) return bail(rc);
+    const Grid& g = P->g;
+    const size_t nd = (size_t)g.ndof;
+    float* base = reinterpret_cast<float*>(P->ws32);
+    BatchProb& q = B->h_probs[b];
+    q = BatchProb{};
+    q.g = g;
+    q.mc = P->mc32;
+    q.ni = P->tile32.ni > 0 ? P->tile32.ni : 2;
+    const KuTile t{tj, tk, q.ni};
+    q.ntj = (g.ny + 1 + (tj - 2)) / (tj - 1);
+    q.ntk = (g.nz + 1 + (tk - 2)) / (tk - 1);
+    q.law = P->spec.law;
+    q.oc_variant = P->spec.oc_variant;
+    any_cpu_oc = any_cpu_oc || q.oc_variant == GP_OC_CPU;
+    q.penal = P->spec.penal;
+    q.E0 = P->spec.E0;
+    q.Emin = P->spec.Emin;
+    q.move = P->spec.move;
+    q.n_pres = P->n_pres;
+    q.ke64 = P->ke64;
+    q.ke32 = P->ke32;
+    q.st = P->st;
+    q.Hs = P->Hs;
+    q.w = P->w;
+    q.pres = P->pres;
+    q.fixmask = P->fixmask;
+    q.f = P->f;
+    q.xnew = P->xnew;
+    q.xphys_new = P->xphys_new;
+    q.dc = P->dc;
+    q.dcf = P->dcf;
+    q.E = P->E;
+    q.u = P->d32;
+    q.rb[0] = base;
+    q.rb[1] = base + nd;
+    q.wb[0] = base + 2 * nd;
+    q.wb[1] = base + 3 * nd;
+    q.minv = base + 4 * nd;
+    q.sb[0] = base + 5 * nd;
+    q.sb[1] = base + 6 * nd;
+    q.pc = base + 7 * nd;
+    q.sc = &P->sc[0];
+    q.sc_oc = &P->sc[1];
+    q.parts = P->parts;
+    const int counts[5] = {modal_blocks(g, t), grid_blocks(g.nn), grid_blocks(g.ne), vec_blocks(g),
+                           oc_blocks(g)};
+    for (int kk = 0; kk < 5; ++kk) {
+      if (counts[kk] > P->parts.cap) return bail(fail(GP_ERR_ARG, "partials too small"));
+      off[kk * (n + 1) + b + 1] = off[kk * (n + 1) + b] + counts[kk];
+    }
+  }
+  B->oc_max_steps = any_cpu_oc ? 50 : 30;
+  int rc = alloc(reinterpret_cast<void**>(&B->d_probs), n * sizeof(BatchProb));
+  if (!rc) rc = alloc(reinterpret_cast<void**>(&B->d_off), off.size() * sizeof(int));
+  if (!rc) rc = alloc(reinterpret_cast<void**>(&B->d_status), 2 * n * sizeof(BatchStatus));
+  if (rc) return bail(rc);
+  if (hipHostMalloc(reinterpret_cast<void**>(&B->h_status), 2 * n * sizeof(BatchStatus)) != hipSuccess)
+    return bail(fail(GP_ERR_OOM, "hipHostMalloc failed"));
+  if (hipEventCreateWithFlags(&B->ev[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&B->ev[1], hipEventDisableTiming) != hipSuccess)
+    return bail(fail(GP_ERR_HIP, "hipEventCreate failed"));
+  if (hipMemcpy(B->d_off, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+    return bail(fail(GP_ERR_HIP, "offset copy failed"));
+  BatchLaunch& bl = B->bl;
+  bl.probs = B->d_probs;
+  bl.n = n;
+  bl.tj = tj;
+  bl.tk = tk;
+  const int* d = B->d_off;
+  bl.off_cg = d;
+  bl.off_node = d + (n + 1);
+  bl.off_elem = d + 2 * (n + 1);
+  bl.off_vec = d + 3 * (n + 1);
+  bl.off_oc = d + 4 * (n + 1);
+  bl.tot_cg = off[n];
+  bl.tot_node = off[(n + 1) + n];
+  bl.tot_elem = off[2 * (n + 1) + n];
+  bl.tot_vec = off[3 * (n + 1) + n];
+  bl.tot_oc = off[4 * (n + 1) + n];
+  *out = B;
+  return GP_OK;
+}
+
+int gp_batch_destroy(gp_batch* B) {
+  if (!B) return GP_OK;
+  hipSetDevice(B->device);
+  hipDeviceSynchronize();
+  if (B->d_probs) hipFree(B->d_probs);
+  if (B->d_off) hipFree(B->d_off);
+  if (B->d_status) hipFree(B->d_status);
+  if (B->h_status) hipHostFree(B->h_status);
+  for (auto& e : B->ev)
+    if (e) hipEventDestroy(e);
+  delete B;
+  return GP_OK;
+}
+
+int gp_batch_simp_run(gp_batch* B, const gp_simp_opts* o, const double* volfracs, double* const* x,
+                      double* const* xPhys, double* const* u, gp_simp_stats* stats, void* stream) {
+  if (!B || !o || !volfracs || !x || !xPhys || !u || !stats) return fail(GP_ERR_ARG, "NULL argument");
+  if (o->pcg.refine || o->pcg.precision != GP_F32)
+    return fail(GP_ERR_ARG, "batched runs use float32 PCG vectors (precision GP_F32, refine 0)");
+  const int n = (int)B->probs.size();
+  for (int b = 0; b < n; ++b)
+    if (!x[b] || !xPhys[b] || !u[b]) return fail(GP_ERR_ARG, "NULL state pointer for problem %d", b);
+  HIPCHK(hipSetDevice(B->device));
+  auto t0 = std::chrono::steady_clock::now();
+  hipStream_t s = S(stream);
+  for (int b = 0; b < n; ++b) {
+    B->h_probs[b].volfrac = volfracs[b];
+    B->h_probs[b].x = x[b];
+    B->h_probs[b].xPhys = xPhys[b];
+    B->h_probs[b].u64 = u[b];
+    std::memset(&stats[b], 0, sizeof(gp_simp_stats));
+  }
+  HIPCHK(hipMemcpyAsync(B->d_probs, B->h_probs.data(), n * sizeof(BatchProb), hipMemcpyHostToDevice, s));
+  const BatchLaunch& bl = B->bl;
+  if (o->warm_start) batch_cast(bl, 0, s);
+  std::vector<int> its(n, 0);
+  for (int it = 0; it < o->n_iters; ++it) {
+    batch_stiffness(bl, s);
+    batch_pcg_setup(bl, o->warm_start != 0, o->pcg.tol, o->pcg.max_iter, s);
+    LAUNCHCHK();
+    int rc = batch_wait_all_done(B, s, o->pcg.max_iter, its.data());
+    if (rc) return rc;
+    for (int b = 0; b < n; ++b) {
+      stats[b].pcg_iters_total += its[b];
+      if (B->h_status[b].status > stats[b].status) stats[b].status = B->h_status[b].status;
+    }
+    batch_post_solve(bl, B->oc_max_steps, s);
+    LAUNCHCHK();
+    for (int b = 0; b < n; ++b) stats[b].iters_done = it + 1;
+  }
+  // final compliance solve at the returned densities (simp_solver.py:78)
+  batch_stiffness(bl, s);
+  batch_pcg_setup(bl, o->warm_start != 0, o->pcg.tol, o->pcg.max_iter, s);
+  int rc = batch_wait_all_done(B, s, o->pcg.max_iter, its.data());
+  if (rc) return rc;
+  batch_compliance(bl, s);
+  batch_cast(bl, 1, s);
+  batch_status(bl, B->d_status, s);
+  HIPCHK(hipMemcpyAsync(B->h_status, B->d_status, n * sizeof(BatchStatus), hipMemcpyDeviceToHost, s));
+  LAUNCHCHK();
+  HIPCHK(hipStreamSynchronize(s));
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  for (int b = 0; b < n; ++b) {
+    stats[b].pcg_iters_total += its[b];
+    stats[b].compliance = B->h_status[b].sum;
+    stats[b].seconds = secs;
+  }
+  return GP_OK;
 }
 
 }  // extern "C"

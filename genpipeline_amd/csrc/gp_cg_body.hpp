@@ -1,0 +1,240 @@
+// Device body of one single-kernel Jacobi-PCG iteration on the modal sweep
+// (see gp_cg_modal.hip for the recurrence), shared by the single-problem
+// and batched launches.
+#pragma once
+// Multiply-adds are fused only where one source expression allows it, so the
+// rounding of a body does not depend on where the compiler inlines it (the
+// single-problem and batched kernels then agree bit for bit).
+#pragma clang fp contract(on)
+#include "gp_bodies.hpp"
+#include "gp_modal.hpp"
+
+namespace gp {
+
+// One node's update for iteration k: s_{k-1} = w_{k-1} + beta s_{k-2},
+// r_k = r_{k-1} - alpha s_{k-1}, z_k = M r_k, and z_{k-1} = M r_{k-1} (for p).
+template <typename T>
+struct NodeCG {
+  T r[3], s[3], z[3], zp[3];
+};
+
+template <typename T>
+__device__ inline void node_cg(const Grid& g, const T* __restrict__ rp, const T* __restrict__ wp,
+                               const T* __restrict__ sp, const T* __restrict__ m, int ip, int jj,
+                               int kk, T alpha, T beta, NodeCG<T>& o) {
+  const bool ok = (unsigned)ip <= (unsigned)g.nx && (unsigned)jj <= (unsigned)g.ny &&
+                  (unsigned)kk <= (unsigned)g.nz;
+  const long long n = ok ? (long long)ip * g.sI + (long long)jj * g.sJ + kk : 0;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const T vr = rp[3 * n + q], vw = wp[3 * n + q], vs = sp[3 * n + q], vm = m[3 * n + q];
+    const T r0 = ok ? vr : T(0), w0 = ok ? vw : T(0), s0 = ok ? vs : T(0), m0 = ok ? vm : T(0);
+    const T sn = w0 + beta * s0;
+    const T rn = r0 - alpha * sn;
+    o.s[q] = sn;
+    o.r[q] = rn;
+    o.z[q] = m0 * rn;
+    o.zp[q] = m0 * r0;
+  }
+}
+
+// 2-D transform of z over the element column's four corners, read from the
+// plane's LDS tile zs[q][aa][bb] (aa, bb = node column - (J0-1, K0-1)).
+template <typename T, int TJ, int TK>
+__device__ inline void corners_transform(const T (&zs)[3][TJ + 1][TK + 1], int a, int b,
+                                         Plane<T>& out) {
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const T c00 = zs[q][a][b], c01 = zs[q][a][b + 1];
+    const T c10 = zs[q][a + 1][b], c11 = zs[q][a + 1][b + 1];
+    const T a0 = c00 + c01, d0 = c01 - c00;
+    const T a1 = c10 + c11, d1 = c11 - c10;
+    out.P[q][0] = a0 + a1;
+    out.P[q][1] = d0 + d1;
+    out.P[q][2] = a1 - a0;
+    out.P[q][3] = d1 - d0;
+  }
+}
+
+template <int NT>
+__device__ inline void block_sum3_to(double v0, double v1, double v2, double* p0, double* p1,
+                                     double* p2) {
+  constexpr int NW = (NT + 63) / 64;
+  __shared__ double red[3][NW];
+  v0 = wave_sum64(v0);
+  v1 = wave_sum64(v1);
+  v2 = wave_sum64(v2);
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = v0;
+    red[1][threadIdx.x >> 6] = v1;
+    red[2][threadIdx.x >> 6] = v2;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int w = 0; w < NW; ++w) {
+      s0 += red[0][w];
+      s1 += red[1][w];
+      s2 += red[2][w];
+    }
+    *p0 = s0;
+    *p1 = s1;
+    *p2 = s2;
+  }
+}
+
+// Node loading: thread (a,b) updates its own node column (J0+a, K0+b); the
+// TJ+TK+1 halo columns with aa = 0 or bb = 0 are spread over the first
+// threads. Each plane's z goes to an LDS tile; element columns then read
+// their four corners from it.
+template <typename T, int TJ, int TK>
+__device__ inline void body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
+                                     const ModalCoef<T>& kc, const float* __restrict__ E,
+                                     const T* __restrict__ rp, const T* __restrict__ wp,
+                                     const T* __restrict__ sp, const T* __restrict__ minv,
+                                     T* __restrict__ rn, T* __restrict__ wn, T* __restrict__ sn,
+                                     T* __restrict__ x, T* __restrict__ p,
+                                     const uint8_t* __restrict__ fixmask, DevScalars* sc,
+                                     const Parts& parts, int k, int lb, int nb) {
+  constexpr int NT = TJ * TK;
+  constexpr int NH = TJ + TK + 1;  // halo node columns (aa = 0 or bb = 0)
+  if (sc->done) return;
+  __shared__ T cv[2][4][3][NT];
+  __shared__ T zs[2][3][TJ + 1][TK + 1];
+  const int tid = threadIdx.x;
+  const int a = tid / TK, b = tid - (tid / TK) * TK;
+  int bid = lb;
+  const int tk_ = bid % ntk;
+  bid /= ntk;
+  const int tj_ = bid % ntj;
+  const int ti_ = bid / ntj;
+  const int J0 = tj_ * (TJ - 1), K0 = tk_ * (TK - 1), I0 = ti_ * NI;
+  const int je = J0 - 1 + a, ke = K0 - 1 + b;
+  const bool col_ok = (unsigned)je < (unsigned)g.ny && (unsigned)ke < (unsigned)g.nz;
+  const bool owner = a < TJ - 1 && b < TK - 1 && (je + 1) <= g.ny && (ke + 1) <= g.nz;
+  // halo assignment
+  const bool has_h = tid < NH;
+  const int ha = tid <= TK ? 0 : tid - TK;
+  const int hb = tid <= TK ? tid : 0;
+
+  T alpha = T(0), beta = T(0);
+  if (k >= 1) {
+    const int q = (k - 1) & 1;
+    double s3[3];
+    reduce3_all<NT>(parts.G[q], parts.D[q], parts.R[q], nb, s3);  // gamma, delta, rho
+    const double gam = s3[0], del = s3[1], rho = s3[2];
+    const double bet = (k == 1) ? 0.0 : gam / sc->cgg[k & 1];
+    const double den = (k == 1) ? del : del - bet * gam / sc->cga[k & 1];
+    const bool conv = rho <= sc->tol2;
+    const bool cap = k - 1 >= sc->max_iter;
+    const bool brk = !(den > 0.0);
+    if (conv || cap || brk) {
+      if (lb == 0 && tid == 0) {
+        sc->iter = k - 1;
+        sc->rr = rho;
+        sc->status = conv ? GP_OK : (brk ? GP_BREAKDOWN : GP_NOT_CONVERGED);
+        sc->done = 1;
+      }
+      return;
+    }
+    const double alp = gam / den;
+    if (lb == 0 && tid == 0) {
+      sc->cgg[(k - 1) & 1] = gam;
+      sc->cga[(k - 1) & 1] = alp;
+      sc->iter = k - 1;
+      sc->rr = rho;
+    }
+    alpha = (T)alp;
+    beta = (T)bet;
+  }
+
+  auto Eat = [&](int ie) -> T {
+    const bool ok = col_ok && (unsigned)ie < (unsigned)g.nx;
+    const int e = ok ? ie * g.eI + je * g.eJ + ke : 0;
+    const float v = E[e];
+    return ok ? (T)v : T(0);
+  };
+  // updates plane ip's own and halo nodes and publishes their z in zs[buf]
+  auto stage = [&](int ip, int buf, NodeCG<T>& own) {
+    node_cg(g, rp, wp, sp, minv, ip, je + 1, ke + 1, alpha, beta, own);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) zs[buf][q][a + 1][b + 1] = own.z[q];
+    if (has_h) {
+      NodeCG<T> h;
+      node_cg(g, rp, wp, sp, minv, ip, J0 - 1 + ha, K0 - 1 + hb, alpha, beta, h);
+#pragma unroll
+      for (int q = 0; q < 3; ++q) zs[buf][q][ha][hb] = h.z[q];
+    }
+  };
+
+  Plane<T> Pold, Pnew, topp, bot, top;
+  NodeCG<T> own_cur, own_nxt;
+  {
+    NodeCG<T> own_m1;
+    stage(I0 - 1, 0, own_m1);
+    stage(I0, 1, own_cur);
+    __syncthreads();
+    corners_transform<T, TJ, TK>(zs[0], a, b, Pold);
+    corners_transform<T, TJ, TK>(zs[1], a, b, Pnew);
+    __syncthreads();  // step 0 rewrites zs[0]
+    element(kc, Eat(I0 - 1), Pold, Pnew, bot, topp);
+    Pold = Pnew;
+  }
+
+  double gk = 0.0, dk = 0.0, rk = 0.0;
+  const int iend = min(I0 + NI, g.nx + 1);
+  for (int ie = I0, t = 0; ie < iend; ++ie, ++t) {
+    const T Ee = Eat(ie);
+    const int zb = t & 1;  // buffer of plane ie+1 (planes I0-1, I0 used 0, 1)
+    if (ie < g.nx) {
+      // zs[zb] was last read two steps earlier (or in the prologue), before a
+      // barrier every thread has since passed
+      stage(ie + 1, zb, own_nxt);
+      __syncthreads();
+      corners_transform<T, TJ, TK>(zs[zb], a, b, Pnew);
+      element(kc, Ee, Pold, Pnew, bot, top);
+    } else {
+#pragma unroll
+      for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+          bot.P[q][m] = 0;
+          top.P[q][m] = 0;
+        }
+    }
+    const int par = t & 1;
+    faces_to_corners<T, NT>(topp, bot, cv[par], tid);
+    __syncthreads();
+    if (owner) {
+      const long long n = (long long)ie * g.sI + (long long)(je + 1) * g.sJ + (ke + 1);
+      T w[3];
+#pragma unroll
+      for (int q = 0; q < 3; ++q) w[q] = gather_node<T, TK, NT>(cv[par], q, tid);
+      if (fixmask != nullptr) {
+        const uint8_t fm = fixmask[n];
+        if (fm & 1) w[0] = 0;
+        if (fm & 2) w[1] = 0;
+        if (fm & 4) w[2] = 0;
+      }
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const T pk = own_cur.zp[q] + beta * p[3 * n + q];  // p_{k-1}
+        const T xk = x[3 * n + q] + alpha * pk;            // x_k
+        p[3 * n + q] = pk;
+        x[3 * n + q] = xk;
+        rn[3 * n + q] = own_cur.r[q];
+        sn[3 * n + q] = own_cur.s[q];
+        wn[3 * n + q] = w[q];
+        gk += (double)own_cur.r[q] * (double)own_cur.z[q];
+        dk += (double)own_cur.z[q] * (double)w[q];
+        rk += (double)own_cur.r[q] * (double)own_cur.r[q];
+      }
+    }
+    topp = top;
+    Pold = Pnew;
+    own_cur = own_nxt;
+  }
+  block_sum3_to<NT>(gk, dk, rk, parts.G[k & 1] + lb, parts.D[k & 1] + lb, parts.R[k & 1] + lb);
+}
+
+}  // namespace gp

@@ -1,0 +1,113 @@
+// One-kernel-per-iteration Jacobi-PCG on the modal hex8 K.u sweep.
+//
+// The recurrence is the Chronopoulos-Gear form of preconditioned CG: both
+// inner products of an iteration, gamma = (r,z) and delta = (z,Az), come from
+// one pass, so a single global reduction separates iterations:
+//   z_k = M r_k, w_k = A z_k, gamma_k = (r_k,z_k), delta_k = (z_k,w_k)
+//   beta_k  = gamma_k / gamma_{k-1}                       (beta_0 = 0)
+//   alpha_k = gamma_k / (delta_k - beta_k gamma_k / alpha_{k-1})
+//   p_k = z_k + beta_k p_{k-1};  s_k = w_k + beta_k s_{k-1}   (s_k = A p_k)
+//   x_{k+1} = x_k + alpha_k p_k;  r_{k+1} = r_k - alpha_k s_k
+// In exact arithmetic these are the iterates of standard Jacobi-PCG; the
+// denominator of alpha_k equals p_k^T A p_k and its sign is the breakdown
+// test.
+//
+// Kernel k first reduces the partials of kernel k-1 (gamma, delta, rho =
+// (r,r)), applies the convergence test on rho_{k-1}, and forms alpha_{k-1},
+// beta_{k-1}. While sweeping it builds s_{k-1}, r_k and z_k on every loaded
+// node (halo included, since z_k is needed at all element corners), computes
+// w_k = A z_k, and at each node it owns writes r_k, s_{k-1}, w_k,
+// p_{k-1} = z_{k-1} + beta p_{k-2} and x_k = x_{k-1} + alpha p_{k-1}.
+// r, w and s alternate between two buffers by iteration parity so that no
+// block reads a value another block has already overwritten.
+#include "gp_cg_body.hpp"
+
+namespace gp {
+
+template <typename T, int TJ, int TK>
+__global__ __launch_bounds__(TJ* TK) void k_cg_modal(Grid g, int NI, int ntj, int ntk,
+                                                    ModalCoef<T> kc,
+                                                    const float* __restrict__ E,
+                                                    const T* __restrict__ rp,
+                                                    const T* __restrict__ wp,
+                                                    const T* __restrict__ sp,
+                                                    const T* __restrict__ minv,
+                                                    T* __restrict__ rn, T* __restrict__ wn,
+                                                    T* __restrict__ sn, T* __restrict__ x,
+                                                    T* __restrict__ p,
+                                                    const uint8_t* __restrict__ fixmask,
+                                                    DevScalars* sc, Parts parts, int k) {
+  body_cg_modal<T, TJ, TK>(g, NI, ntj, ntk, kc, E, rp, wp, sp, minv, rn, wn, sn, x, p, fixmask, sc,
+                           parts, k, blockIdx.x, gridDim.x);
+}
+
+template <int TJ, int TK>
+static void launch_cg_t(const Grid& g, int NI, const void* kc, int prec, const float* E,
+                        const void* rp, const void* wp, const void* sp, const void* minv,
+                        void* rn, void* wn, void* sn, void* x, void* p, const uint8_t* fixmask,
+                        DevScalars* sc, Parts parts, int k, hipStream_t st) {
+  const int ntj = (g.ny + 1 + (TJ - 2)) / (TJ - 1);
+  const int ntk = (g.nz + 1 + (TK - 2)) / (TK - 1);
+  const int nti = (g.nx + 1 + NI - 1) / NI;
+  const dim3 grid(ntj * ntk * nti), block(TJ * TK);
+  if (prec == GP_F32)
+    hipLaunchKernelGGL((k_cg_modal<float, TJ, TK>), grid, block, 0, st, g, NI, ntj, ntk,
+                       *reinterpret_cast<const ModalCoef<float>*>(kc), E, (const float*)rp,
+                       (const float*)wp, (const float*)sp, (const float*)minv, (float*)rn,
+                       (float*)wn, (float*)sn, (float*)x, (float*)p, fixmask, sc, parts, k);
+  else
+    hipLaunchKernelGGL((k_cg_modal<double, TJ, TK>), grid, block, 0, st, g, NI, ntj, ntk,
+                       *reinterpret_cast<const ModalCoef<double>*>(kc), E, (const double*)rp,
+                       (const double*)wp, (const double*)sp, (const double*)minv, (double*)rn,
+                       (double*)wn, (double*)sn, (double*)x, (double*)p, fixmask, sc, parts, k);
+}
+
+bool launch_cg_modal(const Grid& g, const KuTile& t, const void* kc, int prec, const float* E,
+                     const void* rp, const void* wp, const void* sp, const void* minv, void* rn,
+                     void* wn, void* sn, void* x, void* p, const uint8_t* fixmask,
+                     DevScalars* sc, Parts parts, int k, hipStream_t st) {
+#define GP_TILE(TJ_, TK_)                                                                   \
+  if (t.tj == TJ_ && t.tk == TK_) {                                                         \
+    launch_cg_t<TJ_, TK_>(g, t.ni, kc, prec, E, rp, wp, sp, minv, rn, wn, sn, x, p, fixmask, \
+                          sc, parts, k, st);                                                \
+    return true;                                                                            \
+  }
+  GP_TILE(8, 32)
+  GP_TILE(16, 16)
+  GP_TILE(14, 18)
+  GP_TILE(6, 34)
+  GP_TILE(14, 34)
+  GP_TILE(4, 64)
+  GP_TILE(10, 26)
+#undef GP_TILE
+  return false;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_init(int nn, const double* __restrict__ f,
+                                                   const T* __restrict__ Ax, int use_x0,
+                                                   T* __restrict__ x, T* __restrict__ r,
+                                                   T* __restrict__ s0, T* __restrict__ w1,
+                                                   T* __restrict__ p, const T* __restrict__ minv,
+                                                   Parts parts) {
+  body_cg_init<T>(nn, f, Ax, use_x0, x, r, s0, w1, p, minv, parts, blockIdx.x, gridDim.x);
+}
+
+template <typename T>
+void launch_cg_init(const Grid& g, const double* f, const T* Ax, bool use_x0, T* x, T* r, T* s0,
+                    T* w1, T* p, const T* minv, Parts parts, DevScalars* sc, double tol,
+                    int max_iter, hipStream_t st) {
+  const int nV = vec_blocks(g);
+  hipLaunchKernelGGL((k_cg_init<T>), dim3(nV), dim3(kBlock), 0, st, g.nn, f, Ax, use_x0 ? 1 : 0, x,
+                     r, s0, w1, p, minv, parts);
+  launch_pcg_init_fin(parts, nV, tol, max_iter, sc, st);
+}
+
+template void launch_cg_init<float>(const Grid&, const double*, const float*, bool, float*, float*,
+                                    float*, float*, float*, const float*, Parts, DevScalars*,
+                                    double, int, hipStream_t);
+template void launch_cg_init<double>(const Grid&, const double*, const double*, bool, double*,
+                                     double*, double*, double*, double*, const double*, Parts,
+                                     DevScalars*, double, int, hipStream_t);
+
+}  // namespace gp

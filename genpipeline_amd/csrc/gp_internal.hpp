@@ -52,6 +52,9 @@ struct Parts {
   double* O[2];    // OC volume sums per block, by bisection-step parity
   double* M;       // OC max(-dc) per block
   double* S;       // one-shot sums (energy, dot, residual)
+  double* G[2];    // single-kernel PCG: (r,z), (z,Az), (r,r) per block, by parity
+  double* D[2];
+  double* R[2];
   int cap;         // entries per region
 };
 
@@ -63,9 +66,60 @@ struct DevScalars {
   double lmid;               // final multiplier
   double volfrac, move, oc_eps, oc_tol, oc_floor;
   double sum;                // one-shot reduction result
+  double cgg[2], cga[2];     // single-kernel PCG: gamma and alpha by parity
   int iter, max_iter, done, status;
   int oc_steps, oc_max, oc_done, nan_flag;
 };
+
+// One problem of a batch as the batched kernels see it (device memory).
+struct BatchProb {
+  Grid g;
+  ModalCoef<float> mc;
+  int ni, ntj, ntk;             // modal sweep geometry for the batch tile
+  int law, oc_variant;
+  double penal, E0, Emin, move, volfrac, n_pres;
+  const double* ke64;
+  const float* ke32;
+  const Stencil* st;
+  const double* Hs;
+  const double* w;
+  const uint8_t* pres;
+  const uint8_t* fixmask;
+  const double* f;
+  double *x, *xPhys, *xnew, *xphys_new, *dc, *dcf;
+  float* E;
+  float* u;                     // float32 PCG iterate
+  double* u64;                  // caller's float64 state
+  float *rb[2], *wb[2], *sb[2], *pc, *minv;
+  DevScalars* sc;               // PCG scalars
+  DevScalars* sc_oc;            // OC / one-shot scalars
+  Parts parts;
+};
+
+// Launch description of a batch: problem table and, per kernel kind, the
+// block offsets off[0..n] of each problem inside one launch.
+struct BatchLaunch {
+  const BatchProb* probs;
+  int n;
+  int tj, tk;
+  const int *off_cg, *off_node, *off_elem, *off_vec, *off_oc;
+  int tot_cg, tot_node, tot_elem, tot_vec, tot_oc;
+};
+
+struct BatchStatus {
+  int done, iter, status, oc_steps;
+  double rr, bb, sum, lmid;
+  int nan_flag, pad;
+};
+
+void batch_stiffness(const BatchLaunch& bl, hipStream_t st);
+void batch_pcg_setup(const BatchLaunch& bl, bool use_x0, double tol, int max_iter,
+                     hipStream_t st);
+void batch_cg(const BatchLaunch& bl, int k, hipStream_t st);
+void batch_post_solve(const BatchLaunch& bl, int oc_max_steps, hipStream_t st);
+void batch_compliance(const BatchLaunch& bl, hipStream_t st);
+void batch_cast(const BatchLaunch& bl, int dir, hipStream_t st);
+void batch_status(const BatchLaunch& bl, BatchStatus* out, hipStream_t st);
 
 __device__ inline double wave_sum64(double v) {
 #pragma unroll
@@ -80,9 +134,11 @@ void launch_ku(const Grid& g, const T* ke, const float* E, const T* u, T* y,
                const uint8_t* fixmask, const DevScalars* sc, double* partA, bool fuse_pap,
                hipStream_t st);
 int gather_blocks(const Grid& g);
+// mode 0: y = K u. mode 1: PCG front half of iteration `it` (u = z,
+// u2 = p_old, pout = p_new, y = A p_new, p.Ap partials into parts.A).
 bool launch_ku_modal(const Grid& g, const KuTile& t, const void* kc, int prec, const float* E,
-                     const void* u, void* y, const uint8_t* fixmask, const DevScalars* sc,
-                     double* partA, bool fuse, hipStream_t st);
+                     const void* u, const void* u2, void* y, void* pout, const uint8_t* fixmask,
+                     DevScalars* sc, Parts parts, int it, int nV, int mode, hipStream_t st);
 int modal_blocks(const Grid& g, const KuTile& t);
 
 template <typename T>
@@ -95,6 +151,19 @@ template <typename T>
 void launch_pcg_init(const Grid& g, const double* f, const T* Ax, bool use_x0, T* x, T* r,
                      T* z, T* p, const T* minv, Parts parts, DevScalars* sc, double tol,
                      int max_iter, hipStream_t st);
+void launch_pcg_init_fin(Parts parts, int nV, double tol, int max_iter, DevScalars* sc,
+                         hipStream_t st);
+template <typename T>
+void launch_cg_init(const Grid& g, const double* f, const T* Ax, bool use_x0, T* x, T* r, T* s0,
+                    T* w1, T* p, const T* minv, Parts parts, DevScalars* sc, double tol,
+                    int max_iter, hipStream_t st);
+// One Jacobi-PCG iteration k (Chronopoulos-Gear form) on the modal sweep:
+// reads r_{k-1} (rp), w_{k-1} (wp), s_{k-2} (sp); writes r_k, w_k, s_{k-1}
+// (rn, wn, sn); updates x and p in place.
+bool launch_cg_modal(const Grid& g, const KuTile& t, const void* kc, int prec, const float* E,
+                     const void* rp, const void* wp, const void* sp, const void* minv, void* rn,
+                     void* wn, void* sn, void* x, void* p, const uint8_t* fixmask,
+                     DevScalars* sc, Parts parts, int k, hipStream_t st);
 template <typename T>
 void launch_pcg_b(const Grid& g, int it, int nA, T* x, T* r, T* z, const T* p, const T* Ap,
                   const T* minv, Parts parts, DevScalars* sc, hipStream_t st);

@@ -11,87 +11,14 @@
 // butterfly order inside the block); each block of the consuming kernel sums
 // the producer's partials in index order. All blocks therefore agree bit for
 // bit on every scalar, and no kernel uses device-scope fences or atomics.
-#include "gp_internal.hpp"
+#include "gp_bodies.hpp"
 
 namespace gp {
-
-// C3D8 corner offsets (simp_solver.py:255-267) and the inverse map
-// (cx,cy,cz) -> local corner index, keyed by cx*4 + cy*2 + cz.
-__device__ constexpr int CX[8] = {0, 1, 1, 0, 0, 1, 1, 0};
-__device__ constexpr int CY[8] = {0, 0, 1, 1, 0, 0, 1, 1};
-__device__ constexpr int CZ[8] = {0, 0, 0, 0, 1, 1, 1, 1};
-__device__ constexpr int LOC[8] = {0, 4, 3, 7, 1, 5, 2, 6};
-
-// ------------------------------------------------------------ reductions
-__device__ inline double wave_max(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-  return v;
-}
-
-// Block sum (blockDim == kBlock); result in thread 0.
-__device__ inline double block_sum0(double v) {
-  __shared__ double sh[kBlock / 64];
-  v = wave_sum64(v);
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
-  __syncthreads();
-  double s = 0.0;
-  if (threadIdx.x == 0)
-    for (int i = 0; i < kBlock / 64; ++i) s += sh[i];
-  __syncthreads();
-  return s;
-}
-
-// Sum of p[0..n) in a fixed order, returned to every thread of the block.
-__device__ inline double reduce_all(const double* __restrict__ p, int n) {
-  __shared__ double sh[kBlock / 64];
-  double s = 0.0;
-  for (int i = threadIdx.x; i < n; i += kBlock) s += p[i];
-  s = wave_sum64(s);
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
-  __syncthreads();
-  double t = 0.0;
-  for (int i = 0; i < kBlock / 64; ++i) t += sh[i];
-  __syncthreads();
-  return t;
-}
-
-__device__ inline double reduce_max_all(const double* __restrict__ p, int n) {
-  __shared__ double sh[kBlock / 64];
-  double s = -INFINITY;
-  for (int i = threadIdx.x; i < n; i += kBlock) s = fmax(s, p[i]);
-  s = wave_max(s);
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
-  __syncthreads();
-  double t = -INFINITY;
-  for (int i = 0; i < kBlock / 64; ++i) t = fmax(t, sh[i]);
-  __syncthreads();
-  return t;
-}
 
 // One block: *out = sum of p[0..n).
 __global__ __launch_bounds__(kBlock) void k_sum_fin(const double* __restrict__ p, int n,
                                                    double* out) {
-  const double s = reduce_all(p, n);
-  if (threadIdx.x == 0) *out = s;
-}
-
-// --------------------------------------------------------------- helpers
-template <typename T> struct V3 { T x, y, z; };
-
-template <typename T>
-__device__ inline V3<T> ld3(const T* __restrict__ a, long long n) {
-  V3<T> r;
-  r.x = a[3 * n];
-  r.y = a[3 * n + 1];
-  r.z = a[3 * n + 2];
-  return r;
-}
-template <typename T>
-__device__ inline void st3(T* __restrict__ a, long long n, T x, T y, T z) {
-  a[3 * n] = x;
-  a[3 * n + 1] = y;
-  a[3 * n + 2] = z;
+  body_sum_fin(p, n, out);
 }
 
 // ------------------------------------------------------- K . u (gather)
@@ -190,40 +117,12 @@ void launch_ku(const Grid& g, const T* ke, const float* E, const T* u, T* y,
                        sc, partA);
 }
 
-// diag(K) per DOF; with invert: Jacobi M^-1 (0 on fixed DOFs and on DOFs
-// with a non-positive diagonal).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_diag(Grid g, const T* __restrict__ ke,
                                                 const float* __restrict__ E,
                                                 const uint8_t* __restrict__ fixmask,
                                                 T* __restrict__ out, int invert) {
-  const int n = blockIdx.x * kBlock + threadIdx.x;
-  if (n >= g.nn) return;
-  const int i = n / g.sI, rem = n - i * g.sI, j = rem / g.sJ, k = rem - j * g.sJ;
-  T d0 = 0, d1 = 0, d2 = 0;
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b)
-#pragma unroll
-      for (int c = 0; c < 2; ++c) {
-        const int ei = i - 1 + a, ej = j - 1 + b, ek = k - 1 + c;
-        const bool ok = (unsigned)ei < (unsigned)g.nx && (unsigned)ej < (unsigned)g.ny &&
-                        (unsigned)ek < (unsigned)g.nz;
-        const int e = ok ? ei * g.eI + ej * g.eJ + ek : 0;
-        const T Ee = ok ? (T)E[e] : T(0);
-        const int L = LOC[(1 - a) * 4 + (1 - b) * 2 + (1 - c)];
-        d0 += Ee * ke[(3 * L + 0) * 24 + 3 * L + 0];
-        d1 += Ee * ke[(3 * L + 1) * 24 + 3 * L + 1];
-        d2 += Ee * ke[(3 * L + 2) * 24 + 3 * L + 2];
-      }
-  if (invert) {
-    const uint8_t fm = fixmask ? fixmask[n] : 0;
-    d0 = ((fm & 1) || !(d0 > 0)) ? T(0) : T(1) / d0;
-    d1 = ((fm & 2) || !(d1 > 0)) ? T(0) : T(1) / d1;
-    d2 = ((fm & 4) || !(d2 > 0)) ? T(0) : T(1) / d2;
-  }
-  st3(out, n, d0, d1, d2);
+  body_diag<T>(g, ke, E, fixmask, out, invert, blockIdx.x, gridDim.x);
 }
 
 template <typename T>
@@ -293,19 +192,12 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(int nn, const double* __res
 
 __global__ __launch_bounds__(kBlock) void k_pcg_init_fin(Parts parts, int nV, double tol,
                                                         int max_iter, DevScalars* sc) {
-  const double bb = reduce_all(parts.BB, nV);
-  const double rr = reduce_all(parts.RR, nV);
-  const double rz = reduce_all(parts.RZ[0], nV);
-  if (threadIdx.x == 0) {
-    sc->bb = bb;
-    sc->rr = rr;
-    sc->rz = rz;
-    sc->tol2 = tol * tol * bb;
-    sc->max_iter = max_iter;
-    sc->iter = 0;
-    sc->status = GP_OK;
-    sc->done = (bb == 0.0 || rr <= tol * tol * bb) ? 1 : 0;
-  }
+  body_pcg_init_fin(parts, nV, tol, max_iter, sc);
+}
+
+void launch_pcg_init_fin(Parts parts, int nV, double tol, int max_iter, DevScalars* sc,
+                         hipStream_t st) {
+  hipLaunchKernelGGL(k_pcg_init_fin, dim3(1), dim3(kBlock), 0, st, parts, nV, tol, max_iter, sc);
 }
 
 template <typename T>
@@ -456,19 +348,9 @@ void launch_cast_f32_to_f64(const float* a, double* b, long long n, hipStream_t 
 }
 
 // --------------------------------------------------------- material law
-__device__ inline double law_E(double x, int law, double p, double E0, double Emin) {
-  if (law == GP_LAW_CLAMP) return pow(fmax(x, 1e-3), p);
-  return Emin + pow(x, p) * (E0 - Emin);
-}
-__device__ inline double law_dE(double x, int law, double p, double E0, double Emin) {
-  if (law == GP_LAW_CLAMP) return p * pow(fmax(x, 1e-3), p - 1.0);
-  return p * pow(x, p - 1.0) * (E0 - Emin);
-}
-
 __global__ void k_stiffness(int ne, const double* __restrict__ x, float* __restrict__ E, int law,
                             double p, double E0, double Emin) {
-  const int e = blockIdx.x * kBlock + threadIdx.x;
-  if (e < ne) E[e] = (float)law_E(x[e], law, p, E0, Emin);
+  body_stiffness(ne, x, E, law, p, E0, Emin, blockIdx.x, gridDim.x);
 }
 void launch_stiffness(const Grid& g, const double* xPhys, float* E, int law, double penal,
                       double E0, double Emin, hipStream_t st) {
@@ -477,45 +359,14 @@ void launch_stiffness(const Grid& g, const double* xPhys, float* E, int law, dou
 }
 
 // ----------------------------------------------------------- sensitivity
-// One thread per element, FP64 energy u_e^T KE u_e (simp_solver.py:215-218).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_sens(Grid g, const double* __restrict__ ke,
                                                 const double* __restrict__ xPhys,
                                                 const T* __restrict__ u, int law, double p,
                                                 double E0, double Emin, double* __restrict__ dc,
                                                 int want_sum, double* partS, DevScalars* sc) {
-  const int e = blockIdx.x * kBlock + threadIdx.x;
-  const bool valid = e < g.ne;
-  const int ec = valid ? e : 0;
-  const int i = ec / g.eI, rem = ec - i * g.eI, j = rem / g.eJ, k = rem - j * g.eJ;
-  const int n0 = i * g.sI + j * g.sJ + k;
-  double ue[24];
-#pragma unroll
-  for (int m = 0; m < 8; ++m) {
-    const int nm = n0 + CX[m] * g.sI + CY[m] * g.sJ + CZ[m];
-    const V3<T> v = ld3(u, nm);
-    ue[3 * m] = (double)v.x;
-    ue[3 * m + 1] = (double)v.y;
-    ue[3 * m + 2] = (double)v.z;
-  }
-  double ce = 0.0;
-#pragma unroll
-  for (int b = 0; b < 24; ++b) {
-    double t = 0.0;
-#pragma unroll
-    for (int a = 0; a < 24; ++a) t += ue[a] * ke[a * 24 + b];
-    ce += t * ue[b];
-  }
-  const double x = valid ? xPhys[e] : 0.0;
-  const double dcv = -law_dE(x, law, p, E0, Emin) * ce;
-  if (valid) {
-    dc[e] = dcv;
-    if (dcv != dcv) sc->nan_flag = 1;
-  }
-  if (want_sum) {
-    const double s = block_sum0(valid ? law_E(x, law, p, E0, Emin) * ce : 0.0);
-    if (threadIdx.x == 0) partS[blockIdx.x] = s;
-  }
+  body_sens<T>(g, ke, xPhys, u, law, p, E0, Emin, dc, want_sum, partS, sc, blockIdx.x,
+               gridDim.x);
 }
 template <typename T>
 void launch_sens(const Grid& g, const double* ke64, const double* xPhys, const T* u, int law,
@@ -529,32 +380,11 @@ void launch_sens(const Grid& g, const double* ke64, const double* xPhys, const T
 }
 
 // ---------------------------------------------------------------- filter
-// out_e = (sum_o w_o in_{e+o}) / Hs_e            (mode 0, simp_solver.py:240)
-// out_e =  sum_o w_o (in_{e+o} / Hs_{e+o})       (mode 1, simp_solver.py:223)
-// Offsets run in lexicographic (di,dj,dk) order, i.e. ascending neighbour
-// index, and products are not contracted into FMAs: the same operation order
-// as a CSR row product of the assembled filter matrix.
 __global__ __launch_bounds__(kBlock) void k_filter(Grid g, const Stencil* __restrict__ s,
                                                   const double* __restrict__ Hs,
                                                   const double* __restrict__ in,
                                                   double* __restrict__ out, int mode) {
-#pragma clang fp contract(off)
-  const int e = blockIdx.x * kBlock + threadIdx.x;
-  if (e >= g.ne) return;
-  const int i = e / g.eI, rem = e - i * g.eI, j = rem / g.eJ, k = rem - j * g.eJ;
-  const int no = s->n;
-  double acc = 0.0;
-  for (int o = 0; o < no; ++o) {
-    const int ii = i + s->di[o], jj = j + s->dj[o], kk = k + s->dk[o];
-    if ((unsigned)ii < (unsigned)g.nx && (unsigned)jj < (unsigned)g.ny &&
-        (unsigned)kk < (unsigned)g.nz) {
-      const int f = ii * g.eI + jj * g.eJ + kk;
-      const double v = mode ? in[f] / Hs[f] : in[f];
-      const double t = s->w[o] * v;
-      acc = acc + t;
-    }
-  }
-  out[e] = mode ? acc : acc / Hs[e];
+  body_filter(g, s, Hs, in, out, mode, blockIdx.x, gridDim.x);
 }
 void launch_filter(const Grid& g, const Stencil* s, const double* Hs, const double* in,
                    double* out, int mode, hipStream_t st) {
@@ -610,38 +440,15 @@ int oc_blocks(const Grid& g) {
   return b < kVecBlocks ? b : kVecBlocks;
 }
 
-// per-block max(-dc) (gpu variant: l2 = 2 max(max(-dc), 1))
 __global__ __launch_bounds__(kBlock) void k_oc_max(int ne, const double* __restrict__ dc,
                                                   double* partM) {
-  double m = -INFINITY;
-  for (int e = blockIdx.x * kBlock + threadIdx.x; e < ne; e += gridDim.x * kBlock)
-    m = fmax(m, -dc[e]);
-  m = wave_max(m);
-  __shared__ double sh[kBlock / 64];
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int i = 0; i < kBlock / 64; ++i) m = fmax(m, sh[i]);
-    partM[blockIdx.x] = m;
-  }
+  body_oc_max(ne, dc, partM, blockIdx.x, gridDim.x);
 }
 
 __global__ __launch_bounds__(kBlock) void k_oc_init_fin(Parts parts, int nb, int variant,
                                                        double volfrac, double move,
                                                        DevScalars* sc) {
-  const double mm = variant == GP_OC_GPU ? reduce_max_all(parts.M, nb) : 0.0;
-  if (threadIdx.x == 0) {
-    sc->ocl1[0] = 0.0;
-    sc->ocl2[0] = (variant == GP_OC_GPU) ? fmax(mm, 1.0) * 2.0 : 1e9;
-    sc->oc_eps = (variant == GP_OC_GPU) ? 1e-15 : 1e-12;
-    sc->oc_tol = (variant == GP_OC_GPU) ? 1e-5 : 1e-4;
-    sc->oc_floor = (variant == GP_OC_GPU) ? 1e-15 : 1e-12;
-    sc->oc_max = (variant == GP_OC_GPU) ? 30 : 50;
-    sc->volfrac = volfrac;
-    sc->move = move;
-    sc->oc_steps = 0;
-    sc->oc_done = 0;
-  }
+  body_oc_init_fin(parts, nb, variant, volfrac, move, sc);
 }
 
 void launch_oc_init(const Grid& g, const double* dc, int variant, double volfrac, double move,
@@ -653,66 +460,12 @@ void launch_oc_init(const Grid& g, const double* dc, int variant, double volfrac
                      move, sc);
 }
 
-__device__ inline double oc_xnew(double x, double dc, double lmid, double floor_, double move) {
-  double v = x * sqrt(fmax(-dc / lmid, floor_));
-  v = fmin(fmax(v, 0.0), 1.0);
-  return fmin(fmax(v, x - move), x + move);
-}
-
-// Bisection step s (simp_solver.py:230-248). The interval at the start of
-// step s follows from step s-1's interval and its volume sum; the stop test
-// on (l1,l2) precedes the volume evaluation, as in the reference loop body.
 __global__ __launch_bounds__(kBlock) void k_oc_step(int s, int ne, const double* __restrict__ x,
                                                    const double* __restrict__ dc,
                                                    const double* __restrict__ w,
                                                    const uint8_t* __restrict__ pres,
                                                    double n_pres, Parts parts, DevScalars* sc) {
-  if (sc->oc_done) return;
-  const int nb = gridDim.x;
-  double l1, l2;
-  if (s == 0) {
-    l1 = sc->ocl1[0];
-    l2 = sc->ocl2[0];
-  } else {
-    const int pq = (s - 1) & 1;
-    const double pl1 = sc->ocl1[pq], pl2 = sc->ocl2[pq];
-    const double plmid = 0.5 * (pl2 + pl1);
-    const double mean = (reduce_all(parts.O[pq], nb) + n_pres) / (double)ne;
-    if (mean > sc->volfrac) {
-      l1 = plmid;
-      l2 = pl2;
-    } else {
-      l1 = pl1;
-      l2 = plmid;
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      sc->ocl1[s & 1] = l1;
-      sc->ocl2[s & 1] = l2;
-    }
-  }
-  const double lmid = 0.5 * (l2 + l1);
-  if ((l2 - l1) / (l1 + l2 + sc->oc_eps) < sc->oc_tol) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      sc->lmid = lmid;
-      sc->oc_steps = s + 1;
-      sc->oc_done = 1;
-    }
-    return;
-  }
-  if (s + 1 >= sc->oc_max && blockIdx.x == 0 && threadIdx.x == 0) {
-    // last permitted step: its lmid is final (the branch after it is unused)
-    sc->lmid = lmid;
-    sc->oc_steps = s + 1;
-    sc->oc_done = 1;
-  }
-  const double fl = sc->oc_floor, mv = sc->move;
-  double acc = 0.0;
-  for (int e = blockIdx.x * kBlock + threadIdx.x; e < ne; e += gridDim.x * kBlock) {
-    const double xn = (pres != nullptr && pres[e]) ? 1.0 : oc_xnew(x[e], dc[e], lmid, fl, mv);
-    acc += w[e] * xn;
-  }
-  acc = block_sum0(acc);
-  if (threadIdx.x == 0) parts.O[s & 1][blockIdx.x] = acc;
+  body_oc_step(s, ne, x, dc, w, pres, n_pres, parts, sc, blockIdx.x, gridDim.x);
 }
 void launch_oc_step(const Grid& g, int s, const double* x, const double* dc, const double* w,
                     const uint8_t* pres, double n_pres, Parts parts, DevScalars* sc,
@@ -724,10 +477,7 @@ void launch_oc_step(const Grid& g, int s, const double* x, const double* dc, con
 __global__ void k_oc_final(int ne, const double* __restrict__ x, const double* __restrict__ dc,
                            const uint8_t* __restrict__ pres, const DevScalars* sc,
                            double* __restrict__ xnew) {
-  const int e = blockIdx.x * kBlock + threadIdx.x;
-  if (e >= ne) return;
-  xnew[e] = (pres != nullptr && pres[e]) ? 1.0
-                                         : oc_xnew(x[e], dc[e], sc->lmid, sc->oc_floor, sc->move);
+  body_oc_final(ne, x, dc, pres, sc, xnew, blockIdx.x, gridDim.x);
 }
 void launch_oc_final(const Grid& g, const double* x, const double* dc, const uint8_t* pres,
                      const DevScalars* sc, double* xnew, hipStream_t st) {
@@ -736,8 +486,7 @@ void launch_oc_final(const Grid& g, const double* x, const double* dc, const uin
 }
 
 __global__ void k_set_preserved(int ne, const uint8_t* __restrict__ pres, double* xPhys) {
-  const int e = blockIdx.x * kBlock + threadIdx.x;
-  if (e < ne && pres[e]) xPhys[e] = 1.0;
+  body_set_preserved(ne, pres, xPhys, blockIdx.x, gridDim.x);
 }
 void launch_set_preserved(const Grid& g, const uint8_t* pres, double* xPhys, hipStream_t st) {
   if (pres == nullptr) return;
@@ -749,12 +498,7 @@ void launch_set_preserved(const Grid& g, const uint8_t* pres, double* xPhys, hip
 template <typename TA, typename T>
 __global__ __launch_bounds__(kBlock) void k_dot(long long n, const TA* __restrict__ a,
                                                const T* __restrict__ b, double* partS) {
-  double acc = 0.0;
-  for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n;
-       i += (long long)gridDim.x * kBlock)
-    acc += (double)a[i] * (double)b[i];
-  acc = block_sum0(acc);
-  if (threadIdx.x == 0) partS[blockIdx.x] = acc;
+  body_dot<TA, T>(n, a, b, partS, blockIdx.x, gridDim.x);
 }
 template <typename TA, typename T>
 void launch_dot(const TA* a, const T* b, long long n, Parts parts, DevScalars* sc,

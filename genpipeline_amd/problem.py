@@ -131,6 +131,17 @@ class Problem:
         self._h = h
         self.set_bcs(fixed, f)
         self.set_preserved(preserved)
+        if ke is None:
+            self._apply_tuned_tiles()
+
+    def _apply_tuned_tiles(self):
+        """Use the autotuner's cached K.u tile for this grid when there is one."""
+        from . import autotune
+        g = (self.spec.nx, self.spec.ny, self.spec.nz)
+        for prec in ("fp32", "fp64"):
+            t = autotune.lookup(g, prec, self.device)
+            if t is not None:
+                self.set_ku_tile(*t, precision=prec)
 
     # ------------------------------------------------------------ lifecycle
     def close(self):

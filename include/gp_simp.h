@@ -112,6 +112,11 @@ typedef struct gp_simp_stats {
 const char* gp_last_error(void);
 int gp_version(void);                                 /* 100*major + minor */
 
+/* Exact unit-cube hex8 element stiffness (E = 1) for Poisson ratio nu into
+ * ke_out (host, 24x24 row-major). Replaces SIMPSolver._get_Ke
+ * (simp_solver.py:284-1543), whose literal table is not a valid hex8 KE.  */
+int gp_hex8_ke(double nu, double* ke_out);
+
 /* Device bytes a problem of this spec allocates (admission by footprint). */
 int gp_footprint_bytes(const gp_spec* spec, int32_t precision, size_t* bytes);
 
@@ -195,6 +200,25 @@ int gp_compliance(gp_problem* p, const void* u, int32_t dtype, double* out_host,
  * u: float64 device (n_dof) warm-start state in/out.                       */
 int gp_simp_run(gp_problem* p, const gp_simp_opts* opts, double* x, double* xPhys,
                 double* u, gp_simp_stats* stats, void* stream);
+
+/* ---- batches ----------------------------------------------------------- *
+ * Many independent problems (any grid sizes, hex8 KE, BCs set) run as one:
+ * every phase of the SIMP loop is one launch over all problems, a problem
+ * whose PCG has converged idles inside that launch. tj x tk is the modal
+ * K.u workgroup shape shared by the batch (each problem keeps its own sweep
+ * depth from gp_problem_set_ku_tile). No counterpart in the reference,
+ * which solves one problem per call (topo_data_gen.py:235-307 threads them). */
+typedef struct gp_batch gp_batch;
+int gp_batch_create(gp_problem* const* problems, int32_t n, int32_t tj, int32_t tk,
+                    gp_batch** out);
+int gp_batch_destroy(gp_batch* b);
+/* One SIMP run per problem (opts shared; opts->pcg.precision must be GP_F32,
+ * refine 0). volfracs: host array (n). x, xPhys, u: host arrays of n device
+ * pointers (float64 n_elem / n_elem / n_dof, in/out as in gp_simp_run).
+ * stats: host array (n). Blocks until all problems finish.              */
+int gp_batch_simp_run(gp_batch* b, const gp_simp_opts* opts, const double* volfracs,
+                      double* const* x, double* const* xPhys, double* const* u,
+                      gp_simp_stats* stats, void* stream);
 
 /* ---- generic-table kernels (reference kernel API) --------------------- *
  * Explicit KE (24x24 float32 row-major) and edof (int32 n_elem x 24) as the

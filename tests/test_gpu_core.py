@@ -114,12 +114,13 @@ def test_diag_matches_assembled(gpu, grid):
     assert np.abs(d - ref).max() <= 1e-13 * ref.max()
 
 
+@pytest.mark.parametrize("kernel", ["modal", "gather"])
 @pytest.mark.parametrize("precision,refine,tol,bound", [
     ("fp64", False, 1e-11, 1e-9),
     ("fp32", True, 1e-11, 1e-9),
     ("fp32", False, 1e-6, 2e-4),
 ])
-def test_pcg_matches_direct(gpu, precision, refine, tol, bound):
+def test_pcg_matches_direct(gpu, precision, refine, tol, bound, kernel):
     grid = (12, 6, 6)
     spec, fixed, f, x, rng = _problem(*grid, seed=4)
     KE = O.ke_hex8(0.3)
@@ -128,6 +129,9 @@ def test_pcg_matches_direct(gpu, precision, refine, tol, bound):
     K = O.assemble_K(KE, E.astype(np.float64), edof, spec.n_dof)
     u_ref, _ = O.solve(K, f, fixed, "direct")
     with Problem(spec, fixed, f, device=gpu) as P:
+        if kernel == "gather":
+            P.set_ku_tile(0, precision="fp32")
+            P.set_ku_tile(0, precision="fp64")
         u, res = P.pcg(torch.from_numpy(E), tol=tol, max_iter=20000, precision=precision,
                        refine=refine, inner_tol=1e-4)
         u = u.cpu().numpy().astype(np.float64)
@@ -151,13 +155,29 @@ def test_pcg_warm_start_and_zero_load(gpu):
         assert rz.iters == 0 and float(uz.abs().max()) == 0.0
 
 
-def test_pcg_max_iter_reports_not_converged(gpu):
+@pytest.mark.parametrize("kernel", ["modal", "gather"])
+def test_pcg_max_iter_reports_not_converged(gpu, kernel):
     grid = (10, 5, 5)
     spec, fixed, f, x, rng = _problem(*grid, seed=6)
     E = O.stiffness(x, "emin", 3.0).astype(np.float32)
     with Problem(spec, fixed, f, device=gpu) as P:
+        if kernel == "gather":
+            P.set_ku_tile(0, precision="fp64")
         _, res = P.pcg(torch.from_numpy(E), tol=1e-14, max_iter=5, precision="fp64")
     assert res.status == 5 and res.iters == 5
+
+
+def test_pcg_modal_and_gather_agree(gpu):
+    """Same iteration count and solution (FP64) with either K.u kernel."""
+    grid = (9, 7, 5)
+    spec, fixed, f, x, rng = _problem(*grid, seed=13)
+    E = torch.from_numpy(O.stiffness(x, "emin", 3.0).astype(np.float32))
+    with Problem(spec, fixed, f, device=gpu) as P:
+        u1, r1 = P.pcg(E, tol=1e-10, precision="fp64")
+        P.set_ku_tile(0, precision="fp64")
+        u2, r2 = P.pcg(E, tol=1e-10, precision="fp64")
+    assert abs(r1.iters - r2.iters) <= 1
+    assert float((u1 - u2).abs().max()) <= 1e-9 * float(u1.abs().max())
 
 
 def test_pcg_breakdown_on_indefinite_table(gpu):
