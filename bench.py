@@ -6,10 +6,14 @@ point loads on face x=max (seed 0), FP32 Jacobi-PCG (FP64 dots) at relative
 tolerance 1e-6 with warm starts, 100 OC iterations plus the final compliance
 solve. One step = one complete SIMP solve from the uniform initial density.
 
-Roofline: the dominant kernel is the fused PCG K.u. Its algorithmic bytes
-per launch are 24*n_nodes + 4*n_elems (read p, write Ap: 12 B/node each;
-read E_e: 4 B/element); its average duration comes from HIP events that
-bracket sampled K.u launches on the solver's stream during the timed steps.
+Roofline: the dominant kernel is k_cg_modal, one launch per Jacobi-PCG
+iteration (K.z, the vector updates and the dot partials). Its algorithmic
+bytes per launch are those of a minimal fused FP32 PCG iteration,
+120*n_nodes + 4*n_elems: the K.u operand and result (24 B/node), the element
+moduli E_e (4 B/element), and x, p, r, Ap, M^-1 read plus x, r, p written
+(96 B/node). Its average duration comes from HIP events that bracket sampled
+chunks of back-to-back launches on the solver's stream during the timed
+steps (chunks that hold early-exit launches after convergence are dropped).
 
 CPU baseline: the oracle's multi-core FP64 C restatement (oracle/simp_cpu.c,
 matrix-free Jacobi-PCG, OpenMP over all host cores) on a bounded sample of
@@ -44,7 +48,11 @@ def parse():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-oc-iters", type=int, default=2)
-    ap.add_argument("--traffic-file", default=str(ROOT / "profiles" / "ku_traffic.json"))
+    ap.add_argument("--traffic-file",
+                    default=str(ROOT / "profiles" / "k_cg_modal_64x64x64_traffic.json"))
+    ap.add_argument("--calib-mib", type=int, default=0,
+                    help="first run one elementwise add over a float32 buffer of this many MiB "
+                         "(a known-size kernel for scaling HBM counters)")
     return ap.parse_args()
 
 
@@ -102,6 +110,15 @@ def main():
     nx, ny, nz = a.grid
     w = point_loads(nx, ny, nz, seed=a.seed, n_loads=4, volfrac=0.3, n_iters=a.oc_iters, tol=1e-6)
     dev = torch.device("cuda", 0)
+    if a.calib_mib:
+        n = a.calib_mib * (1 << 20) // 4
+        ca = torch.ones(n, dtype=torch.float32, device=dev)
+        cb = torch.empty_like(ca)
+        torch.add(ca, 1.0, out=cb)
+        torch.cuda.synchronize()
+        print(f"[bench] calibration add: read {4 * n} B, wrote {4 * n} B", file=sys.stderr,
+              flush=True)
+        del ca, cb
     P = Problem(w.spec, w.fixed, w.f, device=dev)
     run_kw = dict(tol=w.tol, max_iter=20000, precision="fp32", warm_start=True)
     for k in range(a.warmup):
@@ -110,7 +127,7 @@ def main():
               f"C={rw.compliance:.6g}", file=sys.stderr, flush=True)
     ku_tile = P.ku_tile("fp32")
     print(f"[bench] K.u tile (tj, tk, ni, modal) = {ku_tile}", file=sys.stderr, flush=True)
-    P.set_profiling(4)
+    P.set_profiling(1)
     P.ku_profile(reset=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -125,7 +142,8 @@ def main():
     r = results[-1]
     sp = w.spec
     ku_bytes = 24 * sp.n_nodes + 4 * sp.n_elem
-    achieved = ku_bytes / (ku_ms * 1e-3) / 1e9
+    it_bytes = 120 * sp.n_nodes + 4 * sp.n_elem
+    achieved = it_bytes / (ku_ms * 1e-3) / 1e9
     traffic = None
     tf = Path(a.traffic_file)
     if tf.exists():
@@ -164,7 +182,9 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "algorithmic_bytes_per_launch": ku_bytes,
+            "algorithmic_bytes_per_launch": it_bytes,
+            "ku_only_bytes_per_launch": ku_bytes,
+            "ku_only_gbs": ku_bytes / (ku_ms * 1e-3) / 1e9,
             "avg_launch_us": ku_ms * 1e3,
             "event_samples": ku_samples,
         },

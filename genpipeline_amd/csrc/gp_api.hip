@@ -158,7 +158,9 @@ struct gp_problem {
   double* xnew = nullptr;
   double* xphys_new = nullptr;
   double* ce_sum = nullptr;
-  // sampled K.u timing: events bracket every `prof_every`-th fused K.u launch
+  // sampled iteration timing: events bracket every `prof_every`-th chunk of
+  // PCG iteration launches
+  int iter_hint[2] = {0, 0};  // last PCG iteration count (fp32, fp64)
   int prof_every = 0;
   long long prof_counter = 0;
   double prof_ms = 0.0;
@@ -166,7 +168,8 @@ struct gp_problem {
   std::vector<hipEvent_t> prof_pool;
   struct ProfSample {
     hipEvent_t a, b;
-    int launch;  // index of the K.u launch within its solve
+    int launch;  // index of the chunk's first iteration launch within its solve
+    int n;       // launches in the chunk
   };
   std::vector<ProfSample> prof_pending;
 };
@@ -429,11 +432,14 @@ int pcg_core(gp_problem* P, const float* E, const double* f, T* u, double tol, i
   };
   int launch_idx = 0;
   auto issue = [&](int n_it) {
+    // sampled timing: one event pair brackets a whole chunk of back-to-back
+    // iteration launches (no other work in between)
+    const bool sample = P->prof_every > 0 && (P->prof_counter++ % P->prof_every) == 0;
+    hipEvent_t ea = sample ? take_event() : nullptr;
+    hipEvent_t eb = sample ? take_event() : nullptr;
+    const int first = launch_idx;
+    if (ea && eb) hipEventRecord(ea, s);
     for (int it = 0; it < n_it; ++it, ++launch_idx) {
-      const bool sample = P->prof_every > 0 && (P->prof_counter++ % P->prof_every) == 0;
-      hipEvent_t ea = sample ? take_event() : nullptr;
-      hipEvent_t eb = sample ? take_event() : nullptr;
-      if (ea && eb) hipEventRecord(ea, s);
       if (modal) {
         const int k = launch_idx, c = k & 1, o = c ^ 1;
         launch_cg_modal(g, tile, kc, prec, E, rb[o], wb[o], sb[c], minv, rb[c], wb[c], sb[o], u,
@@ -443,17 +449,31 @@ int pcg_core(gp_problem* P, const float* E, const double* f, T* u, double tol, i
         launch_pcg_b<T>(g, launch_idx, nA, u, r, z, p, Ap, minv, P->parts, sc, s);
         launch_pcg_c<T>(g, launch_idx, p, z, P->parts, sc, s);
       }
-      if (ea && eb) {
-        hipEventRecord(eb, s);
-        P->prof_pending.push_back({ea, eb, launch_idx});
-      } else {
-        if (ea) P->prof_pool.push_back(ea);
-        if (eb) P->prof_pool.push_back(eb);
-      }
+    }
+    if (ea && eb) {
+      hipEventRecord(eb, s);
+      P->prof_pending.push_back({ea, eb, first, n_it});
+    } else {
+      if (ea) P->prof_pool.push_back(ea);
+      if (eb) P->prof_pool.push_back(eb);
     }
   };
-  int chunk = 8;
+  // Chunk sizes: the previous solve's iteration count (consecutive solves of
+  // one problem need similar counts) lets most launches go out in large
+  // chunks; near that count, and without a hint, chunks stay small so few
+  // launches are issued after convergence.
+  const int hint = P->iter_hint[prec == GP_F32 ? 0 : 1];
   int issued = 0;
+  auto next_chunk = [&](int prev) {
+    int c = std::min(prev * 2, 64);
+    if (hint > 0) {
+      const int target = (int)(0.8 * hint) - issued;
+      c = target >= 32 ? std::min(target, 256) : 16;
+    }
+    // never far past the iteration cap
+    return std::max(8, std::min(c, max_iter + 2 - issued));
+  };
+  int chunk = next_chunk(4);
   int slot = 0;
   // the init status doubles as the first checkpoint
   HIPCHK(hipMemcpyAsync(&P->h_sc[0], sc, sizeof(DevScalars), hipMemcpyDeviceToHost, s));
@@ -467,7 +487,7 @@ int pcg_core(gp_problem* P, const float* E, const double* f, T* u, double tol, i
     HIPCHK(hipEventSynchronize(P->ev[slot]));
     const DevScalars& h = P->h_sc[slot];
     if (h.done || issued >= max_iter + 2 * chunk) break;
-    chunk = std::min(chunk * 2, 64);
+    chunk = next_chunk(chunk);
     issue(chunk);
     issued += chunk;
     HIPCHK(hipMemcpyAsync(&P->h_sc[slot], sc, sizeof(DevScalars), hipMemcpyDeviceToHost, s));
@@ -479,18 +499,21 @@ int pcg_core(gp_problem* P, const float* E, const double* f, T* u, double tol, i
   if (rc) return rc;
   const DevScalars& h = P->h_sc[0];
   // Launch m ran the kernel body iff m < iterations completed (later launches
-  // found the solve done and returned at once); only those samples count.
+  // found the solve done and returned at once); only chunks whose launches
+  // all ran count.
   for (auto& pr : P->prof_pending) {
     float ms = 0.f;
-    const bool ran = modal ? pr.launch <= h.iter : pr.launch < h.iter;
+    const int last = pr.launch + pr.n - 1;
+    const bool ran = modal ? last <= h.iter : last < h.iter;
     if (ran && hipEventElapsedTime(&ms, pr.a, pr.b) == hipSuccess) {
       P->prof_ms += ms;
-      P->prof_n += 1;
+      P->prof_n += pr.n;
     }
     P->prof_pool.push_back(pr.a);
     P->prof_pool.push_back(pr.b);
   }
   P->prof_pending.clear();
+  P->iter_hint[prec == GP_F32 ? 0 : 1] = h.iter;
   st->status = h.status;
   st->iters += h.iter;
   st->relres = h.bb > 0 ? std::sqrt(h.rr / h.bb) : 0.0;

@@ -191,11 +191,13 @@ class Problem:
             L.call("gp_problem_set_preserved", self._h, arg)
 
     def set_profiling(self, every: int):
-        """Bracket every `every`-th fused K.u launch of PCG with HIP events."""
+        """Bracket every `every`-th chunk of PCG iteration launches with HIP
+        events (0 = off)."""
         L.call("gp_problem_set_profiling", self._h, int(every))
 
     def ku_profile(self, reset=True):
-        """(mean K.u launch milliseconds, samples) since the last reset."""
+        """(mean milliseconds per PCG iteration launch, launches timed) since
+        the last reset."""
         ms = C.c_double(0)
         n = C.c_int64(0)
         L.call("gp_problem_get_profile", self._h, C.byref(ms), C.byref(n), 1 if reset else 0)

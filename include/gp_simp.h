@@ -135,9 +135,10 @@ int gp_problem_set_bcs(gp_problem* p, const int64_t* fixed_dofs, int64_t n_fixed
 /* Preserved (non-design) elements: host uint8 per element or NULL.         */
 int gp_problem_set_preserved(gp_problem* p, const uint8_t* mask_host);
 /* Sampled kernel timing: with every > 0, HIP events bracket every every-th
- * fused K.u launch of the PCG loop on the caller's stream (0 = off).
- * get_profile returns the summed milliseconds and the sample count of the
- * launches that executed (early exits after convergence are dropped).
+ * chunk of back-to-back PCG iteration launches on the caller's stream
+ * (0 = off). get_profile returns the summed milliseconds and the number of
+ * launches in the chunks whose launches all executed (chunks holding early
+ * exits after convergence are dropped).
  * No counterpart in the reference; measurement hook for benchmarks.        */
 int gp_problem_set_profiling(gp_problem* p, int32_t every);
 int gp_problem_get_profile(gp_problem* p, double* ku_ms_sum, int64_t* ku_samples,

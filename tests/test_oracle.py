@@ -1,0 +1,401 @@
+"""CPU pins of the FP64 oracle (no GPU).
+
+Each oracle function is checked against something it does not restate: the
+golden vectors written by importing the reference CPU solver
+(tests/golden/make_golden.py), closed forms of the mathematics, brute force on
+tiny grids, or invariants chosen so that a dropped term, a wrong sign or index
+or a transposed operand fails. The product's own host-side BC builder and the
+library's host KE are checked against the same golden data.
+"""
+import itertools
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import simp_oracle as O
+
+GOLD = Path(__file__).resolve().parent / "golden"
+
+
+@pytest.fixture(scope="module")
+def golden():
+    with np.load(GOLD / "ref_simp_golden.npz", allow_pickle=False) as d:
+        arrays = {k: np.array(d[k]) for k in d.files}
+    meta = json.loads((GOLD / "ref_simp_golden.json").read_text())
+    return arrays, meta
+
+
+def _grid_of(tag):
+    return tuple(int(v) for v in tag.split("/")[2].split("x"))
+
+
+# ------------------------------------------------------------------ numbering
+def _brute_edof(nx, ny, nz):
+    """Nodes numbered by enumeration in (i, j, k) loop order, elements in the
+    same order, corners in the C3D8 order written out literally
+    (simp_solver.py:255-270)."""
+    nodes = {}
+    for i in range(nx + 1):
+        for j in range(ny + 1):
+            for k in range(nz + 1):
+                nodes[(i, j, k)] = len(nodes)
+    c3d8 = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
+    rows = []
+    for i in range(nx):
+        for j in range(ny):
+            for k in range(nz):
+                row = []
+                for (a, b, c) in c3d8:
+                    n = nodes[(i + a, j + b, k + c)]
+                    row += [3 * n, 3 * n + 1, 3 * n + 2]
+                rows.append(row)
+    return np.array(rows, dtype=np.int64)
+
+
+@pytest.mark.parametrize("grid", [(1, 1, 1), (2, 1, 1), (1, 2, 3), (3, 2, 2), (4, 3, 5)])
+def test_edof_brute_force(grid):
+    assert np.array_equal(O.edof_matrix(*grid), _brute_edof(*grid))
+
+
+def test_edof_node_multiplicity():
+    nx, ny, nz = 5, 4, 3
+    ed = O.edof_matrix(nx, ny, nz)
+    nodes = ed[:, 0::3] // 3
+    cnt = np.bincount(nodes.ravel(), minlength=(nx + 1) * (ny + 1) * (nz + 1))
+    i, j, k = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1), indexing="ij")
+    inner = lambda a, n: ((a > 0) & (a < n)).astype(int) + 1  # 2 inside, 1 on a face
+    expect = (inner(i, nx) * inner(j, ny) * inner(k, nz)).ravel()
+    assert np.array_equal(cnt, expect)
+
+
+def test_c1_load_dof():
+    """Node (32, 8, 0) of the 32x16x16 grid carries DOF 28154 in z."""
+    n = O.node_id(32, 8, 0, 16, 16)
+    assert n == 9384 and 3 * n + 2 == 28154
+
+
+# ------------------------------------------------------------------------- KE
+def test_ke_hex8_symmetric_psd_rigid_modes():
+    KE = O.ke_hex8(0.3)
+    assert np.allclose(KE, KE.T, atol=1e-15)
+    ev = np.linalg.eigvalsh(KE)
+    assert ev.min() > -1e-12
+    assert int((np.abs(ev) < 1e-10).sum()) == 6
+    X = O.CORNERS.astype(float)
+    modes = []
+    for c in range(3):  # translations
+        t = np.zeros((8, 3))
+        t[:, c] = 1.0
+        modes.append(t.ravel())
+    for a, b in [(0, 1), (1, 2), (0, 2)]:  # infinitesimal rotations
+        r = np.zeros((8, 3))
+        r[:, a] = -X[:, b]
+        r[:, b] = X[:, a]
+        modes.append(r.ravel())
+    for m in modes:
+        assert np.abs(KE @ m).max() < 1e-13
+
+
+@pytest.mark.parametrize("nu", [0.0, 0.3, 0.45])
+def test_ke_hex8_uniform_strain_energy(nu):
+    """u = eps x on the unit cube: u^T KE u = integral of eps^T D eps (exact for
+    trilinear elements): lambda+2mu for eps_xx=1; mu for gamma_xy=1;
+    3(3 lambda + 2 mu) for the dilatation eps_xx=eps_yy=eps_zz=1."""
+    E = 1.0
+    lam = E * nu / ((1 + nu) * (1 - 2 * nu))
+    mu = E / (2 * (1 + nu))
+    KE = O.ke_hex8(nu, E)
+    X = O.CORNERS.astype(float)
+    u = np.zeros((8, 3))
+    u[:, 0] = X[:, 0]
+    assert u.ravel() @ KE @ u.ravel() == pytest.approx(lam + 2 * mu, rel=1e-13)
+    u = np.zeros((8, 3))
+    u[:, 0] = X[:, 1]  # gamma_xy = 1
+    assert u.ravel() @ KE @ u.ravel() == pytest.approx(mu, rel=1e-13)
+    u = X.copy()
+    assert u.ravel() @ KE @ u.ravel() == pytest.approx(3 * (3 * lam + 2 * mu), rel=1e-13)
+
+
+def test_ke_hex8_modal_form_has_45_nonzeros():
+    """In the 8-corner Walsh-Hadamard basis the element matrix has exactly 45
+    nonzero entries (the structure the GPU operator relies on)."""
+    KE = O.ke_hex8(0.3)
+    M = np.array(list(itertools.product([0, 1], repeat=3)))
+    W = np.array([[(-1.0) ** (m @ c) for m in M] for c in O.CORNERS])
+    T = np.kron(W, np.eye(3))
+    Kh = T.T @ KE @ T
+    assert int((np.abs(Kh) > 1e-12 * np.abs(Kh).max()).sum()) == 45
+
+
+def test_ke_ref_is_the_defective_table(golden):
+    """The reference table (golden dump) is neither symmetric nor PSD; the
+    product therefore defaults to the exact hex8 matrix."""
+    KE = O.ke_ref()
+    assert KE.shape == (24, 24)
+    assert np.array_equal(KE, golden[0]["ke_ref_nu0.3"])
+    assert not np.allclose(KE, KE.T)
+    assert np.linalg.eigvalsh(0.5 * (KE + KE.T)).min() < 0
+
+
+# --------------------------------------------------------------------- filter
+def test_filter_weights_closed_form():
+    nx, ny, nz = 5, 5, 5
+    H, Hs = O.build_filter(nx, ny, nz, 1.5)
+    c = 2 * 25 + 2 * 5 + 2  # interior element (2,2,2)
+    row = H.getrow(c).toarray().ravel()
+    assert row[c] == 1.5
+    assert np.count_nonzero(row) == 19
+    assert sorted(set(np.round(row[row > 0], 15))) == sorted(
+        {1.5, 0.5, round(1.5 - np.sqrt(2.0), 15)})
+    assert Hs[c] == pytest.approx(1.5 + 6 * 0.5 + 12 * (1.5 - np.sqrt(2.0)), rel=1e-15)
+
+
+def test_filter_hs_brute_force_and_partition_of_unity():
+    nx, ny, nz = 4, 3, 5
+    rmin = 1.5
+    H, Hs = O.build_filter(nx, ny, nz, rmin)
+    hs = np.zeros(nx * ny * nz)
+    for i, j, k in itertools.product(range(nx), range(ny), range(nz)):
+        s = 0.0
+        for a, b, c in itertools.product(range(nx), range(ny), range(nz)):
+            d = np.sqrt((a - i) ** 2 + (b - j) ** 2 + (c - k) ** 2)
+            s += max(0.0, rmin - d)
+        hs[i * ny * nz + j * nz + k] = s
+    assert np.allclose(Hs, hs, rtol=1e-14)
+    assert np.allclose(H @ np.ones(nx * ny * nz) / Hs, 1.0, rtol=1e-14)
+    assert (H != H.T).nnz == 0
+
+
+# ------------------------------------------------------------------------ BCs
+def test_bcs_cpu_match_golden(golden):
+    arrays, meta = golden
+    for tag, bc in meta["bcs"].items():
+        nx, ny, nz = _grid_of(tag)
+        f, fixed = O.get_bcs_cpu(bc, nx, ny, nz, 1.5)
+        nzi = np.nonzero(f)[0]
+        assert np.array_equal(nzi, arrays[tag + "/f_idx"]), tag
+        assert np.array_equal(f[nzi], arrays[tag + "/f_val"]), tag
+        assert np.array_equal(np.asarray(fixed), arrays[tag + "/fixed"]), tag
+
+
+def test_product_bcs_match_golden(golden):
+    from genpipeline_amd.bcs import simp_bcs
+    arrays, meta = golden
+    for tag, bc in meta["bcs"].items():
+        nx, ny, nz = _grid_of(tag)
+        f, fixed = simp_bcs(bc, nx, ny, nz, 1.5)
+        nzi = np.nonzero(f)[0]
+        assert np.array_equal(nzi, arrays[tag + "/f_idx"]), tag
+        assert np.array_equal(f[nzi], arrays[tag + "/f_val"]), tag
+        assert np.array_equal(np.sort(fixed), np.sort(arrays[tag + "/fixed"])), tag
+
+
+@pytest.mark.parametrize("grid", [(8, 4, 4), (9, 5, 6)])
+def test_bcs_gpu_invariants(grid):
+    """SIMPSolverGPU._get_bcs (simp_solver_gpu.py:236-246): the plain branch
+    spreads -force evenly over the non-fixed load-face nodes."""
+    from genpipeline_amd.bcs import simp_gpu_bcs
+    nx, ny, nz = grid
+    for bc in [{"fixed_face": "x_min", "load_face": "x_max", "load_dof": 2},
+               {"fixed_face": "z_min", "load_face": "x_max", "load_dof": 1},
+               {"fixed_face": "x_min", "load_face": "bogus", "load_dof": 0}]:
+        f, fixed = O.get_bcs_gpu(bc, nx, ny, nz, 2.0)
+        fp, fixedp = simp_gpu_bcs(bc, nx, ny, nz, 2.0)
+        assert np.array_equal(f, fp) and np.array_equal(np.sort(fixed), np.sort(fixedp))
+        assert f.sum() == pytest.approx(-2.0, rel=1e-14)
+        loaded = np.nonzero(f)[0]
+        assert np.all(loaded % 3 == bc["load_dof"])
+        assert not np.intersect1d(loaded, fixed).size
+        vals = f[loaded]
+        assert np.all(vals == vals[0])
+        n_face = (ny + 1) * (nz + 1)
+        n_fixed_nodes = len(fixed) // 3
+        if bc["fixed_face"] == "x_min":
+            assert len(loaded) == n_face and n_fixed_nodes == n_face
+        else:  # z_min clamp shares one node row with face x_max
+            assert len(loaded) == n_face - (ny + 1)
+
+
+# ---------------------------------------------------------------- solve / KAT
+def _axial_problem(nx, ny, nz, F=1.0):
+    """Uniform traction of total F in +x on face i=nx with consistent bilinear
+    nodal weights (1/4 per element face corner), clamp on i=0."""
+    f = np.zeros(3 * (nx + 1) * (ny + 1) * (nz + 1))
+    w = np.zeros((ny + 1, nz + 1))
+    for j in range(ny):
+        for k in range(nz):
+            w[j:j + 2, k:k + 2] += 0.25
+    w *= F / w.sum()
+    for j in range(ny + 1):
+        for k in range(nz + 1):
+            f[3 * O.node_id(nx, j, k, ny, nz)] = w[j, k]
+    fixed = np.concatenate([3 * np.arange((ny + 1) * (nz + 1)) + c for c in range(3)])
+    return f, fixed
+
+
+@pytest.mark.parametrize("method", ["direct", "pcg"])
+def test_axial_bar_compliance_closed_form(method):
+    """nu = 0: trilinear elements reproduce the uniform axial state exactly,
+    so compliance = F^2 L / (E A) = F^2 nx / (E ny nz)."""
+    nx, ny, nz = 6, 2, 3
+    f, fixed = _axial_problem(nx, ny, nz, F=2.0)
+    KE = O.ke_hex8(0.0)
+    ed = O.edof_matrix(nx, ny, nz)
+    K = O.assemble_K(KE, np.ones(nx * ny * nz), ed, f.size)
+    u, info = O.solve(K, f, fixed, method, tol=1e-12)
+    assert info.status == "ok"
+    assert f @ u == pytest.approx(4.0 * nx / (ny * nz), rel=1e-9)
+    ux = u[0::3].reshape(nx + 1, ny + 1, nz + 1)
+    assert np.allclose(ux, (2.0 / (ny * nz)) * np.arange(nx + 1)[:, None, None], atol=1e-9)
+
+
+def test_pcg_matches_direct_and_matfree_matches_assembly():
+    rng = np.random.default_rng(3)
+    nx, ny, nz = 5, 3, 4
+    KE = O.ke_hex8(0.3)
+    ed = O.edof_matrix(nx, ny, nz)
+    Ee = rng.uniform(1e-3, 1.0, nx * ny * nz)
+    n = 3 * (nx + 1) * (ny + 1) * (nz + 1)
+    K = O.assemble_K(KE, Ee, ed, n)
+    v = rng.standard_normal(n)
+    assert np.allclose(O.ku_matfree(KE, Ee, v, ed), K @ v, rtol=1e-13, atol=1e-13)
+    # assembled as KE^T per element (the reference's kron index vectors)
+    A = rng.standard_normal((24, 24))
+    KA = O.assemble_K(A, Ee, ed, n).toarray()
+    e = 7
+    blk = KA[np.ix_(ed[e], ed[e])]
+    alone = O.assemble_K(A, np.where(np.arange(Ee.size) == e, Ee, 0.0), ed, n).toarray()
+    assert np.allclose(alone[np.ix_(ed[e], ed[e])], Ee[e] * A.T)
+    assert blk.shape == (24, 24)
+    f, fixed = O.get_bcs_cpu({"load_dof": 2}, nx, ny, nz, 1.0)
+    ud, _ = O.solve(K, f, fixed, "direct")
+    up, info = O.solve(K, f, fixed, "pcg", tol=1e-13)
+    assert info.status == "ok"
+    assert np.abs(up - ud).max() <= 1e-10 * np.abs(ud).max()
+
+
+# ------------------------------------------------------------- sensitivity
+def test_sensitivity_zero_displacement_kat():
+    """tests/test_aggressive_kernel.py:106-125: u = 0 gives dc = 0."""
+    nx, ny, nz = 4, 3, 2
+    ed = O.edof_matrix(nx, ny, nz)
+    x = np.random.default_rng(0).uniform(0.1, 1.0, nx * ny * nz)
+    dc = O.sensitivity(O.ke_hex8(0.3), np.zeros(3 * 5 * 4 * 3), ed, x, "clamp", 3.0)
+    assert np.all(dc == 0.0)
+
+
+def test_compliance_identity_and_sensitivity_derivative():
+    """sum_e E_e ce_e = u^T K u = f^T u at the solution; and dc is the
+    derivative of f^T u(x) (adjoint: dc_e = -dE_e ce_e), checked by central
+    differences on one element."""
+    nx, ny, nz = 4, 2, 3
+    rng = np.random.default_rng(1)
+    P = O.reference_problem(nx, ny, nz, ke="hex8", law="emin", volfrac=0.4)
+    KE, ed, _, _ = P.setup()
+    x = rng.uniform(0.2, 0.9, P.n_elem)
+    u, _ = P.solve_u(x, "direct")
+    Ee = O.stiffness(x, "emin", 3.0)
+    ce = O.element_energy(KE, u, ed)
+    assert (Ee * ce).sum() == pytest.approx(P.f @ u, rel=1e-12)
+    dc = O.sensitivity(KE, u, ed, x, "emin", 3.0)
+    e, h = 5, 1e-6
+    xp, xm = x.copy(), x.copy()
+    xp[e] += h
+    xm[e] -= h
+    cp = P.f @ P.solve_u(xp, "direct")[0]
+    cm = P.f @ P.solve_u(xm, "direct")[0]
+    assert dc[e] == pytest.approx((cp - cm) / (2 * h), rel=1e-6)
+
+
+# --------------------------------------------------------------------- OC
+@pytest.mark.parametrize("variant", ["cpu", "gpu"])
+def test_oc_invariants(variant):
+    nx, ny, nz = 6, 4, 3
+    rng = np.random.default_rng(2)
+    H, Hs = O.build_filter(nx, ny, nz, 1.5)
+    n = nx * ny * nz
+    x = rng.uniform(0.05, 0.95, n)
+    dc = -rng.uniform(0.0, 5.0, n)
+    pres = np.zeros(n, bool)
+    pres[:5] = True
+    fn = O.oc_update_cpu if variant == "cpu" else O.oc_update_gpu
+    xnew, xp, lmid, steps = fn(x, dc, H, Hs, 0.45, 0.2, pres)
+    assert np.all(xnew >= 0) and np.all(xnew <= 1)
+    assert np.all(np.abs(xnew[~pres] - x[~pres]) <= 0.2 + 1e-15)
+    assert np.all(xnew[pres] == 1) and np.all(xp[pres] == 1)
+    assert np.allclose(xp[~pres], (H @ xnew / Hs)[~pres])
+    assert abs(xp.mean() - 0.45) < 1e-3
+    assert steps <= (50 if variant == "cpu" else 30)
+    # the multiplier moves the volume the right way: a larger one removes material
+    lo = fn(x, dc, H, Hs, 0.42, 0.2, pres)[2]
+    assert lo > lmid
+
+
+def test_oc_gpu_bracket_saturates():
+    """simp_solver_gpu.py:801-841 brackets lambda in [0, 2 max(max(-dc), 1)];
+    when the volume target needs a larger multiplier the bisection ends at
+    the top of the bracket with the volume above target (the CPU variant's
+    [0, 1e9] bracket reaches it)."""
+    nx, ny, nz = 6, 4, 3
+    rng = np.random.default_rng(2)
+    H, Hs = O.build_filter(nx, ny, nz, 1.5)
+    n = nx * ny * nz
+    x = rng.uniform(0.05, 0.95, n)
+    dc = -rng.uniform(0.0, 5.0, n)
+    pres = np.zeros(n, bool)
+    pres[:5] = True
+    _, xc, lc, _ = O.oc_update_cpu(x, dc, H, Hs, 0.4, 0.2, pres)
+    _, xg, lg, _ = O.oc_update_gpu(x, dc, H, Hs, 0.4, 0.2, pres)
+    l2 = 2 * max(np.max(-dc), 1.0)
+    assert lc > l2 and lg == pytest.approx(l2, rel=1e-5)
+    assert abs(xc.mean() - 0.4) < 1e-4 and xg.mean() > 0.4 + 1e-3
+
+
+# --------------------------------------------------------- golden trajectories
+def test_runs_match_golden(golden):
+    """SIMPSolver.run trajectories (reference KE, SuperLU, CPU OC) from the
+    reference import: the oracle's driver reproduces xPhys and compliance."""
+    arrays, meta = golden
+    for name, r in meta["runs"].items():
+        pres = None
+        if r["preserved"]:
+            m = np.zeros((r["nx"], r["ny"], r["nz"]), bool)
+            m[r["nx"] - 2:, :, :2] = True
+            pres = m
+        for k in r["iters"]:
+            P = O.reference_problem(r["nx"], r["ny"], r["nz"], bcs=r["bcs"], force_mag=r["force"],
+                                    volfrac=r["volfrac"], preserved=pres, ke="ref", law="clamp",
+                                    oc="cpu")
+            xp, comp, _ = P.run(k, method="direct")
+            gx = arrays[f"run/{name}/{k}/xPhys"].ravel()
+            gc = float(arrays[f"run/{name}/{k}/compliance"])
+            assert np.abs(xp - gx).max() < 1e-9, (name, k)
+            assert comp == pytest.approx(gc, rel=1e-9), (name, k)
+
+
+# ------------------------------------------------------ C restatement vs numpy
+def test_c_oracle_matches_numpy_oracle():
+    from oracle import cpu_ref
+    nx, ny, nz = 6, 3, 4
+    rng = np.random.default_rng(4)
+    KE = O.ke_hex8(0.3)
+    ed = O.edof_matrix(nx, ny, nz)
+    Ee = rng.uniform(1e-3, 1.0, nx * ny * nz)
+    n = 3 * (nx + 1) * (ny + 1) * (nz + 1)
+    f, fixed = O.get_bcs_cpu({"load_dof": 2}, nx, ny, nz, 1.0)
+    v = rng.standard_normal(n)
+    y = cpu_ref.ku((nx, ny, nz), KE, Ee, v, fixed)
+    assert np.allclose(y, O.ku_matfree(KE, Ee, v, ed, fixed), rtol=1e-13, atol=1e-13)
+    u, its = cpu_ref.pcg((nx, ny, nz), KE, Ee, f, fixed, tol=1e-12)[:2]
+    K = O.assemble_K(KE, Ee, ed, n)
+    ud, _ = O.solve(K, f, fixed, "direct")
+    assert np.abs(u - ud).max() <= 1e-9 * np.abs(ud).max()
+
+
+def test_library_hex8_ke_matches_oracle():
+    """The library's host-side KE (used by every device kernel)."""
+    from genpipeline_amd import _lib
+    for nu in (0.0, 0.3, 0.45):
+        assert np.abs(_lib.hex8_ke(nu) - O.ke_hex8(nu)).max() < 1e-15
