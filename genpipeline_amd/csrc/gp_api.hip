@@ -1020,12 +1020,12 @@ int gp_simp_run(gp_problem* P, const gp_simp_opts* o, double* x, double* xPhys, 
   bool have_u = o->warm_start != 0;  // caller's u is a valid guess only when asked
   for (int it = 0; it < o->n_iters; ++it) {
     double tol = o->pcg.tol;
-    int max_iter = o->pcg.max_iter;
-    if (o->tol_schedule) {
-      if (it < 3) { tol = 1e-4; max_iter = 200; }
-      else if (it < 8) { tol = 1e-5; max_iter = 400; }
-      else { tol = 1e-6; max_iter = 800; }
-    }
+    const int max_iter = o->pcg.max_iter;
+    // simp_solver_gpu.py:97-103 tolerance schedule. Its iteration caps
+    // (200/400/800) belong to the reference's incomplete-Cholesky PCG; a
+    // Jacobi-PCG needs more iterations for the same tolerance, so the caller's
+    // cap applies instead.
+    if (o->tol_schedule) tol = it < 3 ? 1e-4 : (it < 8 ? 1e-5 : 1e-6);
     launch_stiffness(g, xPhys, P->E, sp.law, sp.penal, sp.E0, sp.Emin, s);
     gp_pcg_stats ps;
     rc = solve(tol, max_iter, o->warm_start && have_u, &ps);
@@ -1052,10 +1052,11 @@ int gp_simp_run(gp_problem* P, const gp_simp_opts* o, double* x, double* xPhys, 
     HIPCHK(hipMemcpyAsync(xPhys, P->xphys_new, g.ne * sizeof(double), hipMemcpyDeviceToDevice, s));
     st->iters_done = it + 1;
     if (o->compliance_each_iter) {
-      // simp_solver_gpu.py:747-751 -- a fresh solve (zero start, tol 1e-6,
-      // 2000 its); its u becomes the next warm start (simp_solver_gpu.py:593,704)
+      // simp_solver_gpu.py:747-751 -- a fresh solve (zero start, tol 1e-6;
+      // the caller's iteration cap as above); its u becomes the next warm
+      // start (simp_solver_gpu.py:593,704)
       launch_stiffness(g, xPhys, P->E, sp.law, sp.penal, sp.E0, sp.Emin, s);
-      rc = solve(1e-6 > o->pcg.tol ? o->pcg.tol : 1e-6, 2000, false, &ps);
+      rc = solve(1e-6 > o->pcg.tol ? o->pcg.tol : 1e-6, max_iter, false, &ps);
       if (rc) return rc;
       rc = compliance(&curr_c);
       if (rc) return rc;

@@ -91,8 +91,9 @@ typedef struct gp_simp_opts {
   int32_t n_iters;            /* OC iterations                               */
   double volfrac;
   gp_pcg_opts pcg;
-  int32_t tol_schedule;       /* 1: SIMPSolverGPU schedule 1e-4/1e-5/1e-6 and */
-                              /*    200/400/800 its (simp_solver_gpu.py:95-101)*/
+  int32_t tol_schedule;       /* 1: SIMPSolverGPU tolerance schedule         */
+                              /*    1e-4/1e-5/1e-6 (simp_solver_gpu.py:97-103);*/
+                              /*    pcg.max_iter stays the iteration cap     */
   int32_t early_stop;         /* 1: stop after 3 consecutive dC<1%           */
                               /*    (simp_solver_gpu.py:118-125)             */
   int32_t compliance_each_iter; /* 1: compliance solve every OC iteration     */

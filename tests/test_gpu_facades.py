@@ -1,0 +1,272 @@
+"""GPU parity of the drop-in facades against the FP64 oracle driven with the
+same semantics (reference solver interfaces, reference kernel functions)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import simp_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+# --------------------------------------------------------------- SIMPSolver
+CASES = [
+    ("default_8x4x4", 8, 4, 4, None, 0.4, 1.0, False),
+    ("lbracket_6x3x4", 6, 3, 4, {"fixed_face": "z_min", "load_face": "x_max", "load_dof": 2},
+     0.3, 2.0, False),
+    ("preserved_8x4x4", 8, 4, 4, None, 0.35, 1.0, True),
+    ("3dload_9x5x6", 9, 5, 6, {"fixed_face": "x_min", "load_face": "x_max", "load_dof": [1, 2],
+                              "multi_load": True}, 0.3, 1.5, False),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_simp_solver_matches_oracle_every_iteration(gpu, case):
+    """SIMPSolver.run(n_iters=k) equals the oracle's k-th iterate (exact hex8
+    KE, clamp law, CPU OC constants, direct solves): density within 1e-5
+    max-abs, compliance within 1e-6 relative."""
+    from genpipeline_amd.topology import SIMPSolver
+    name, nx, ny, nz, bcs, vf, force, pres = case
+    mask = None
+    if pres:
+        mask = np.zeros((nx, ny, nz), bool)
+        mask[nx - 2:, :, :2] = True
+    P = O.reference_problem(nx, ny, nz, bcs=bcs, force_mag=force, volfrac=vf, preserved=mask,
+                            ke="hex8", law="clamp", oc="cpu")
+    n = 5
+    _, _, hist = P.run(n, method="direct", record=True)
+    s = SIMPSolver(nx, ny, nz, boundary_conditions=bcs, preserved_mask=mask, device=gpu)
+    for k in (1, 2, 3, 5):
+        xp = s.run(volfrac=vf, n_iters=k, force_mag=force)
+        assert xp.shape == (nx, ny, nz) and xp.dtype == np.float64
+        ref_x = hist[k - 1][1]
+        err = np.abs(xp.ravel() - ref_x).max()
+        assert err < 1e-5, (name, k, err)
+        uref, _ = P.solve_u(ref_x, "direct")
+        cref = float(P.f @ uref)
+        assert abs(s.last_compliance - cref) <= 1e-6 * abs(cref), (name, k, s.last_compliance, cref)
+
+
+# ------------------------------------------------------------ SIMPSolverGPU
+def _oracle_simp_gpu_run(P, n_iters, volfrac, x_init=None):
+    """SIMPSolverGPU.run semantics (simp_solver_gpu.py:71-136) on the oracle:
+    tolerance schedule, warm start from the previous compliance solve,
+    compliance solve after every update, early stop."""
+    KE, edof, H, Hs = P.setup()
+    if x_init is not None:
+        x = np.clip(x_init.ravel() * (volfrac / max(x_init.mean(), 1e-6)), 1e-3, 1.0)
+    else:
+        x = np.full(P.n_elem, float(volfrac))
+    if P.preserved is not None:
+        x[P.preserved] = 1.0
+    xPhys = x.copy()
+    u_prev, prev, hist, curr, done = None, np.inf, [], None, 0
+    for i in range(n_iters):
+        tol = 1e-4 if i < 3 else (1e-5 if i < 8 else 1e-6)
+        u, _ = P.solve_u(xPhys, "pcg", tol, x0=u_prev)
+        dc = O.sensitivity(KE, u, edof, xPhys, P.law, P.penal)
+        x, xPhys, _, _ = O.oc_update_gpu(x, O.filter_dc(H, Hs, dc), H, Hs, volfrac,
+                                         preserved=P.preserved)
+        done = i + 1
+        uc, _ = P.solve_u(xPhys, "pcg", 1e-6)
+        u_prev = uc
+        curr = float(P.f @ uc)
+        if np.isfinite(prev):
+            hist.append(abs(prev - curr) / (prev + 1e-10))
+        if len(hist) >= 3 and all(c < 0.01 for c in hist[-3:]):
+            break
+        prev = curr
+    return xPhys, curr, done
+
+
+@pytest.mark.parametrize("x_init", [False, True])
+def test_simp_solver_gpu_fp64_matches_oracle(gpu, x_init):
+    from genpipeline_amd.topology import SIMPSolverGPU
+    nx, ny, nz = 10, 5, 4
+    bcs = {"fixed_face": "x_min", "load_face": "x_max", "load_dof": 2}
+    P = O.reference_problem(nx, ny, nz, bcs=bcs, force_mag=1.0, volfrac=0.35, ke="hex8",
+                            law="clamp", oc="gpu", gpu_bcs=True)
+    xi = np.random.default_rng(5).uniform(0.05, 1.0, (nx, ny, nz)) if x_init else None
+    ref_x, ref_c, ref_n = _oracle_simp_gpu_run(P, 12, 0.35, xi)
+    s = SIMPSolverGPU(nx, ny, nz, boundary_conditions=bcs, dtype=torch.float64)
+    xp = s.run(volfrac=0.35, n_iters=12, force_mag=1.0, x_init=xi)
+    assert s.last_result.iters_done == ref_n
+    assert np.abs(xp.ravel() - ref_x).max() < 1e-6
+    assert abs(s.last_compliance - ref_c) <= 1e-6 * abs(ref_c)
+
+
+def test_simp_solver_gpu_fp32_close_to_oracle(gpu):
+    from genpipeline_amd.topology import SIMPSolverGPU
+    nx, ny, nz = 12, 6, 6
+    P = O.reference_problem(nx, ny, nz, force_mag=1.0, volfrac=0.3, ke="hex8", law="clamp",
+                            oc="gpu", gpu_bcs=True)
+    ref_x, ref_c, ref_n = _oracle_simp_gpu_run(P, 6, 0.3)
+    s = SIMPSolverGPU(nx, ny, nz)
+    xp = s.run(volfrac=0.3, n_iters=6)
+    assert xp.dtype == np.float32 and xp.shape == (nx, ny, nz)
+    assert np.abs(xp.ravel() - ref_x).max() < 2e-3
+    assert abs(s.last_compliance - ref_c) <= 1e-4 * abs(ref_c)
+
+
+def test_simp_solver_gpu_semi_private_api(gpu):
+    """_get_bcs / _assemble_K / _solve as the FEM evaluator calls them."""
+    from genpipeline_amd.topology import SIMPSolverGPU
+    nx, ny, nz = 8, 4, 4
+    s = SIMPSolverGPU(nx, ny, nz, penal=1.0)
+    f, fixed = s._get_bcs(3.0)
+    fo, fixedo = O.get_bcs_gpu(s.bcs, nx, ny, nz, 3.0)
+    assert np.allclose(f.cpu().numpy(), fo, rtol=1e-7, atol=0)
+    assert np.array_equal(np.sort(fixed.cpu().numpy()), np.sort(fixedo))
+    x = np.random.default_rng(1).uniform(0.0, 1.0, nx * ny * nz)
+    K = s._assemble_K(torch.from_numpy(x))
+    assert K.shape == (3 * 9 * 5 * 5,) * 2
+    v = torch.randn(K.shape[0], dtype=torch.float64, device=gpu)
+    # element moduli are held in float32 on the device
+    E32 = np.maximum(x, 1e-3).astype(np.float32).astype(np.float64)
+    Kd = O.assemble_K(O.ke_hex8(0.3), E32, O.edof_matrix(nx, ny, nz), K.shape[0])
+    assert np.allclose((K @ v).cpu().numpy(), Kd @ v.cpu().numpy(), rtol=1e-12, atol=1e-12)
+    u = s._solve(K, f, fixed, tol=1e-6)
+    assert u.dtype == torch.float32 and s._u_prev is u
+    # FP32 vectors on a field with stiffness contrast 1e3: the attainable
+    # accuracy is ~cond(K) * 6e-8, far above the recursive residual's 1e-6
+    fd = f.double().cpu().numpy()
+    ud, _ = O.solve(Kd, fd, fixedo, "direct")
+    assert np.abs(u.double().cpu().numpy() - ud).max() <= 1e-3 * np.abs(ud).max()
+    assert abs(fd @ u.double().cpu().numpy() - fd @ ud) <= 1e-4 * abs(fd @ ud)
+
+
+# ---------------------------------------------------------- DenseGPUSolver
+def test_dense_solver_semantics(gpu):
+    from genpipeline_amd.topology import DenseGPUSolver
+    from genpipeline_amd.topology.simp_solver_dense_gpu import dense_bcs
+    nx, ny, nz = 8, 4, 4
+    f, fixed = dense_bcs(nx, ny, nz, 1000.0)
+    P = O.OracleProblem(nx, ny, nz, f, fixed, penal=3.0, volfrac=0.25, law="clamp", oc="cpu")
+    KE, edof, H, Hs = P.setup()
+    x = np.full(P.n_elem, 0.25)
+    xPhys = x.copy()
+    for _ in range(3):
+        u, _ = P.solve_u(xPhys, "direct")
+        c = float(f @ u)
+        dc = O.filter_dc(H, Hs, O.sensitivity(KE, u, edof, xPhys, "clamp", 3.0))
+        x, xPhys, _, _ = O.oc_update_cpu(x, dc, H, Hs, 0.25)
+    s = DenseGPUSolver(nx, ny, nz)
+    xp = s.run(volfrac=0.25, n_iters=3, force_mag=1000.0)
+    assert xp.shape == (nx, ny, nz)
+    assert np.abs(xp.ravel() - xPhys).max() < 2e-3
+    assert abs(s.last_compliance - c) <= 1e-3 * abs(c)
+
+
+# -------------------------------------------------------------- GPUVoxelFEM
+def _voxels(shape, frac, seed):
+    from scipy.ndimage import gaussian_filter
+    g = gaussian_filter(np.random.default_rng(seed).standard_normal(shape), 1.5)
+    return (g > np.quantile(g, 1 - frac)).astype(np.float32) * 0.98 + 0.01
+
+
+def _oracle_fem(vox, force, fixed_face="x_min", load_face="x_max", load_dof=2):
+    nx, ny, nz = vox.shape
+    bc = {"fixed_face": fixed_face, "load_face": load_face, "load_dof": load_dof}
+    f, fixed = O.get_bcs_gpu(bc, nx, ny, nz, force)
+    f = f.astype(np.float32).astype(np.float64)  # the load vector is float32 on the device
+    P = O.OracleProblem(nx, ny, nz, f, fixed, penal=1.0, law="clamp")
+    u, _ = P.solve_u(vox.ravel().astype(np.float64), "direct")
+    return float(f @ u), float(np.linalg.norm(u.reshape(-1, 3), axis=1).max())
+
+
+@pytest.mark.parametrize("load_dof", [2, 0])
+def test_gpu_voxel_fem_matches_oracle(gpu, load_dof):
+    from genpipeline_amd.fem import GPUVoxelFEM
+    vox = _voxels((12, 6, 6), 0.6, 3)
+    r = GPUVoxelFEM(vox, load_dof=load_dof, force_n=1000.0).solve()
+    c, dmax = _oracle_fem(vox, 1000.0, load_dof=load_dof)
+    n_solid = int((vox > 0.5).sum())
+    assert c >= 1.0 and n_solid >= 50
+    # mixed-precision solve refined to 1e-8: compliance to 1e-6 relative
+    assert abs(r["compliance"] - min(c, 1e8)) <= 1e-6 * min(c, 1e8)
+    assert abs(r["displacement_max"] - min(dmax, 1e4)) <= 1e-6 * min(dmax, 1e4)
+    assert abs(r["stress_max"] - min(c / n_solid * 10.0, 1e4)) <= 1e-6 * r["stress_max"]
+
+
+def test_gpu_voxel_fem_sentinels(gpu):
+    from genpipeline_amd.fem import GPUVoxelFEM
+    vox = np.full((8, 4, 4), 0.01, np.float32)
+    vox[:, 1:3, 1:3] = 1.0  # 32 solid voxels < 50
+    assert GPUVoxelFEM(vox).solve() == {"stress_max": 1e4, "displacement_max": 1e3,
+                                        "compliance": 1e6}
+    vox = _voxels((10, 5, 5), 0.7, 1)
+    assert GPUVoxelFEM(vox, force_n=1e-3).solve()["compliance"] == 1e6  # compliance < 1
+
+
+def test_voxel_fem_evaluator_gpu_branch(gpu):
+    from scipy.ndimage import zoom
+    from genpipeline_amd.fem import GPUVoxelFEM, VoxelFEMEvaluator
+    from genpipeline_amd.pipeline_utils import smooth_voxels
+    from genpipeline_amd.schema import DesignParameters
+    from test_facades_cpu import _TinyDecoder
+    dec = _TinyDecoder()
+    ev = VoxelFEMEvaluator(vae_model=dec, use_gpu=True)
+    r = ev.evaluate(np.zeros(8), dec)
+    vox = smooth_voxels(zoom(dec.field.numpy(), 0.5, order=1))
+    direct = GPUVoxelFEM(vox, force_n=1000.0).solve()
+    assert r["stress"] == pytest.approx(direct["stress_max"], rel=1e-6)
+    assert r["compliance"] == pytest.approx(direct["compliance"], rel=1e-6)
+    assert r["mass"] == pytest.approx(512 * float((vox > 0.5).mean()) * 7900 / 1e9, rel=1e-12)
+    res = ev.evaluate_batch([DesignParameters(h_mm=1, r_mm=1, latent_z=[0.0] * 8)] * 3)
+    assert all(x.success for x in res)
+    assert len({round(x.compliance, 6) for x in res}) == 1
+
+
+# ------------------------------------------------------------- kernels.py
+def test_kernel_functions_match_oracle(gpu):
+    from genpipeline_amd import kernels as K
+    nx, ny, nz = 5, 4, 3
+    rng = np.random.default_rng(7)
+    ed = O.edof_matrix(nx, ny, nz)
+    nd = 3 * 6 * 5 * 4
+    x = rng.uniform(0.0, 1.0, (nx, ny, nz))
+    u = rng.standard_normal(nd)
+    KE = O.ke_hex8(0.3)
+    t = lambda a, dt=torch.float64: torch.as_tensor(a, dtype=dt, device=gpu)
+    dc = K.simp_sensitivity(t(x), t(u), t(KE), t(ed, torch.int64), 3.0, nx, ny, nz)
+    ref = O.sensitivity(KE, u.astype(np.float32).astype(np.float64), ed, x.ravel(), "clamp", 3.0)
+    assert dc.shape == (nx, ny, nz)
+    assert np.abs(dc.cpu().numpy().ravel() - ref).max() <= 1e-5 * np.abs(ref).max()
+    z = K.simp_sensitivity_aggressive(t(x), t(np.zeros(nd)), t(KE), t(ed, torch.int64), 3.0,
+                                      nx, ny, nz)
+    assert torch.count_nonzero(z) == 0  # tests/test_aggressive_kernel.py:106-125
+    A = rng.standard_normal((24, 24))  # asymmetric: pins y_e = A p_e, not A^T p_e
+    y = K.fused_spmv(t(x), t(u), t(A), t(ed, torch.int64), 3.0, nx, ny, nz)
+    E = np.maximum(x.ravel(), 1e-3) ** 3
+    ye = (u.astype(np.float32).astype(np.float64)[ed] @ A.astype(np.float32).astype(np.float64).T)
+    yref = np.bincount(ed.ravel(), weights=(ye * E[:, None]).ravel(), minlength=nd)
+    assert np.abs(y.cpu().numpy() - yref).max() <= 1e-5 * np.abs(yref).max()
+
+
+def test_pcg_solve_cuda_matches_direct(gpu):
+    import scipy.sparse as sp
+    from genpipeline_amd import kernels as K
+    nx, ny, nz = 6, 3, 3
+    x = np.random.default_rng(2).uniform(0.3, 1.0, nx * ny * nz)
+    nd = 3 * 7 * 4 * 4
+    Kd = O.assemble_K(O.ke_hex8(0.3), x ** 3, O.edof_matrix(nx, ny, nz), nd)
+    f, fixed = O.get_bcs_cpu({"load_dof": 2}, nx, ny, nz, 1.0)
+    Kc = sp.csr_matrix(Kd)
+    Kc.sort_indices()
+    t = lambda a, dt: torch.as_tensor(np.asarray(a), dtype=dt, device=gpu)
+    sol = K.pcg_solve_cuda(t(Kc.indptr, torch.int64), t(Kc.indices, torch.int64),
+                           t(Kc.data, torch.float32), t(Kc.diagonal(), torch.float32),
+                           t(f, torch.float32), t(np.zeros(nd), torch.float32),
+                           t(fixed, torch.int64), max_iter=3000, tol=1e-6)
+    ud, _ = O.solve(Kd, f, fixed, "direct")
+    assert np.abs(sol.cpu().numpy() - ud).max() <= 1e-4 * np.abs(ud).max()
+    # the iteration cap is honoured: one iteration from x0 = 0 moves along z0 only
+    one = K.pcg_solve_cuda(t(Kc.indptr, torch.int64), t(Kc.indices, torch.int64),
+                           t(Kc.data, torch.float32), t(Kc.diagonal(), torch.float32),
+                           t(f, torch.float32), t(np.zeros(nd), torch.float32),
+                           t(fixed, torch.int64), max_iter=1, tol=1e-12).cpu().numpy()
+    free = np.ones(nd, bool)
+    free[fixed] = False
+    z0 = np.where(free, f, 0.0) / Kd.diagonal()
+    a0 = (np.where(free, f, 0) @ z0) / (z0 @ np.where(free, Kd @ z0, 0))
+    assert np.allclose(one, a0 * z0, rtol=1e-4, atol=1e-7 * np.abs(a0 * z0).max())
