@@ -57,6 +57,20 @@ class GpSimpStats(C.Structure):
                 ("compliance", C.c_double), ("seconds", C.c_double)]
 
 
+class GpJob(C.Structure):
+    _fields_ = [("n_iters", C.c_int32), ("volfrac", C.c_double), ("tol", C.c_double),
+                ("max_iter", C.c_int32), ("tol_schedule", C.c_int32), ("early_stop", C.c_int32),
+                ("compliance_each_iter", C.c_int32), ("warm_start", C.c_int32)]
+
+
+class GpSlotStatus(C.Structure):
+    _fields_ = [("phase", C.c_int32), ("done", C.c_int32), ("iters_done", C.c_int32),
+                ("pcg_iters_total", C.c_int32), ("bisect_steps_total", C.c_int32),
+                ("status", C.c_int32), ("compliance", C.c_double)]
+
+
+PHASE_IDLE, PHASE_DONE = 0, 18
+
 P = C.c_void_p
 I32, I64, F64, SZ = C.c_int32, C.c_int64, C.c_double, C.c_size_t
 
@@ -85,6 +99,15 @@ SIGNATURES = {
     "gp_batch_create": [P, I32, I32, I32, C.POINTER(P)],
     "gp_batch_destroy": [P],
     "gp_batch_simp_run": [P, C.POINTER(GpSimpOpts), P, P, P, P, P, P],
+    "gp_sched_create": [I32, P, I32, I32, C.POINTER(P)],
+    "gp_sched_destroy": [P],
+    "gp_sched_blocks": [P, I32, C.POINTER(I32)],
+    "gp_sched_admit": [P, I32, P, C.POINTER(GpJob), P, P, P, P],
+    "gp_sched_release": [P, I32, P],
+    "gp_sched_set_graph": [P, I32],
+    "gp_sched_tick": [P, I64, P],
+    "gp_sched_status": [P, P, P],
+    "gp_sched_run": [P, I64, C.POINTER(I64), P, P],
     "gp_generic_sens": [P, P, P, P, I64, F64, P, P],
     "gp_generic_spmv": [P, P, P, P, I64, I64, F64, P, P],
     "gp_csr_pcg": [P, P, P, P, P, P, P, I64, I64, I32, F64, C.POINTER(GpPcgStats), P],

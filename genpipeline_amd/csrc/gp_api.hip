@@ -909,10 +909,10 @@ int gp_sens(gp_problem* P, const double* xPhys, const void* u, int32_t dtype, do
   HIPCHK(hipSetDevice(P->device));
   const gp_spec& sp = P->spec;
   if (dtype == GP_F32)
-    launch_sens<float>(P->g, P->ke64, xPhys, (const float*)u, sp.law, sp.penal, sp.E0, sp.Emin, dc,
+    launch_sens<float>(P->g, P->ke64, P->modal_ok ? &P->mc64 : nullptr, xPhys, (const float*)u, sp.law, sp.penal, sp.E0, sp.Emin, dc,
                        ce_sum, P->parts, &P->sc[1], S(stream));
   else if (dtype == GP_F64)
-    launch_sens<double>(P->g, P->ke64, xPhys, (const double*)u, sp.law, sp.penal, sp.E0, sp.Emin,
+    launch_sens<double>(P->g, P->ke64, P->modal_ok ? &P->mc64 : nullptr, xPhys, (const double*)u, sp.law, sp.penal, sp.E0, sp.Emin,
                         dc, ce_sum, P->parts, &P->sc[1], S(stream));
   else
     return fail(GP_ERR_ARG, "gp_sens: dtype must be GP_F32 or GP_F64");
@@ -1033,10 +1033,10 @@ int gp_simp_run(gp_problem* P, const gp_simp_opts* o, double* x, double* xPhys, 
     have_u = true;
     launch_reset_flags(&P->sc[1], s);
     if (refine)
-      launch_sens<double>(g, P->ke64, xPhys, u, sp.law, sp.penal, sp.E0, sp.Emin, P->dc, nullptr,
+      launch_sens<double>(g, P->ke64, P->modal_ok ? &P->mc64 : nullptr, xPhys, u, sp.law, sp.penal, sp.E0, sp.Emin, P->dc, nullptr,
                           P->parts, &P->sc[1], s);
     else
-      launch_sens<float>(g, P->ke64, xPhys, u32, sp.law, sp.penal, sp.E0, sp.Emin, P->dc, nullptr,
+      launch_sens<float>(g, P->ke64, P->modal_ok ? &P->mc64 : nullptr, xPhys, u32, sp.law, sp.penal, sp.E0, sp.Emin, P->dc, nullptr,
                          P->parts, &P->sc[1], s);
     launch_filter(g, P->st, P->Hs, P->dc, P->dcf, 1, s);
     rc = run_oc(P, x, P->dcf, o->volfrac, P->xnew, P->xphys_new, s);
@@ -1212,6 +1212,67 @@ struct gp_batch {
 
 namespace {
 
+// Device view of problem P for the batched and scheduled kernels: pointers
+// into P's workspaces, modal sweep geometry for tile (tj, tk, ni).
+int fill_batch_prob(gp_problem* P, int tj, int tk, int ni, BatchProb& q) {
+  int rc = ensure_simp_ws(P);
+  if (!rc) rc = ensure_refine_ws(P);
+  if (!rc) rc = ensure_pcg_ws(P, GP_F32);
+  if (rc) return rc;
+  const Grid& g = P->g;
+  const size_t nd = (size_t)g.ndof;
+  float* base = reinterpret_cast<float*>(P->ws32);
+  q = BatchProb{};
+  q.g = g;
+  q.mc = P->mc32;
+  q.mc64 = P->mc64;
+  q.ni = ni;
+  q.ntj = (g.ny + 1 + (tj - 2)) / (tj - 1);
+  q.ntk = (g.nz + 1 + (tk - 2)) / (tk - 1);
+  q.law = P->spec.law;
+  q.oc_variant = P->spec.oc_variant;
+  q.penal = P->spec.penal;
+  q.E0 = P->spec.E0;
+  q.Emin = P->spec.Emin;
+  q.move = P->spec.move;
+  q.n_pres = P->n_pres;
+  q.ke64 = P->ke64;
+  q.ke32 = P->ke32;
+  q.st = P->st;
+  q.Hs = P->Hs;
+  q.w = P->w;
+  q.pres = P->pres;
+  q.fixmask = P->fixmask;
+  q.f = P->f;
+  q.xnew = P->xnew;
+  q.xphys_new = P->xphys_new;
+  q.dc = P->dc;
+  q.dcf = P->dcf;
+  q.E = P->E;
+  q.u = P->d32;
+  q.rb[0] = base;
+  q.rb[1] = base + nd;
+  q.wb[0] = base + 2 * nd;
+  q.wb[1] = base + 3 * nd;
+  q.minv = base + 4 * nd;
+  q.sb[0] = base + 5 * nd;
+  q.sb[1] = base + 6 * nd;
+  q.pc = base + 7 * nd;
+  q.sc = &P->sc[0];
+  q.sc_oc = &P->sc[1];
+  q.parts = P->parts;
+  return GP_OK;
+}
+
+// Block counts of the five kernel kinds (cg, node, elem, vec, oc) for grid g.
+void kind_counts(const Grid& g, const KuTile& t, int out[5]) {
+  out[0] = modal_blocks(g, t);
+  out[1] = grid_blocks(g.nn);
+  out[2] = grid_blocks(g.ne);
+  out[3] = vec_blocks(g);
+  out[4] = oc_blocks(g);
+}
+
 int batch_wait_all_done(gp_batch* B, hipStream_t s, int max_iter, int* iters_out) {
   const int n = (int)B->probs.size();
   int chunk = 8, issued = 0, slot = 0, k = 0;
@@ -1283,54 +1344,13 @@ int gp_batch_create(gp_problem* const* probs, int32_t n, int32_t tj, int32_t tk,
   bool any_cpu_oc = false;
   for (int b = 0; b < n; ++b) {
     gp_problem* P = probs[b];
-    int rc = ensure_simp_ws(P);
-    if (!rc) rc = ensure_refine_ws(P);
-    if (!rc) rc = ensure_pcg_ws(P, GP_F32);
-    if (rc) return bail(rc);
-    const Grid& g = P->g;
-    const size_t nd = (size_t)g.ndof;
-    float* base = reinterpret_cast<float*>(P->ws32);
     BatchProb& q = B->h_probs[b];
-    q = BatchProb{};
-    q.g = g;
-    q.mc = P->mc32;
-    q.ni = P->tile32.ni > 0 ? P->tile32.ni : 2;
-    const KuTile t{tj, tk, q.ni};
-    q.ntj = (g.ny + 1 + (tj - 2)) / (tj - 1);
-    q.ntk = (g.nz + 1 + (tk - 2)) / (tk - 1);
-    q.law = P->spec.law;
-    q.oc_variant = P->spec.oc_variant;
+    const int ni = P->tile32.ni > 0 ? P->tile32.ni : 2;
+    int rc = fill_batch_prob(P, tj, tk, ni, q);
+    if (rc) return bail(rc);
     any_cpu_oc = any_cpu_oc || q.oc_variant == GP_OC_CPU;
-    q.penal = P->spec.penal;
-    q.E0 = P->spec.E0;
-    q.Emin = P->spec.Emin;
-    q.move = P->spec.move;
-    q.n_pres = P->n_pres;
-    q.ke64 = P->ke64;
-    q.ke32 = P->ke32;
-    q.st = P->st;
-    q.Hs = P->Hs;
-    q.w = P->w;
-    q.pres = P->pres;
-    q.fixmask = P->fixmask;
-    q.f = P->f;
-    q.xnew = P->xnew;
-    q.xphys_new = P->xphys_new;
-    q.dc = P->dc;
-    q.dcf = P->dcf;
-    q.E = P->E;
-    q.u = P->d32;
-    q.rb[0] = base;
-    q.rb[1] = base + nd;
-    q.wb[0] = base + 2 * nd;
-    q.wb[1] = base + 3 * nd;
-    q.minv = base + 4 * nd;
-    q.sb[0] = base + 5 * nd;
-    q.sb[1] = base + 6 * nd;
-    q.pc = base + 7 * nd;
-    q.sc = &P->sc[0];
-    q.sc_oc = &P->sc[1];
-    q.parts = P->parts;
+    const Grid& g = P->g;
+    const KuTile t{tj, tk, q.ni};
     const int counts[5] = {modal_blocks(g, t), grid_blocks(g.nn), grid_blocks(g.ne), vec_blocks(g),
                            oc_blocks(g)};
     for (int kk = 0; kk < 5; ++kk) {
@@ -1437,6 +1457,314 @@ int gp_batch_simp_run(gp_batch* B, const gp_simp_opts* o, const double* volfracs
     stats[b].compliance = B->h_status[b].sum;
     stats[b].seconds = secs;
   }
+  return GP_OK;
+}
+
+
+}  // extern "C"
+
+// ============================================================ scheduler
+struct gp_sched {
+  int device = 0;
+  int tj = 8, tk = 32;
+  int n = 0;
+  std::vector<SchedSlot> h_slots;
+  std::vector<int> dims;       // per slot: nx, ny, nz, ni (capacity)
+  std::vector<int> off;        // n + 1 block offsets
+  std::vector<char> busy;
+  SchedSlot* d_slots = nullptr;
+  int* d_off = nullptr;
+  int* d_b2s = nullptr;
+  SchedState* d_status = nullptr;
+  SchedState* h_status = nullptr;  // pinned, 2 snapshots of n
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  long long t = 0;                 // ticks issued
+  hipStream_t cap = nullptr;       // capture-only stream
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int graph_ticks = 0;
+  SchedTable table() const { return SchedTable{d_slots, n, d_off, d_b2s, d_status}; }
+};
+
+namespace {
+
+void sched_free_graph(gp_sched* Q) {
+  if (Q->exec) hipGraphExecDestroy(Q->exec);
+  if (Q->graph) hipGraphDestroy(Q->graph);
+  Q->exec = nullptr;
+  Q->graph = nullptr;
+  Q->graph_ticks = 0;
+}
+
+SchedState idle_state() {
+  SchedState z{};
+  z.phase = PH_IDLE;
+  z.prev_c = INFINITY;
+  return z;
+}
+
+int sched_issue(gp_sched* Q, long long n, hipStream_t s) {
+  const SchedTable tab = Q->table();
+  const int blocks = Q->off[Q->n];
+  while (n > 0) {
+    if (Q->exec && (Q->t & 1) == 0 && n >= Q->graph_ticks) {
+      HIPCHK(hipGraphLaunch(Q->exec, s));
+      Q->t += Q->graph_ticks;
+      n -= Q->graph_ticks;
+    } else {
+      sched_tick(tab, Q->tj, Q->tk, blocks, (int)(Q->t & 1), s);
+      Q->t += 1;
+      n -= 1;
+    }
+  }
+  LAUNCHCHK();
+  return GP_OK;
+}
+
+void to_public(const SchedState& st, gp_slot_status* o) {
+  o->phase = st.phase;
+  o->done = st.phase == PH_DONE ? 1 : 0;
+  o->iters_done = st.oc_it;
+  o->pcg_iters_total = st.pcg_total;
+  o->bisect_steps_total = st.bis_total;
+  o->status = st.status;
+  o->compliance = st.curr_c;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gp_sched_create(int32_t n_slots, const int32_t* slot_dims, int32_t tj, int32_t tk,
+                    gp_sched** out) {
+  if (n_slots < 1 || !slot_dims || !out) return fail(GP_ERR_ARG, "bad argument");
+  if (!((tj == 8 && tk == 32) || (tj == 16 && tk == 16)))
+    return fail(GP_ERR_ARG, "scheduler tiles are 8x32 or 16x16 (256 threads), got %dx%d", tj, tk);
+  *out = nullptr;
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  gp_sched* Q = new gp_sched();
+  Q->device = dev;
+  Q->tj = tj;
+  Q->tk = tk;
+  Q->n = n_slots;
+  Q->dims.assign(slot_dims, slot_dims + 4 * n_slots);
+  Q->h_slots.assign(n_slots, SchedSlot{});
+  Q->busy.assign(n_slots, 0);
+  Q->off.assign(n_slots + 1, 0);
+  auto bail = [&](int code) {
+    gp_sched_destroy(Q);
+    return code;
+  };
+  for (int i = 0; i < n_slots; ++i) {
+    gp_spec sp{};
+    sp.nx = slot_dims[4 * i];
+    sp.ny = slot_dims[4 * i + 1];
+    sp.nz = slot_dims[4 * i + 2];
+    const int ni = slot_dims[4 * i + 3];
+    if (sp.nx < 1 || sp.ny < 1 || sp.nz < 1 || ni < 1)
+      return bail(fail(GP_ERR_ARG, "slot %d: bad capacity dims", i));
+    const Grid g = make_grid(&sp);
+    const int cap = std::max(1, modal_blocks(g, KuTile{tj, tk, ni}));
+    Q->off[i + 1] = Q->off[i] + cap;
+    SchedSlot& q = Q->h_slots[i];
+    q.cap = cap;
+    q.st[0] = idle_state();
+    q.st[1] = idle_state();
+  }
+  const int total = Q->off[n_slots];
+  std::vector<int> b2s(total);
+  for (int i = 0; i < n_slots; ++i)
+    for (int k = Q->off[i]; k < Q->off[i + 1]; ++k) b2s[k] = i;
+  int rc = alloc(reinterpret_cast<void**>(&Q->d_slots), n_slots * sizeof(SchedSlot));
+  if (!rc) rc = alloc(reinterpret_cast<void**>(&Q->d_off), (n_slots + 1) * sizeof(int));
+  if (!rc) rc = alloc(reinterpret_cast<void**>(&Q->d_b2s), total * sizeof(int));
+  if (!rc) rc = alloc(reinterpret_cast<void**>(&Q->d_status), n_slots * sizeof(SchedState));
+  if (rc) return bail(rc);
+  if (hipHostMalloc(reinterpret_cast<void**>(&Q->h_status), 2 * n_slots * sizeof(SchedState)) !=
+      hipSuccess)
+    return bail(fail(GP_ERR_OOM, "hipHostMalloc failed"));
+  if (hipEventCreateWithFlags(&Q->ev[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&Q->ev[1], hipEventDisableTiming) != hipSuccess ||
+      hipStreamCreateWithFlags(&Q->cap, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(GP_ERR_HIP, "event/stream creation failed"));
+  std::vector<SchedState> st(n_slots, idle_state());
+  if (hipMemcpy(Q->d_slots, Q->h_slots.data(), n_slots * sizeof(SchedSlot), hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(Q->d_off, Q->off.data(), (n_slots + 1) * sizeof(int), hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(Q->d_b2s, b2s.data(), total * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(Q->d_status, st.data(), n_slots * sizeof(SchedState), hipMemcpyHostToDevice) !=
+          hipSuccess)
+    return bail(fail(GP_ERR_HIP, "scheduler table upload failed"));
+  *out = Q;
+  return GP_OK;
+}
+
+int gp_sched_destroy(gp_sched* Q) {
+  if (!Q) return GP_OK;
+  hipSetDevice(Q->device);
+  hipDeviceSynchronize();
+  sched_free_graph(Q);
+  if (Q->cap) hipStreamDestroy(Q->cap);
+  if (Q->d_slots) hipFree(Q->d_slots);
+  if (Q->d_off) hipFree(Q->d_off);
+  if (Q->d_b2s) hipFree(Q->d_b2s);
+  if (Q->d_status) hipFree(Q->d_status);
+  if (Q->h_status) hipHostFree(Q->h_status);
+  for (auto& e : Q->ev)
+    if (e) hipEventDestroy(e);
+  delete Q;
+  return GP_OK;
+}
+
+int gp_sched_blocks(gp_sched* Q, int32_t slot, int32_t* blocks) {
+  if (!Q || !blocks || slot < -1 || slot >= Q->n) return fail(GP_ERR_ARG, "bad argument");
+  *blocks = slot < 0 ? Q->off[Q->n] : Q->off[slot + 1] - Q->off[slot];
+  return GP_OK;
+}
+
+int gp_sched_admit(gp_sched* Q, int32_t slot, gp_problem* P, const gp_job* job, double* x,
+                   double* xPhys, double* u, void* stream) {
+  if (!Q || !P || !job || !x || !xPhys || !u) return fail(GP_ERR_ARG, "NULL argument");
+  if (slot < 0 || slot >= Q->n) return fail(GP_ERR_ARG, "slot %d out of range", slot);
+  if (Q->busy[slot]) return fail(GP_ERR_STATE, "slot %d is occupied", slot);
+  if (!P->modal_ok) return fail(GP_ERR_ARG, "problem KE is not an isotropic hex8 table");
+  if (!P->has_bcs) return fail(GP_ERR_STATE, "boundary conditions not set");
+  if (P->device != Q->device) return fail(GP_ERR_ARG, "problem lives on another device");
+  if (job->n_iters < 0 || job->max_iter < 1 || !(job->tol >= 0))
+    return fail(GP_ERR_ARG, "bad job parameters");
+  HIPCHK(hipSetDevice(Q->device));
+  SchedSlot& q = Q->h_slots[slot];
+  const int ni = Q->dims[4 * slot + 3];
+  int rc = fill_batch_prob(P, Q->tj, Q->tk, ni, q.P);
+  if (rc) return rc;
+  int c[5];
+  kind_counts(P->g, KuTile{Q->tj, Q->tk, ni}, c);
+  if (c[0] > q.cap)
+    return fail(GP_ERR_ARG, "problem %dx%dx%d does not fit slot %d", P->g.nx, P->g.ny, P->g.nz,
+                slot);
+  for (int kk = 0; kk < 5; ++kk)
+    if (c[kk] > P->parts.cap) return fail(GP_ERR_ARG, "partials too small for slot tile");
+  q.nb_cg = c[0];
+  q.nb_node = c[1];
+  q.nb_elem = c[2];
+  q.nb_vec = c[3];
+  q.nb_oc = c[4];
+  q.P.volfrac = job->volfrac;
+  q.P.x = x;
+  q.P.xPhys = xPhys;
+  q.P.u64 = u;
+  q.cfg.n_iters = job->n_iters;
+  q.cfg.max_iter = job->max_iter;
+  q.cfg.tol_schedule = job->tol_schedule;
+  q.cfg.compl_each = job->compliance_each_iter;
+  q.cfg.early_stop = job->early_stop;
+  q.cfg.warm_start = job->warm_start;
+  q.cfg.tol = job->tol;
+  SchedState s0 = idle_state();
+  s0.phase = job->warm_start ? PH_CAST_IN : PH_STIFF;
+  s0.solve = job->n_iters > 0 ? SK_OC : SK_FINAL;
+  q.st[0] = s0;
+  q.st[1] = s0;
+  hipStream_t s = S(stream);
+  HIPCHK(hipMemcpyAsync(Q->d_slots + slot, &q, sizeof(SchedSlot), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(Q->d_status + slot, &q.st[0], sizeof(SchedState), hipMemcpyHostToDevice, s));
+  Q->busy[slot] = 1;
+  return GP_OK;
+}
+
+int gp_sched_release(gp_sched* Q, int32_t slot, void* stream) {
+  if (!Q || slot < 0 || slot >= Q->n) return fail(GP_ERR_ARG, "bad argument");
+  HIPCHK(hipSetDevice(Q->device));
+  SchedSlot& q = Q->h_slots[slot];
+  q.st[0] = idle_state();
+  q.st[1] = idle_state();
+  hipStream_t s = S(stream);
+  HIPCHK(hipMemcpyAsync(&Q->d_slots[slot].st[0], &q.st[0], 2 * sizeof(SchedState),
+                        hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(Q->d_status + slot, &q.st[0], sizeof(SchedState), hipMemcpyHostToDevice, s));
+  Q->busy[slot] = 0;
+  return GP_OK;
+}
+
+int gp_sched_set_graph(gp_sched* Q, int32_t ticks) {
+  if (!Q || ticks < 0 || (ticks & 1)) return fail(GP_ERR_ARG, "graph ticks must be even and >= 0");
+  HIPCHK(hipSetDevice(Q->device));
+  sched_free_graph(Q);
+  if (ticks == 0) return GP_OK;
+  const SchedTable tab = Q->table();
+  const int blocks = Q->off[Q->n];
+  HIPCHK(hipStreamBeginCapture(Q->cap, hipStreamCaptureModeThreadLocal));
+  for (int i = 0; i < ticks; ++i) sched_tick(tab, Q->tj, Q->tk, blocks, i & 1, Q->cap);
+  hipGraph_t g = nullptr;
+  HIPCHK(hipStreamEndCapture(Q->cap, &g));
+  Q->graph = g;
+  if (hipGraphInstantiate(&Q->exec, g, nullptr, nullptr, 0) != hipSuccess) {
+    sched_free_graph(Q);
+    return fail(GP_ERR_HIP, "hipGraphInstantiate failed");
+  }
+  Q->graph_ticks = ticks;
+  return GP_OK;
+}
+
+int gp_sched_tick(gp_sched* Q, int64_t n_ticks, void* stream) {
+  if (!Q || n_ticks < 0) return fail(GP_ERR_ARG, "bad argument");
+  HIPCHK(hipSetDevice(Q->device));
+  return sched_issue(Q, n_ticks, S(stream));
+}
+
+int gp_sched_status(gp_sched* Q, gp_slot_status* out, void* stream) {
+  if (!Q || !out) return fail(GP_ERR_ARG, "NULL argument");
+  HIPCHK(hipSetDevice(Q->device));
+  hipStream_t s = S(stream);
+  HIPCHK(hipMemcpyAsync(Q->h_status, Q->d_status, Q->n * sizeof(SchedState), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  for (int i = 0; i < Q->n; ++i) to_public(Q->h_status[i], &out[i]);
+  return GP_OK;
+}
+
+int gp_sched_run(gp_sched* Q, int64_t max_ticks, int64_t* ticks_run, gp_slot_status* out,
+                 void* stream) {
+  if (!Q || max_ticks < 0) return fail(GP_ERR_ARG, "bad argument");
+  HIPCHK(hipSetDevice(Q->device));
+  hipStream_t s = S(stream);
+  const int n = Q->n;
+  const long long chunk = Q->graph_ticks > 0 ? Q->graph_ticks : 32;
+  const long long t0 = Q->t;
+  auto snap = [&](int sl) -> int {
+    HIPCHK(hipMemcpyAsync(Q->h_status + sl * n, Q->d_status, n * sizeof(SchedState),
+                          hipMemcpyDeviceToHost, s));
+    HIPCHK(hipEventRecord(Q->ev[sl], s));
+    return GP_OK;
+  };
+  auto all_done = [&](int sl) {
+    for (int i = 0; i < n; ++i) {
+      const int ph = Q->h_status[sl * n + i].phase;
+      if (Q->busy[i] && ph != PH_DONE && ph != PH_IDLE) return false;
+    }
+    return true;
+  };
+  // one chunk always queued behind the snapshot the host is inspecting
+  int rc = sched_issue(Q, chunk, s);
+  if (!rc) rc = snap(0);
+  int slot = 0;
+  while (!rc) {
+    if (Q->t - t0 < max_ticks) {
+      rc = sched_issue(Q, chunk, s);
+      if (!rc) rc = snap(slot ^ 1);
+      if (rc) break;
+    }
+    HIPCHK(hipEventSynchronize(Q->ev[slot]));
+    if (all_done(slot) || Q->t - t0 >= max_ticks) break;
+    slot ^= 1;
+  }
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(s));
+  HIPCHK(hipMemcpy(Q->h_status, Q->d_status, n * sizeof(SchedState), hipMemcpyDeviceToHost));
+  if (ticks_run) *ticks_run = Q->t - t0;
+  if (out)
+    for (int i = 0; i < n; ++i) to_public(Q->h_status[i], &out[i]);
   return GP_OK;
 }
 

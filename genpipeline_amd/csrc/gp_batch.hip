@@ -11,18 +11,6 @@
 
 namespace gp {
 
-__device__ inline int find_prob(const int* __restrict__ off, int n, int blk) {
-  int lo = 0, hi = n - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (off[mid] <= blk)
-      lo = mid;
-    else
-      hi = mid - 1;
-  }
-  return lo;
-}
-
 #define GP_BATCH_BLOCK(OFF)                              \
   const int b_ = find_prob(bl.OFF, bl.n, blockIdx.x);    \
   const BatchProb& P = bl.probs[b_];                     \
@@ -55,23 +43,25 @@ template <int TJ, int TK>
 __global__ __launch_bounds__(TJ* TK) void k_b_ku(BatchLaunch bl) {
   GP_BATCH_BLOCK(off_cg)
   (void)nb;
+  __shared__ CornerLds<float, TJ * TK> cv;
   body_ku_modal<float, TJ, TK, 0>(P.g, P.ni, P.ntj, P.ntk, P.mc, P.E, P.u, nullptr, P.wb[0],
-                                  nullptr, P.fixmask, P.sc, P.parts, 0, 0, lb);
+                                  nullptr, P.fixmask, P.sc, P.parts, 0, 0, lb, cv);
 }
 
 template <int TJ, int TK>
 __global__ __launch_bounds__(TJ* TK) void k_b_cg(BatchLaunch bl, int k) {
   GP_BATCH_BLOCK(off_cg)
   const int c = k & 1, o = c ^ 1;
+  __shared__ CornerLds<float, TJ * TK> cv;
   body_cg_modal<float, TJ, TK>(P.g, P.ni, P.ntj, P.ntk, P.mc, P.E, P.rb[o], P.wb[o], P.sb[c],
                                P.minv, P.rb[c], P.wb[c], P.sb[o], P.u, P.pc, P.fixmask, P.sc,
-                               P.parts, k, lb, nb);
+                               P.parts, k, lb, nb, cv);
 }
 
 __global__ __launch_bounds__(kBlock) void k_b_sens(BatchLaunch bl) {
   GP_BATCH_BLOCK(off_elem)
-  body_sens<float>(P.g, P.ke64, P.xPhys, P.u, P.law, P.penal, P.E0, P.Emin, P.dc, 0, nullptr,
-                   P.sc_oc, lb, nb);
+  body_sens<float, true>(P.g, P.ke64, P.mc64, P.xPhys, P.u, P.law, P.penal, P.E0, P.Emin, P.dc, 0,
+                         nullptr, P.sc_oc, lb, nb);
 }
 
 // mode 1: dcf = H^T (dc/Hs); mode 0: xphys_new = (H xnew)/Hs

@@ -8,6 +8,7 @@
 // single-problem and batched kernels then agree bit for bit).
 #pragma clang fp contract(on)
 #include "gp_internal.hpp"
+#include "gp_modal.hpp"
 
 namespace gp {
 
@@ -96,6 +97,19 @@ __device__ inline double oc_xnew(double x, double dc, double lmid, double floor_
   double v = x * sqrt(fmax(-dc / lmid, floor_));
   v = fmin(fmax(v, 0.0), 1.0);
   return fmin(fmax(v, x - move), x + move);
+}
+
+// Index of the entry whose block range [off[i], off[i+1]) holds block blk.
+__device__ inline int find_prob(const int* __restrict__ off, int n, int blk) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= blk)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
 }
 
 // ------------------------------------------------------------- bodies
@@ -197,12 +211,48 @@ __device__ inline void body_pcg_init_fin(const Parts& parts, int nV, double tol,
 
 // Element energy and sensitivity, FP64 (simp_solver.py:215-218). With
 // want_sum the block's sum of E_e u_e^T KE u_e goes to partS[lb].
-template <typename T>
+// u_e^T KE u_e in the modal basis (KE = T Khat T^T / 64, coefficients of k
+// already divided by 64): with b = T^T u_e and v = Khat b / 64 the energy is
+// b.v, 45 products instead of the 576 of the dense form.
+__device__ inline double elem_energy_modal(const ModalCoef<double>& k, const double (&ue)[24]) {
+  Plane<double> pl[2];
+#pragma unroll
+  for (int cx = 0; cx < 2; ++cx)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      double c[2][2];
+#pragma unroll
+      for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+        for (int cz = 0; cz < 2; ++cz) c[cy][cz] = ue[3 * LOC[cx * 4 + cy * 2 + cz] + q];
+      const double a0 = c[0][0] + c[0][1], d0 = c[0][1] - c[0][0];
+      const double a1 = c[1][0] + c[1][1], d1 = c[1][1] - c[1][0];
+      pl[cx].P[q][0] = a0 + a1;
+      pl[cx].P[q][1] = d0 + d1;
+      pl[cx].P[q][2] = a1 - a0;
+      pl[cx].P[q][3] = d1 - d0;
+    }
+  double b0[3][4], b1[3][4], v0[3][4], v1[3][4];
+  element_modes(k, 1.0, pl[0], pl[1], b0, b1, v0, v1);
+  double ce = 0.0;
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      ce += b0[q][m] * v0[q][m];
+      ce += b1[q][m] * v1[q][m];
+    }
+  return ce;
+}
+
+// dc_e = -dE/dx u_e^T KE u_e. MODAL: the element table is the isotropic hex8
+// matrix held as modal coefficients mk; otherwise the dense table ke.
+template <typename T, bool MODAL>
 __device__ inline void body_sens(const Grid& g, const double* __restrict__ ke,
-                                 const double* __restrict__ xPhys, const T* __restrict__ u,
-                                 int law, double p, double E0, double Emin,
-                                 double* __restrict__ dc, int want_sum, double* partS,
-                                 DevScalars* sc, int lb, int nb) {
+                                 const ModalCoef<double>& mk, const double* __restrict__ xPhys,
+                                 const T* __restrict__ u, int law, double p, double E0,
+                                 double Emin, double* __restrict__ dc, int want_sum,
+                                 double* partS, DevScalars* sc, int lb, int nb) {
   double esum = 0.0;
   const int nloop = (g.ne + nb * kBlock - 1) / (nb * kBlock);
   for (int it = 0; it < nloop; ++it) {
@@ -221,12 +271,16 @@ __device__ inline void body_sens(const Grid& g, const double* __restrict__ ke,
       ue[3 * m + 2] = (double)v.z;
     }
     double ce = 0.0;
+    if (MODAL) {
+      ce = elem_energy_modal(mk, ue);
+    } else {
 #pragma unroll
-    for (int b = 0; b < 24; ++b) {
-      double t = 0.0;
+      for (int b = 0; b < 24; ++b) {
+        double t = 0.0;
 #pragma unroll
-      for (int a = 0; a < 24; ++a) t += ue[a] * ke[a * 24 + b];
-      ce += t * ue[b];
+        for (int a = 0; a < 24; ++a) t += ue[a] * ke[a * 24 + b];
+        ce += t * ue[b];
+      }
     }
     const double x = valid ? xPhys[e] : 0.0;
     const double dcv = -law_dE(x, law, p, E0, Emin) * ce;
@@ -305,11 +359,13 @@ __device__ inline void body_oc_init_fin(const Parts& parts, int nb, int variant,
 // Bisection step s (simp_solver.py:230-248). The interval at the start of
 // step s follows from step s-1's interval and its volume sum; the stop test
 // on (l1,l2) precedes the volume evaluation, as in the reference loop body.
-__device__ inline void body_oc_step(int s, int ne, const double* __restrict__ x,
+// Returns true when the multiplier is final after this step (every block of
+// the launch reaches the same answer).
+__device__ inline bool body_oc_step(int s, int ne, const double* __restrict__ x,
                                     const double* __restrict__ dc, const double* __restrict__ w,
                                     const uint8_t* __restrict__ pres, double n_pres,
                                     const Parts& parts, DevScalars* sc, int lb, int nb) {
-  if (sc->oc_done) return;
+  if (sc->oc_done) return true;
   double l1, l2;
   if (s == 0) {
     l1 = sc->ocl1[0];
@@ -338,9 +394,10 @@ __device__ inline void body_oc_step(int s, int ne, const double* __restrict__ x,
       sc->oc_steps = s + 1;
       sc->oc_done = 1;
     }
-    return;
+    return true;
   }
-  if (s + 1 >= sc->oc_max && lb == 0 && threadIdx.x == 0) {
+  const bool last = s + 1 >= sc->oc_max;
+  if (last && lb == 0 && threadIdx.x == 0) {
     // last permitted step: its lmid is final (the branch after it is unused)
     sc->lmid = lmid;
     sc->oc_steps = s + 1;
@@ -354,6 +411,7 @@ __device__ inline void body_oc_step(int s, int ne, const double* __restrict__ x,
   }
   acc = block_sum0(acc);
   if (threadIdx.x == 0) parts.O[s & 1][lb] = acc;
+  return last;
 }
 
 __device__ inline void body_oc_final(int ne, const double* __restrict__ x,

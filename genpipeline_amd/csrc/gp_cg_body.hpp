@@ -87,19 +87,21 @@ __device__ inline void block_sum3_to(double v0, double v1, double v2, double* p0
 // TJ+TK+1 halo columns with aa = 0 or bb = 0 are spread over the first
 // threads. Each plane's z goes to an LDS tile; element columns then read
 // their four corners from it.
+// Returns true when the solve is finished at this launch (converged, capped
+// or broken down at iteration k-1, or already finished): no work was done.
 template <typename T, int TJ, int TK>
-__device__ inline void body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
+__device__ inline bool body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
                                      const ModalCoef<T>& kc, const float* __restrict__ E,
                                      const T* __restrict__ rp, const T* __restrict__ wp,
                                      const T* __restrict__ sp, const T* __restrict__ minv,
                                      T* __restrict__ rn, T* __restrict__ wn, T* __restrict__ sn,
                                      T* __restrict__ x, T* __restrict__ p,
                                      const uint8_t* __restrict__ fixmask, DevScalars* sc,
-                                     const Parts& parts, int k, int lb, int nb) {
+                                     const Parts& parts, int k, int lb, int nb,
+                                     CornerLds<T, TJ * TK>& cv) {
   constexpr int NT = TJ * TK;
   constexpr int NH = TJ + TK + 1;  // halo node columns (aa = 0 or bb = 0)
-  if (sc->done) return;
-  __shared__ T cv[2][4][3][NT];
+  if (sc->done) return true;
   __shared__ T zs[2][3][TJ + 1][TK + 1];
   const int tid = threadIdx.x;
   const int a = tid / TK, b = tid - (tid / TK) * TK;
@@ -135,7 +137,7 @@ __device__ inline void body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
         sc->status = conv ? GP_OK : (brk ? GP_BREAKDOWN : GP_NOT_CONVERGED);
         sc->done = 1;
       }
-      return;
+      return true;
     }
     const double alp = gam / den;
     if (lb == 0 && tid == 0) {
@@ -235,6 +237,7 @@ __device__ inline void body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
     own_cur = own_nxt;
   }
   block_sum3_to<NT>(gk, dk, rk, parts.G[k & 1] + lb, parts.D[k & 1] + lb, parts.R[k & 1] + lb);
+  return false;
 }
 
 }  // namespace gp

@@ -37,8 +37,9 @@ __global__ __launch_bounds__(TJ* TK) void k_cg_modal(Grid g, int NI, int ntj, in
                                                     T* __restrict__ p,
                                                     const uint8_t* __restrict__ fixmask,
                                                     DevScalars* sc, Parts parts, int k) {
+  __shared__ CornerLds<T, TJ * TK> cv;
   body_cg_modal<T, TJ, TK>(g, NI, ntj, ntk, kc, E, rp, wp, sp, minv, rn, wn, sn, x, p, fixmask, sc,
-                           parts, k, blockIdx.x, gridDim.x);
+                           parts, k, blockIdx.x, gridDim.x, cv);
 }
 
 template <int TJ, int TK>

@@ -75,6 +75,7 @@ struct DevScalars {
 struct BatchProb {
   Grid g;
   ModalCoef<float> mc;
+  ModalCoef<double> mc64;
   int ni, ntj, ntk;             // modal sweep geometry for the batch tile
   int law, oc_variant;
   double penal, E0, Emin, move, volfrac, n_pres;
@@ -120,6 +121,68 @@ void batch_post_solve(const BatchLaunch& bl, int oc_max_steps, hipStream_t st);
 void batch_compliance(const BatchLaunch& bl, hipStream_t st);
 void batch_cast(const BatchLaunch& bl, int dir, hipStream_t st);
 void batch_status(const BatchLaunch& bl, BatchStatus* out, hipStream_t st);
+
+// ------------------------------------------------------------ scheduler
+// A slot holds one problem and a phase machine; one tick launch advances every
+// slot by one phase. Phases run the same device bodies, with the same block
+// counts per kind, as the single-problem kernels, so a problem's results do
+// not depend on the scheduler or on the other slots.
+enum SchedPhase : int {
+  PH_IDLE = 0,   // free slot
+  PH_CAST_IN,    // u32 <- (float) u64 (warm start from the caller's u)
+  PH_STIFF,      // E = law(xPhys)
+  PH_DIAG,       // Jacobi M^-1
+  PH_KX0,        // w0 = K u (warm start)
+  PH_CGINIT,     // r = f - K x0, PCG vectors
+  PH_INITFIN,    // set-up sums, initial convergence test
+  PH_CG,         // one PCG iteration (k)
+  PH_SENS,       // dc
+  PH_FILT,       // dcf = H^T (dc / Hs); NaN check
+  PH_OCMAX,      // max(-dcf) partials (GPU OC constants)
+  PH_OCINIT,     // bisection interval
+  PH_OCSTEP,     // bisection step (sub)
+  PH_OCFINAL,    // xnew at the final multiplier
+  PH_COMMIT,     // xPhys_new = H xnew / Hs, preserved, x <- xnew, xPhys <- xPhys_new
+  PH_DOT,        // f.u partials
+  PH_SUMFIN,     // compliance; early-stop test
+  PH_CAST_OUT,   // u64 <- (double) u32
+  PH_DONE
+};
+enum SolveKind : int { SK_OC = 0, SK_COMPL = 1, SK_FINAL = 2 };
+
+struct SchedCfg {
+  int n_iters, max_iter, tol_schedule, compl_each, early_stop, warm_start;
+  double tol;
+};
+
+struct SchedState {
+  int phase, k, oc_it, sub;
+  int solve, small, status, pcg_total;
+  int bis_total, nan, pad0, pad1;
+  double prev_c, curr_c;
+};
+
+// A slot's block range holds as many blocks as its job's PCG sweep uses
+// (nb_cg). Element/node phases spread over those blocks; phases whose block
+// count fixes a partial-sum layout (nb_vec, nb_oc) run that many virtual
+// blocks on them, so partial sums keep the single-problem layout.
+struct SchedSlot {
+  BatchProb P;
+  SchedCfg cfg;
+  int nb_cg, nb_node, nb_elem, nb_vec, nb_oc, cap;
+  SchedState st[2];  // by tick parity: tick t reads st[t&1], writes st[(t+1)&1]
+};
+
+struct SchedTable {
+  SchedSlot* slots;
+  int n;
+  const int* off;      // off[0..n]: slot s owns blocks [off[s], off[s+1])
+  const int* b2s;      // slot of every block of the tick
+  SchedState* status;  // state after the last tick, one per slot
+};
+
+// Enqueues one tick with parity t & 1 (tj x tk must be 256 threads).
+bool sched_tick(const SchedTable& tab, int tj, int tk, int blocks, int t, hipStream_t st);
 
 __device__ inline double wave_sum64(double v) {
 #pragma unroll
@@ -181,10 +244,11 @@ void launch_cast_f32_to_f64(const float* a, double* b, long long n, hipStream_t 
 
 void launch_stiffness(const Grid& g, const double* xPhys, float* E, int law, double penal,
                       double E0, double Emin, hipStream_t st);
+// mk: modal coefficients of an isotropic hex8 table (nullptr: dense ke64)
 template <typename T>
-void launch_sens(const Grid& g, const double* ke64, const double* xPhys, const T* u, int law,
-                 double penal, double E0, double Emin, double* dc, double* ce_sum,
-                 Parts parts, DevScalars* sc, hipStream_t st);
+void launch_sens(const Grid& g, const double* ke64, const ModalCoef<double>* mk,
+                 const double* xPhys, const T* u, int law, double penal, double E0, double Emin,
+                 double* dc, double* ce_sum, Parts parts, DevScalars* sc, hipStream_t st);
 // `s` points to a Stencil in device memory.
 void launch_filter(const Grid& g, const Stencil* s, const double* Hs, const double* in,
                    double* out, int mode, hipStream_t st);

@@ -359,22 +359,28 @@ void launch_stiffness(const Grid& g, const double* xPhys, float* E, int law, dou
 }
 
 // ----------------------------------------------------------- sensitivity
-template <typename T>
+template <typename T, bool MODAL>
 __global__ __launch_bounds__(kBlock) void k_sens(Grid g, const double* __restrict__ ke,
+                                                ModalCoef<double> mk,
                                                 const double* __restrict__ xPhys,
                                                 const T* __restrict__ u, int law, double p,
                                                 double E0, double Emin, double* __restrict__ dc,
                                                 int want_sum, double* partS, DevScalars* sc) {
-  body_sens<T>(g, ke, xPhys, u, law, p, E0, Emin, dc, want_sum, partS, sc, blockIdx.x,
-               gridDim.x);
+  body_sens<T, MODAL>(g, ke, mk, xPhys, u, law, p, E0, Emin, dc, want_sum, partS, sc, blockIdx.x,
+                      gridDim.x);
 }
 template <typename T>
-void launch_sens(const Grid& g, const double* ke64, const double* xPhys, const T* u, int law,
-                 double penal, double E0, double Emin, double* dc, double* ce_sum, Parts parts,
-                 DevScalars* sc, hipStream_t st) {
+void launch_sens(const Grid& g, const double* ke64, const ModalCoef<double>* mk,
+                 const double* xPhys, const T* u, int law, double penal, double E0, double Emin,
+                 double* dc, double* ce_sum, Parts parts, DevScalars* sc, hipStream_t st) {
   const int nb = grid_blocks(g.ne);  // parts.cap >= nb
-  hipLaunchKernelGGL((k_sens<T>), dim3(nb), dim3(kBlock), 0, st, g, ke64, xPhys, u, law, penal,
-                     E0, Emin, dc, ce_sum != nullptr ? 1 : 0, parts.S, sc);
+  const ModalCoef<double> m = mk ? *mk : ModalCoef<double>{};
+  if (mk)
+    hipLaunchKernelGGL((k_sens<T, true>), dim3(nb), dim3(kBlock), 0, st, g, ke64, m, xPhys, u,
+                       law, penal, E0, Emin, dc, ce_sum != nullptr ? 1 : 0, parts.S, sc);
+  else
+    hipLaunchKernelGGL((k_sens<T, false>), dim3(nb), dim3(kBlock), 0, st, g, ke64, m, xPhys, u,
+                       law, penal, E0, Emin, dc, ce_sum != nullptr ? 1 : 0, parts.S, sc);
   if (ce_sum != nullptr)
     hipLaunchKernelGGL(k_sum_fin, dim3(1), dim3(kBlock), 0, st, parts.S, nb, ce_sum);
 }
@@ -635,12 +641,12 @@ template void launch_pcg_c<double>(const Grid&, int, double*, const double*, Par
                                    hipStream_t);
 template void launch_add_correction<float>(const Grid&, double*, const float*, hipStream_t);
 template void launch_add_correction<double>(const Grid&, double*, const double*, hipStream_t);
-template void launch_sens<float>(const Grid&, const double*, const double*, const float*, int,
-                                 double, double, double, double*, double*, Parts, DevScalars*,
-                                 hipStream_t);
-template void launch_sens<double>(const Grid&, const double*, const double*, const double*, int,
-                                  double, double, double, double*, double*, Parts, DevScalars*,
-                                  hipStream_t);
+template void launch_sens<float>(const Grid&, const double*, const ModalCoef<double>*,
+                                 const double*, const float*, int, double, double, double,
+                                 double*, double*, Parts, DevScalars*, hipStream_t);
+template void launch_sens<double>(const Grid&, const double*, const ModalCoef<double>*,
+                                  const double*, const double*, int, double, double, double,
+                                  double*, double*, Parts, DevScalars*, hipStream_t);
 template void launch_dot<double, float>(const double*, const float*, long long, Parts,
                                         DevScalars*, hipStream_t);
 template void launch_dot<double, double>(const double*, const double*, long long, Parts,

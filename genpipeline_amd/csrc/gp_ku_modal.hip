@@ -31,8 +31,9 @@ __global__ __launch_bounds__(TJ* TK) void k_ku_modal(Grid g, int NI, int ntj, in
                                                     const uint8_t* __restrict__ fixmask,
                                                     DevScalars* sc, Parts parts, int it,
                                                     int nV) {
+  __shared__ CornerLds<T, TJ * TK> cv;
   body_ku_modal<T, TJ, TK, MODE>(g, NI, ntj, ntk, kc, E, u, u2, y, pout, fixmask, sc, parts, it,
-                                 nV, blockIdx.x);
+                                 nV, blockIdx.x, cv);
 }
 
 template <int TJ, int TK>

@@ -10,6 +10,11 @@
 
 namespace gp {
 
+// LDS staging of element-corner contributions, [buffer][corner group][component][thread].
+// Declared by the calling kernel so that kernels running several bodies hold one copy.
+template <typename T, int NT>
+using CornerLds = T[2][4][3][NT];
+
 template <typename T>
 struct Plane {   // 2-D modal coefficients of one node plane of an element column
   T P[3][4];     // [component][mj*2+mk]
@@ -67,13 +72,13 @@ __device__ inline void transform(const Raw<T, PCG>& r, T beta, Plane<T>& out, T 
   }
 }
 
-// Element between planes `lo` (i) and `hi` (i+1): returns the 2-D modal
-// outputs of its bottom (plane i) and top (plane i+1) faces.
+// Modal inputs b = T^T u_e (b0: mode mi = 0, b1: mi = 1, [component][m2])
+// and outputs v = Ee Khat b / 64 of the element between planes `lo` (i) and
+// `hi` (i+1).
 template <typename T>
-__device__ inline void element(const ModalCoef<T>& k, T Ee, const Plane<T>& lo, const Plane<T>& hi,
-                               Plane<T>& bot, Plane<T>& top) {
-  // b[mi][q][m2]
-  T b0[3][4], b1[3][4];
+__device__ inline void element_modes(const ModalCoef<T>& k, T Ee, const Plane<T>& lo,
+                                     const Plane<T>& hi, T (&b0)[3][4], T (&b1)[3][4],
+                                     T (&v0)[3][4], T (&v1)[3][4]) {
 #pragma unroll
   for (int q = 0; q < 3; ++q)
 #pragma unroll
@@ -101,7 +106,6 @@ __device__ inline void element(const ModalCoef<T>& k, T Ee, const Plane<T>& lo, 
   const T s3 = bXYz + bYZx + bXZy;
   const T vXYz = T1 * bXYz + T2 * s3, vYZx = T1 * bYZx + T2 * s3, vXZy = T1 * bXZy + T2 * s3;
   // v[mi][q][m2]; v[0][q][0] is the rigid translation (zero output)
-  T v0[3][4], v1[3][4];
   v0[0][0] = 0; v0[1][0] = 0; v0[2][0] = 0;
   v1[0][0] = vxx; v1[1][0] = sxy; v1[2][0] = sxz;      // mode x
   v0[0][2] = sxy; v0[1][2] = vyy; v0[2][2] = syz;      // mode y
@@ -110,6 +114,15 @@ __device__ inline void element(const ModalCoef<T>& k, T Ee, const Plane<T>& lo, 
   v0[0][3] = vYZx; v0[1][3] = vYZy; v0[2][3] = vYZz;   // mode yz
   v1[0][1] = vXZx; v1[1][1] = vXZy; v1[2][1] = vXZz;   // mode xz
   v1[0][3] = X * bXYZx; v1[1][3] = X * bXYZy; v1[2][3] = X * bXYZz;
+}
+
+// Element between planes `lo` (i) and `hi` (i+1): returns the 2-D modal
+// outputs of its bottom (plane i) and top (plane i+1) faces.
+template <typename T>
+__device__ inline void element(const ModalCoef<T>& k, T Ee, const Plane<T>& lo, const Plane<T>& hi,
+                               Plane<T>& bot, Plane<T>& top) {
+  T b0[3][4], b1[3][4], v0[3][4], v1[3][4];
+  element_modes(k, Ee, lo, hi, b0, b1, v0, v1);
 #pragma unroll
   for (int q = 0; q < 3; ++q)
 #pragma unroll
@@ -188,11 +201,11 @@ __device__ inline void body_ku_modal(const Grid& g, int NI, int ntj, int ntk,
                                      const T* __restrict__ u, const T* __restrict__ u2,
                                      T* __restrict__ y, T* __restrict__ pout,
                                      const uint8_t* __restrict__ fixmask, DevScalars* sc,
-                                     const Parts& parts, int it, int nV, int lb) {
+                                     const Parts& parts, int it, int nV, int lb,
+                                     CornerLds<T, TJ * TK>& cv) {
   constexpr int NT = TJ * TK;
   constexpr bool PCG = MODE == 1;
   if (PCG && sc->done) return;
-  __shared__ T cv[2][4][3][NT];
   const int tid = threadIdx.x;
   const int a = tid / TK, b = tid - (tid / TK) * TK;
   int bid = lb;

@@ -1,0 +1,277 @@
+// Device-side scheduler: many independent problems, one launch per tick.
+//
+// Every slot carries a small state machine (SchedState) that walks the SIMP
+// loop phase by phase: material law, Jacobi set-up, PCG iterations, sensitivity,
+// filter, OC bisection, commit, compliance. One tick launch runs, for every
+// slot, the phase its state names, on the slot's own block range, and block 0
+// of the slot writes the state for the next tick. Slots therefore progress
+// independently: a 32^3 problem does not wait for a 64^3 problem's PCG, and a
+// slot that finishes can be refilled by the host between ticks.
+//
+// The state is double-buffered by tick parity: tick t reads st[t & 1] and
+// writes st[(t + 1) & 1], so no block reads a state another block of the same
+// launch is writing. Everything a phase needs from an earlier phase was
+// written by an earlier launch.
+//
+// A slot's job uses as many blocks as its PCG sweep (nb_cg). Element and
+// node phases spread over them; phases that leave per-block partial sums run
+// the single-problem kernel's block count as virtual blocks on them, so every
+// partial sum covers the same index set in the same order and a problem's
+// results are bitwise those of Problem.run with the same K.u tile.
+#include "gp_cg_body.hpp"
+
+namespace gp {
+
+__device__ inline double sched_tol(const SchedState& s, const SchedCfg& c) {
+  if (s.solve == SK_COMPL) return c.tol < 1e-6 ? c.tol : 1e-6;
+  if (s.solve == SK_OC && c.tol_schedule)  // simp_solver_gpu.py:97-103
+    return s.oc_it < 3 ? 1e-4 : (s.oc_it < 8 ? 1e-5 : 1e-6);
+  return c.tol;
+}
+
+__device__ inline bool sched_use_x0(const SchedState& s, const SchedCfg& c) {
+  return c.warm_start != 0 && s.solve != SK_COMPL;
+}
+
+// State after a finished OC update (iteration count oc_it already advanced).
+__device__ inline void sched_after_update(SchedState& nx, const SchedCfg& c) {
+  nx.phase = PH_STIFF;
+  if (c.compl_each)
+    nx.solve = SK_COMPL;
+  else
+    nx.solve = nx.oc_it < c.n_iters ? SK_OC : SK_FINAL;
+}
+
+// Occupancy floor of 4 waves per SIMD: the 128-register budget the PCG sweep
+// fits in, kept for every phase of the kernel.
+// Next state of a slot after tick work; run by the slot's lead thread only.
+// fin: the PCG solve (PH_CG) or the bisection (PH_OCSTEP) finished this tick.
+__device__ inline SchedState sched_next(const SchedState& cur, const SchedCfg& C,
+                                        const BatchProb& P, bool fin) {
+  SchedState nx = cur;
+  switch (cur.phase) {
+    case PH_CAST_IN:
+      nx.phase = PH_STIFF;
+      break;
+    case PH_STIFF:
+      nx.phase = PH_DIAG;
+      break;
+    case PH_DIAG:
+      nx.phase = sched_use_x0(cur, C) ? PH_KX0 : PH_CGINIT;
+      break;
+    case PH_KX0:
+      nx.phase = PH_CGINIT;
+      break;
+    case PH_CGINIT:
+      nx.phase = PH_INITFIN;
+      break;
+    case PH_INITFIN:
+      nx.phase = PH_CG;
+      nx.k = 0;
+      break;
+    case PH_CG:
+      if (fin) {
+        nx.pcg_total += P.sc->iter;
+        if (P.sc->status > nx.status) nx.status = P.sc->status;
+        nx.phase = cur.solve == SK_OC ? PH_SENS : PH_DOT;
+      } else {
+        nx.k = cur.k + 1;
+      }
+      break;
+    case PH_SENS:
+      nx.phase = PH_FILT;
+      break;
+    case PH_FILT:
+      if (P.sc_oc->nan_flag) {
+        // simp_solver_gpu.py:106-108: stop before the update
+        nx.status = GP_ERR_NAN;
+        nx.nan = 1;
+        if (C.compl_each && cur.oc_it > 0) {
+          nx.phase = PH_CAST_OUT;
+        } else {
+          nx.phase = PH_STIFF;
+          nx.solve = SK_FINAL;
+        }
+      } else {
+        nx.phase = PH_OCMAX;
+      }
+      break;
+    case PH_OCMAX:
+      nx.phase = PH_OCINIT;
+      break;
+    case PH_OCINIT:
+      nx.phase = PH_OCSTEP;
+      nx.sub = 0;
+      break;
+    case PH_OCSTEP:
+      if (fin) {
+        nx.bis_total += P.sc_oc->oc_steps;
+        nx.phase = PH_OCFINAL;
+      } else {
+        nx.sub = cur.sub + 1;
+      }
+      break;
+    case PH_OCFINAL:
+      nx.phase = PH_COMMIT;
+      break;
+    case PH_COMMIT:
+      nx.oc_it = cur.oc_it + 1;
+      sched_after_update(nx, C);
+      break;
+    case PH_DOT:
+      nx.phase = PH_SUMFIN;
+      break;
+    case PH_SUMFIN: {
+      const double c = P.sc_oc->sum;
+      nx.curr_c = c;
+      if (cur.solve == SK_COMPL) {
+        // simp_solver_gpu.py:110-125
+        bool stop = false;
+        if (C.early_stop) {
+          if (isfinite(cur.prev_c)) {
+            const double ch = fabs(cur.prev_c - c) / (cur.prev_c + 1e-10);
+            nx.small = ch < 0.01 ? cur.small + 1 : 0;
+            stop = nx.small >= 3;
+          }
+          if (!stop) nx.prev_c = c;
+        }
+        if (stop || cur.oc_it >= C.n_iters) {
+          nx.phase = PH_CAST_OUT;
+        } else {
+          nx.phase = PH_STIFF;
+          nx.solve = SK_OC;
+        }
+      } else {
+        nx.phase = PH_CAST_OUT;
+      }
+      break;
+    }
+    case PH_CAST_OUT:
+      nx.phase = PH_DONE;
+      break;
+    default:
+      break;
+  }
+  return nx;
+}
+
+// Occupancy floor of 4 waves per SIMD: the 128-register budget the PCG sweep
+// fits in, kept for every phase of the kernel.
+template <int TJ, int TK>
+__global__ __launch_bounds__(TJ* TK, 4) void k_tick(SchedTable tab, int t) {
+  static_assert(TJ * TK == kBlock, "the scheduler runs every body with kBlock threads");
+  __shared__ CornerLds<float, TJ * TK> cv;
+  const int b = tab.b2s[blockIdx.x];
+  SchedSlot& S = tab.slots[b];
+  const int lb = blockIdx.x - tab.off[b];
+  const SchedState& cur = S.st[t & 1];
+  const BatchProb& P = S.P;
+  const int phase = cur.phase;
+  const int nb = S.nb_cg;  // blocks of this job; the rest of the range idles
+  bool fin = false;
+
+  switch (lb < nb ? phase : PH_IDLE) {
+    case PH_CAST_IN:
+      for (long long d = lb * (long long)kBlock + threadIdx.x; d < P.g.ndof;
+           d += (long long)nb * kBlock)
+        P.u[d] = (float)P.u64[d];
+      break;
+    case PH_STIFF:
+      body_stiffness(P.g.ne, P.xPhys, P.E, P.law, P.penal, P.E0, P.Emin, lb, nb);
+      break;
+    case PH_DIAG:
+      body_diag<float>(P.g, P.ke32, P.E, P.fixmask, P.minv, 1, lb, nb);
+      break;
+    case PH_KX0:
+      body_ku_modal<float, TJ, TK, 0>(P.g, P.ni, P.ntj, P.ntk, P.mc, P.E, P.u, nullptr, P.wb[0],
+                                      nullptr, P.fixmask, P.sc, P.parts, 0, 0, lb, cv);
+      break;
+    case PH_CGINIT: {
+      const int x0 = sched_use_x0(cur, S.cfg) ? 1 : 0;
+      for (int v = lb; v < S.nb_vec; v += nb)
+        body_cg_init<float>(P.g.nn, P.f, P.wb[0], x0, P.u, P.rb[1], P.sb[0], P.wb[1], P.pc,
+                            P.minv, P.parts, v, S.nb_vec);
+      if (lb == 0 && threadIdx.x == 0) P.sc_oc->nan_flag = 0;
+      break;
+    }
+    case PH_INITFIN:
+      if (lb == 0)
+        body_pcg_init_fin(P.parts, S.nb_vec, sched_tol(cur, S.cfg), S.cfg.max_iter, P.sc);
+      break;
+    case PH_CG: {
+      const int k = cur.k, c = k & 1, o = c ^ 1;
+      fin = body_cg_modal<float, TJ, TK>(P.g, P.ni, P.ntj, P.ntk, P.mc, P.E, P.rb[o], P.wb[o],
+                                         P.sb[c], P.minv, P.rb[c], P.wb[c], P.sb[o], P.u, P.pc,
+                                         P.fixmask, P.sc, P.parts, k, lb, nb, cv);
+      break;
+    }
+    case PH_SENS:
+      body_sens<float, true>(P.g, P.ke64, P.mc64, P.xPhys, P.u, P.law, P.penal, P.E0, P.Emin,
+                             P.dc, 0, nullptr, P.sc_oc, lb, nb);
+      break;
+    case PH_FILT:
+      body_filter(P.g, P.st, P.Hs, P.dc, P.dcf, 1, lb, nb);
+      break;
+    case PH_OCMAX:
+      if (P.oc_variant == GP_OC_GPU)
+        for (int v = lb; v < S.nb_oc; v += nb) body_oc_max(P.g.ne, P.dcf, P.parts.M, v, S.nb_oc);
+      break;
+    case PH_OCINIT:
+      if (lb == 0) body_oc_init_fin(P.parts, S.nb_oc, P.oc_variant, P.volfrac, P.move, P.sc_oc);
+      break;
+    case PH_OCSTEP:
+      // every virtual block reaches the same decision; the lead's is used
+      for (int v = lb; v < S.nb_oc; v += nb) {
+        const bool f = body_oc_step(cur.sub, P.g.ne, P.x, P.dcf, P.w, P.pres, P.n_pres, P.parts,
+                                    P.sc_oc, v, S.nb_oc);
+        if (v == lb) fin = f;
+      }
+      break;
+    case PH_OCFINAL:
+      body_oc_final(P.g.ne, P.x, P.dcf, P.pres, P.sc_oc, P.xnew, lb, nb);
+      break;
+    case PH_COMMIT:
+      // every element is filtered, pinned and committed by the one thread
+      // that owns it; the stencil reads only xnew, complete since last tick
+      body_filter(P.g, P.st, P.Hs, P.xnew, P.xphys_new, 0, lb, nb);
+      if (P.pres != nullptr) body_set_preserved(P.g.ne, P.pres, P.xphys_new, lb, nb);
+      for (int e = lb * kBlock + threadIdx.x; e < P.g.ne; e += nb * kBlock) {
+        P.x[e] = P.xnew[e];
+        P.xPhys[e] = P.xphys_new[e];
+      }
+      break;
+    case PH_DOT:
+      for (int v = lb; v < S.nb_vec; v += nb)
+        body_dot<double, float>(P.g.ndof, P.f, P.u, P.parts.S, v, S.nb_vec);
+      break;
+    case PH_SUMFIN:
+      if (lb == 0) body_sum_fin(P.parts.S, S.nb_vec, &P.sc_oc->sum);
+      break;
+    case PH_CAST_OUT:
+      for (long long d = lb * (long long)kBlock + threadIdx.x; d < P.g.ndof;
+           d += (long long)nb * kBlock)
+        P.u64[d] = (double)P.u[d];
+      break;
+    default:
+      break;
+  }
+  if (lb == 0 && threadIdx.x == 0) {
+    const SchedState nx = sched_next(S.st[t & 1], S.cfg, P, fin);
+    S.st[(t + 1) & 1] = nx;
+    tab.status[b] = nx;
+  }
+}
+
+bool sched_tick(const SchedTable& tab, int tj, int tk, int blocks, int t, hipStream_t st) {
+  if (tj == 8 && tk == 32) {
+    hipLaunchKernelGGL((k_tick<8, 32>), dim3(blocks), dim3(kBlock), 0, st, tab, t);
+    return true;
+  }
+  if (tj == 16 && tk == 16) {
+    hipLaunchKernelGGL((k_tick<16, 16>), dim3(blocks), dim3(kBlock), 0, st, tab, t);
+    return true;
+  }
+  return false;
+}
+
+}  // namespace gp

@@ -222,6 +222,69 @@ int gp_batch_simp_run(gp_batch* b, const gp_simp_opts* opts, const double* volfr
                       double* const* x, double* const* xPhys, double* const* u,
                       gp_simp_stats* stats, void* stream);
 
+/* ---- device scheduler: independent problems, continuous batching ------ *
+ * No counterpart in the reference (it runs one solve per call from Python
+ * threads: voxel_fem.py:559-611, topo_data_gen.py:235-307). A scheduler has
+ * n slots; each slot runs one job (a SIMP run or a single solve) on one
+ * problem. One tick launch advances every occupied slot by one phase of its
+ * job (material law, Jacobi set-up, one PCG iteration, sensitivity, filter,
+ * one OC bisection step, commit, compliance), so slots progress
+ * independently and a finished slot can be released and refilled between
+ * ticks while the others keep running. Results are bitwise those of
+ * gp_simp_run with the same K.u tile (tj, tk, ni), whatever else runs.   */
+typedef struct gp_sched gp_sched;
+
+typedef struct gp_job {
+  int32_t n_iters;            /* OC iterations; 0: one solve (compliance)   */
+  double volfrac;
+  double tol;                 /* PCG relative tolerance (FP32 vectors)      */
+  int32_t max_iter;           /* PCG iteration cap per solve                */
+  int32_t tol_schedule;       /* as gp_simp_opts                            */
+  int32_t early_stop;
+  int32_t compliance_each_iter;
+  int32_t warm_start;         /* first solve starts from u; later solves    */
+                              /* from the previous solution                */
+} gp_job;
+
+typedef struct gp_slot_status {
+  int32_t phase;              /* 0 idle, 18 done, else running              */
+  int32_t done;
+  int32_t iters_done;         /* OC iterations completed                    */
+  int32_t pcg_iters_total;
+  int32_t bisect_steps_total;
+  int32_t status;             /* worst PCG status, or GP_ERR_NAN            */
+  double compliance;          /* final f.u once done                        */
+} gp_slot_status;
+
+/* slot_dims: 4 ints per slot (nx, ny, nz, ni): the largest grid the slot
+ * accepts and the modal sweep depth its jobs use. (tj, tk) = (8, 32) or
+ * (16, 16). The device is the current one.                               */
+int gp_sched_create(int32_t n_slots, const int32_t* slot_dims, int32_t tj, int32_t tk,
+                    gp_sched** out);
+int gp_sched_destroy(gp_sched* s);
+/* Blocks of one slot's range (slot >= 0) or of the whole tick (slot -1). */
+int gp_sched_blocks(gp_sched* s, int32_t slot, int32_t* blocks);
+/* Starts a job in a free slot. x, xPhys (float64 n_elem) and u (float64
+ * n_dof) are device buffers the job reads and updates in place, as in
+ * gp_simp_run; they and the problem must stay alive until the slot is
+ * released. Stream-ordered: takes effect at the next tick on `stream`.   */
+int gp_sched_admit(gp_sched* s, int32_t slot, gp_problem* p, const gp_job* job, double* x,
+                   double* xPhys, double* u, void* stream);
+/* Frees a slot (its job need not be done). Stream-ordered.               */
+int gp_sched_release(gp_sched* s, int32_t slot, void* stream);
+/* Captures `ticks` (even) tick launches in a HIP graph that later ticks
+ * replay; 0 goes back to one launch per tick.                            */
+int gp_sched_set_graph(gp_sched* s, int32_t ticks);
+/* Enqueues n ticks on `stream` (asynchronous).                           */
+int gp_sched_tick(gp_sched* s, int64_t n_ticks, void* stream);
+/* Copies every slot's status (n_slots entries) after the ticks enqueued
+ * so far; synchronises `stream`.                                          */
+int gp_sched_status(gp_sched* s, gp_slot_status* out, void* stream);
+/* Ticks until every occupied slot is done or max_ticks have run; out
+ * (n_slots entries, may be NULL) receives the final statuses.            */
+int gp_sched_run(gp_sched* s, int64_t max_ticks, int64_t* ticks_run, gp_slot_status* out,
+                 void* stream);
+
 /* ---- generic-table kernels (reference kernel API) --------------------- *
  * Explicit KE (24x24 float32 row-major) and edof (int32 n_elem x 24) as the
  * reference kernels take them; edof may be arbitrary (no grid assumption).

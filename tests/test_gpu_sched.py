@@ -1,0 +1,144 @@
+"""Device scheduler: a job gives bitwise the results of Problem.run with the
+same K.u tile, alone, in a mixed batch, admitted while other slots run, and
+with ticks replayed from a HIP graph."""
+import numpy as np
+import pytest
+import torch
+
+from genpipeline_amd import Problem, ProblemSpec
+from genpipeline_amd.bcs import clamp_x0
+from genpipeline_amd.problems import point_load_vector
+from oracle import simp_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(dev, nx, ny, nz, seed, volfrac=0.3, law="emin", oc="cpu", penal=3.0, pres=False):
+    rng = np.random.default_rng(seed)
+    f = point_load_vector(nx, ny, nz, rng, 2)
+    spec = ProblemSpec(nx, ny, nz, volfrac=volfrac, law=law, oc=oc, penal=penal)
+    mask = None
+    if pres:
+        mask = np.zeros((nx, ny, nz), bool)
+        mask[nx - 2:, :, :2] = True
+    return Problem(spec, clamp_x0(nx, ny, nz), f, preserved=mask, device=dev)
+
+
+def _alone(P, tile, ni, n_iters, **kw):
+    P.set_ku_tile(tile[0], tile[1], ni, precision="fp32")
+    return P.run(n_iters, precision="fp32", **kw)
+
+
+def _same(a, b):
+    assert torch.equal(a.x, b.x) and torch.equal(a.xPhys, b.xPhys) and torch.equal(a.u, b.u)
+    assert a.compliance == b.compliance
+    assert a.pcg_iters == b.pcg_iters and a.bisect_steps == b.bisect_steps
+    assert a.iters_done == b.iters_done and a.status == b.status
+
+
+@pytest.mark.parametrize("tile", [(8, 32), (16, 16)])
+def test_single_slot_equals_problem_run(gpu, tile):
+    from genpipeline_amd.sched import Scheduler
+    P = _problem(gpu, 12, 6, 5, 1, pres=True)
+    ref = _alone(P, tile, 2, 4, tol=1e-6, max_iter=5000, warm_start=True)
+    S = Scheduler([(12, 6, 5, 2)], tile=tile)
+    S.admit(0, P, n_iters=4, tol=1e-6, max_iter=5000, warm_start=True)
+    ticks, st = S.run()
+    assert st[0].done and ticks > ref.pcg_iters
+    _same(S.result(0, st[0]), ref)
+
+
+def test_mixed_batch_each_equals_alone(gpu):
+    from genpipeline_amd.sched import run_batch
+    shapes = [(10, 5, 5), (16, 8, 8), (8, 8, 8), (20, 6, 6), (12, 10, 4)]
+    Ps = [_problem(gpu, *s, seed=i, volfrac=0.25 + 0.03 * i,
+                   oc="gpu" if i % 2 else "cpu", law="clamp" if i == 3 else "emin")
+          for i, s in enumerate(shapes)]
+    refs = [_alone(P, (8, 32), 2, 3, tol=1e-6, max_iter=5000, warm_start=True) for P in Ps]
+    res, ticks = run_batch(Ps, 3, ni=2, graph_ticks=0, tol=1e-6, max_iter=5000)
+    for a, b in zip(res, refs):
+        _same(a, b)
+
+
+def test_admission_while_running_and_graph_replay(gpu):
+    """A job admitted into a free slot after other slots have run for a
+    while, and ticks replayed from a captured graph (several replays with
+    eager host work in between), still give the alone results."""
+    from genpipeline_amd.sched import Scheduler
+    A = _problem(gpu, 14, 6, 6, 3)
+    B = _problem(gpu, 10, 8, 6, 4, volfrac=0.35)
+    Cp = _problem(gpu, 12, 6, 6, 5, oc="gpu")
+    ra = _alone(A, (8, 32), 2, 3, tol=1e-6, max_iter=5000)
+    rb = _alone(B, (8, 32), 2, 2, tol=1e-6, max_iter=5000)
+    rc = _alone(Cp, (8, 32), 2, 2, tol=1e-6, max_iter=5000)
+    for graph in (0, 16):
+        S = Scheduler([(16, 8, 8, 2)] * 3, graph_ticks=graph)
+        S.admit(0, A, n_iters=3, tol=1e-6, max_iter=5000)
+        S.tick(37)
+        S.admit(1, B, n_iters=2, tol=1e-6, max_iter=5000)
+        S.tick(64)
+        st = S.status()
+        assert not st[0].done and st[2].phase == 0
+        S.admit(2, Cp, n_iters=2, tol=1e-6, max_iter=5000)
+        for _ in range(3):  # replays interleaved with host-side checks
+            S.tick(32)
+            torch.cuda.synchronize()
+            assert all(s.phase in range(0, 19) for s in S.status())
+        _, st = S.run()
+        assert all(s.done for s in st)
+        _same(S.result(0, st[0]), ra)
+        _same(S.result(1, st[1]), rb)
+        _same(S.result(2, st[2]), rc)
+        # a released slot can be refilled with a new job
+        S.release(1)
+        S.admit(1, B, n_iters=2, tol=1e-6, max_iter=5000)
+        _, st = S.run()
+        _same(S.result(1, st[1]), rb)
+        S.close()
+
+
+def test_gpu_solver_semantics_job(gpu):
+    """Tolerance schedule, compliance solve after every update and early
+    stop (SIMPSolverGPU.run) equal gp_simp_run with the same options."""
+    from genpipeline_amd.sched import Scheduler
+    P = _problem(gpu, 12, 6, 6, 7, law="clamp", oc="gpu", volfrac=0.35)
+    kw = dict(tol=1e-6, max_iter=5000, tol_schedule=True, early_stop=True,
+              compliance_each_iter=True, warm_start=True)
+    ref = _alone(P, (8, 32), 2, 25, **kw)
+    assert ref.iters_done < 25  # the early stop fired
+    S = Scheduler([(12, 6, 6, 2)])
+    S.admit(0, P, n_iters=25, **kw)
+    _, st = S.run()
+    _same(S.result(0, st[0]), ref)
+
+
+def test_single_solve_job(gpu):
+    """n_iters = 0: one solve at the given density; compliance f.u."""
+    from genpipeline_amd.sched import Scheduler
+    P = _problem(gpu, 10, 6, 6, 8, law="clamp", penal=1.0)
+    dens = np.random.default_rng(2).uniform(0.01, 1.0, P.spec.n_elem)
+    ref = _alone(P, (8, 32), 2, 0, x0=dens, xphys0=dens, tol=1e-6, max_iter=5000,
+                 warm_start=False)
+    S = Scheduler([(10, 6, 6, 2)])
+    S.admit(0, P, n_iters=0, x0=dens, xphys0=dens, tol=1e-6, max_iter=5000, warm_start=False)
+    _, st = S.run()
+    _same(S.result(0, st[0]), ref)
+    # and it is the linear-elastic compliance of that density field
+    Po = O.OracleProblem(10, 6, 6, P.f_host, P.fixed, penal=1.0, law="clamp")
+    u, _ = Po.solve_u(dens, "direct")
+    assert abs(st[0].compliance - P.f_host @ u) <= 1e-5 * abs(P.f_host @ u)
+
+
+def test_admission_errors(gpu):
+    from genpipeline_amd import _lib
+    from genpipeline_amd.sched import Scheduler
+    P = _problem(gpu, 12, 6, 6, 9)
+    S = Scheduler([(8, 6, 6, 2), (12, 6, 6, 2)])
+    with pytest.raises(_lib.GpError, match="does not fit"):
+        S.admit(0, P, n_iters=1)
+    S.admit(1, P, n_iters=1)
+    with pytest.raises(_lib.GpError, match="occupied"):
+        S.admit(1, P, n_iters=1)
+    with pytest.raises(_lib.GpError):
+        Scheduler([(8, 8, 8, 2)], tile=(8, 32)).admit(5, P)
+    S.run()
