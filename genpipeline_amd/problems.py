@@ -129,9 +129,12 @@ class Request:
     volfrac: float
 
 
-def serving_requests(lam: float, n: int, seed=0, p_small=0.6, p_solve=0.5) -> list[Request]:
-    """Open-loop Poisson arrivals; densities are smoothed noise thresholded at
-    the (1-phi) quantile with solid fraction phi ~ U[0.15,0.45]."""
+def serving_requests(lam: float, n: int, seed=0, p_small=0.6, p_solve=0.5,
+                     shapes=((32, 32, 32), (64, 64, 64))) -> list[Request]:
+    """Open-loop Poisson arrivals (rate lam per second); the grid is shapes[0]
+    with probability p_small, else shapes[1]; densities are smoothed noise
+    (sigma 2) thresholded at the (1-phi) quantile with solid fraction
+    phi ~ U[0.15,0.45]; the kind is "solve" with probability p_solve."""
     from scipy.ndimage import gaussian_filter
 
     rng = np.random.default_rng(seed)
@@ -139,7 +142,7 @@ def serving_requests(lam: float, n: int, seed=0, p_small=0.6, p_solve=0.5) -> li
     out = []
     for r in range(n):
         t += float(rng.exponential(1.0 / lam))
-        shape = (32, 32, 32) if rng.uniform() < p_small else (64, 64, 64)
+        shape = tuple(shapes[0]) if rng.uniform() < p_small else tuple(shapes[1])
         phi = float(rng.uniform(0.15, 0.45))
         g = gaussian_filter(rng.standard_normal(shape), 2.0, mode="reflect")
         thr = np.quantile(g, 1.0 - phi)
