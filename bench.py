@@ -53,7 +53,103 @@ def parse():
     ap.add_argument("--calib-mib", type=int, default=0,
                     help="first run one elementwise add over a float32 buffer of this many MiB "
                          "(a known-size kernel for scaling HBM counters)")
+    ap.add_argument("--workload", choices=["single64", "batch48", "serve"], default="single64",
+                    help="single64: one 64^3 problem (default); batch48: 48 mixed-size problems "
+                         "on the device scheduler against the same problems one by one; serve: "
+                         "open-loop request serving at 50/200/800 req/s")
+    ap.add_argument("--batch-n", type=int, default=48)
+    ap.add_argument("--batch-oc-iters", type=int, default=60)
+    ap.add_argument("--serve-n", type=int, default=60, help="requests per arrival rate")
     return ap.parse_args()
+
+
+def _line(metric, value, unit, steps, warmup, ms, dtype, config, **extra):
+    out = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps,
+           "warmup": warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": config}
+    out.update(extra)
+    return out
+
+
+def bench_batch48(a):
+    """C4: 48 SIMP problems, grids drawn by seed from {32^3, 48x24x24, 64x32x32, 64^3},
+    volfrac ~ U[0.2, 0.4], 1-4 seeded point loads on face x = max, clamp at x = 0,
+    60 OC iterations, FP32 PCG at 1e-6, all in one device scheduler (independent
+    progress, ticks replayed from a HIP graph); timed beside the same problems
+    run one after another through the single-problem path."""
+    import torch
+    from genpipeline_amd import Problem
+    from genpipeline_amd.problems import batch_specs
+    from genpipeline_amd.sched import run_batch
+
+    ws = batch_specs(seed=0, n=a.batch_n, n_iters=a.batch_oc_iters)
+    dev = torch.device("cuda", 0)
+    Ps = [Problem(w.spec, w.fixed, w.f, device=dev) for w in ws]
+    run_batch(Ps[:4], 2, tile=(16, 16), ni=8, graph_ticks=32)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res, ticks = run_batch(Ps, a.batch_oc_iters, tile=(16, 16), ni=8, graph_ticks=32, tol=1e-6,
+                           max_iter=20000)
+    torch.cuda.synchronize()
+    t_b = time.perf_counter() - t0
+    print(f"[bench] batch48 scheduler: {t_b:.2f} s, {ticks} ticks", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    seq = [P.run(a.batch_oc_iters, tol=1e-6, max_iter=20000, precision="fp32", warm_start=True)
+           for P in Ps]
+    torch.cuda.synchronize()
+    t_s = time.perf_counter() - t0
+    print(f"[bench] batch48 one by one: {t_s:.2f} s", file=sys.stderr, flush=True)
+    rel = max(abs(x.compliance - y.compliance) / abs(y.compliance) for x, y in zip(res, seq))
+    its = sum(r.pcg_iters for r in res)
+    # algorithmic bytes of every PCG iteration the batch ran (120 B/node + 4 B/elem each)
+    pcg_bytes = sum(r.pcg_iters * (120 * P.spec.n_nodes + 4 * P.spec.n_elem)
+                    for r, P in zip(res, Ps))
+    n = len(Ps)
+    return _line("SIMP solves/sec (48-problem batch, device scheduler)", n / t_b, "solves/s", 1,
+                 1, t_b * 1e3, "f32",
+                 {"workload": "simp_batch48_c4_seed0", "problems": n, "oc_iters": a.batch_oc_iters,
+                  "grids": "32^3 / 48x24x24 / 64x32x32 / 64^3 by seed",
+                  "pcg": "jacobi, fp32 vectors, fp64 dots, rel tol 1e-6, warm start",
+                  "scheduler": "16x16 tile, ni 8, 32-tick HIP graph"},
+                 extra={"one_by_one_solves_per_s": n / t_s, "speedup_vs_one_by_one": t_s / t_b,
+                        "ticks": ticks, "us_per_tick": t_b / ticks * 1e6, "pcg_iters_total": its,
+                        "pcg_algorithmic_gbs": pcg_bytes / t_b / 1e9,
+                        "max_rel_compliance_diff_vs_one_by_one": rel})
+
+
+def bench_serve(a):
+    """C5: open-loop Poisson arrivals at 50/200/800 req/s of 32^3 (60%) / 64^3 density
+    fields (smoothed noise, solid fraction U[0.15, 0.45]); half single solves
+    (GPUVoxelFEM semantics), half 30-iteration SIMP refinements (SIMPSolverGPU.run
+    with x_init); continuous batching on the device scheduler."""
+    import torch
+    from genpipeline_amd.problems import serving_requests
+    from genpipeline_amd.serve import Server, latency_summary
+
+    srv = Server({(32, 32, 32): 16, (64, 64, 64): 8}, tile=(16, 16), ni=4, chunk_ticks=64)
+    srv.run(serving_requests(1000, 6, seed=99))  # warm-up
+    per = {}
+    total_n, total_t = 0, 0.0
+    for lam in (50, 200, 800):
+        reqs = serving_requests(lam, a.serve_n, seed=lam)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = srv.run(reqs)
+        wall = time.perf_counter() - t0
+        s = latency_summary(res, wall)
+        s["pcg_iters_mean"] = {k: float(sum(r.pcg_iters for r in res if r.kind == k)
+                                        / max(1, sum(1 for r in res if r.kind == k)))
+                               for k in ("solve", "simp")}
+        per[str(lam)] = s
+        total_n += len(res)
+        total_t += wall
+        print(f"[bench] serve lam={lam}: {s}", file=sys.stderr, flush=True)
+    srv.close()
+    return _line("SIMP/FEM request serving: solves/sec with p50/p99 latency", total_n / total_t,
+                 "solves/s", len(per), 1, total_t / len(per) * 1e3, "f32",
+                 {"workload": "serve_c5_32cube_64cube", "requests_per_rate": a.serve_n,
+                  "rates_per_s": [50, 200, 800], "slots": {"32^3": 16, "64^3": 8}},
+                 extra={"per_rate": per, "peak_problem_bytes": srv.peak_used})
 
 
 def cpu_baseline(w, gpu_pcg_iters, gpu_oc_iters, n_sample):
@@ -103,6 +199,12 @@ def main():
               "problems shard one per GPU, and no multi-GPU figure is measured in this build.",
               file=sys.stderr)
         sys.exit(2)
+    if a.workload == "batch48":
+        print(json.dumps(bench_batch48(a)), flush=True)
+        return
+    if a.workload == "serve":
+        print(json.dumps(bench_serve(a)), flush=True)
+        return
     import torch
     from genpipeline_amd import Problem
     from genpipeline_amd.problems import point_loads

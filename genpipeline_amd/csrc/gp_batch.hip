@@ -11,10 +11,11 @@
 
 namespace gp {
 
-#define GP_BATCH_BLOCK(OFF)                              \
-  const int b_ = find_prob(bl.OFF, bl.n, blockIdx.x);    \
-  const BatchProb& P = bl.probs[b_];                     \
-  const int lb = blockIdx.x - bl.OFF[b_];                \
+#define GP_BATCH_BLOCK(OFF)                                   \
+  const int gb_ = xcd_block(blockIdx.x, gridDim.x);           \
+  const int b_ = find_prob(bl.OFF, bl.n, gb_);                \
+  const BatchProb& P = bl.probs[b_];                          \
+  const int lb = gb_ - bl.OFF[b_];                            \
   const int nb = bl.OFF[b_ + 1] - bl.OFF[b_];
 
 __global__ __launch_bounds__(kBlock) void k_b_stiffness(BatchLaunch bl) {

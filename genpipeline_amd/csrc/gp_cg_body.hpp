@@ -18,24 +18,49 @@ struct NodeCG {
   T r[3], s[3], z[3], zp[3];
 };
 
+// Raw inputs of one node (r_{k-1}, w_{k-1}, s_{k-2}, M), zero outside the grid.
 template <typename T>
-__device__ inline void node_cg(const Grid& g, const T* __restrict__ rp, const T* __restrict__ wp,
-                               const T* __restrict__ sp, const T* __restrict__ m, int ip, int jj,
-                               int kk, T alpha, T beta, NodeCG<T>& o) {
+struct RawNode {
+  T r[3], w[3], s[3], m[3];
+};
+
+template <typename T>
+__device__ inline void load_node(const Grid& g, const T* __restrict__ rp, const T* __restrict__ wp,
+                                 const T* __restrict__ sp, const T* __restrict__ m, int ip, int jj,
+                                 int kk, RawNode<T>& o) {
   const bool ok = (unsigned)ip <= (unsigned)g.nx && (unsigned)jj <= (unsigned)g.ny &&
                   (unsigned)kk <= (unsigned)g.nz;
   const long long n = ok ? (long long)ip * g.sI + (long long)jj * g.sJ + kk : 0;
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
     const T vr = rp[3 * n + q], vw = wp[3 * n + q], vs = sp[3 * n + q], vm = m[3 * n + q];
-    const T r0 = ok ? vr : T(0), w0 = ok ? vw : T(0), s0 = ok ? vs : T(0), m0 = ok ? vm : T(0);
-    const T sn = w0 + beta * s0;
-    const T rn = r0 - alpha * sn;
+    o.r[q] = ok ? vr : T(0);
+    o.w[q] = ok ? vw : T(0);
+    o.s[q] = ok ? vs : T(0);
+    o.m[q] = ok ? vm : T(0);
+  }
+}
+
+template <typename T>
+__device__ inline void finish_node(const RawNode<T>& in, T alpha, T beta, NodeCG<T>& o) {
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const T sn = in.w[q] + beta * in.s[q];
+    const T rn = in.r[q] - alpha * sn;
     o.s[q] = sn;
     o.r[q] = rn;
-    o.z[q] = m0 * rn;
-    o.zp[q] = m0 * r0;
+    o.z[q] = in.m[q] * rn;
+    o.zp[q] = in.m[q] * in.r[q];
   }
+}
+
+template <typename T>
+__device__ inline void node_cg(const Grid& g, const T* __restrict__ rp, const T* __restrict__ wp,
+                               const T* __restrict__ sp, const T* __restrict__ m, int ip, int jj,
+                               int kk, T alpha, T beta, NodeCG<T>& o) {
+  RawNode<T> raw;
+  load_node(g, rp, wp, sp, m, ip, jj, kk, raw);
+  finish_node(raw, alpha, beta, o);
 }
 
 // 2-D transform of z over the element column's four corners, read from the
@@ -157,13 +182,19 @@ __device__ inline bool body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
     return ok ? (T)v : T(0);
   };
   // updates plane ip's own and halo nodes and publishes their z in zs[buf]
+  // updates plane ip's own and halo nodes and publishes their z in zs[buf];
+  // both nodes' loads are issued before either is used, so the halo threads
+  // (all in wave 0) wait for one memory latency per plane, not two
   auto stage = [&](int ip, int buf, NodeCG<T>& own) {
-    node_cg(g, rp, wp, sp, minv, ip, je + 1, ke + 1, alpha, beta, own);
+    RawNode<T> ro, rh;
+    load_node(g, rp, wp, sp, minv, ip, je + 1, ke + 1, ro);
+    if (has_h) load_node(g, rp, wp, sp, minv, ip, J0 - 1 + ha, K0 - 1 + hb, rh);
+    finish_node(ro, alpha, beta, own);
 #pragma unroll
     for (int q = 0; q < 3; ++q) zs[buf][q][a + 1][b + 1] = own.z[q];
     if (has_h) {
       NodeCG<T> h;
-      node_cg(g, rp, wp, sp, minv, ip, J0 - 1 + ha, K0 - 1 + hb, alpha, beta, h);
+      finish_node(rh, alpha, beta, h);
 #pragma unroll
       for (int q = 0; q < 3; ++q) zs[buf][q][ha][hb] = h.z[q];
     }
@@ -188,6 +219,17 @@ __device__ inline bool body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
   for (int ie = I0, t = 0; ie < iend; ++ie, ++t) {
     const T Ee = Eat(ie);
     const int zb = t & 1;  // buffer of plane ie+1 (planes I0-1, I0 used 0, 1)
+    // the owned node's p and x are loaded first: in flight across the plane's
+    // loads, barriers and LDS work, used only at the end of the step
+    T pv[3] = {T(0), T(0), T(0)}, xv[3] = {T(0), T(0), T(0)};
+    const long long nown = (long long)ie * g.sI + (long long)(je + 1) * g.sJ + (ke + 1);
+    if (owner) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        pv[q] = p[3 * nown + q];
+        xv[q] = x[3 * nown + q];
+      }
+    }
     if (ie < g.nx) {
       // zs[zb] was last read two steps earlier (or in the prologue), before a
       // barrier every thread has since passed
@@ -208,7 +250,7 @@ __device__ inline bool body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
     faces_to_corners<T, NT>(topp, bot, cv[par], tid);
     __syncthreads();
     if (owner) {
-      const long long n = (long long)ie * g.sI + (long long)(je + 1) * g.sJ + (ke + 1);
+      const long long n = nown;
       T w[3];
 #pragma unroll
       for (int q = 0; q < 3; ++q) w[q] = gather_node<T, TK, NT>(cv[par], q, tid);
@@ -220,8 +262,8 @@ __device__ inline bool body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
       }
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
-        const T pk = own_cur.zp[q] + beta * p[3 * n + q];  // p_{k-1}
-        const T xk = x[3 * n + q] + alpha * pk;            // x_k
+        const T pk = own_cur.zp[q] + beta * pv[q];  // p_{k-1}
+        const T xk = xv[q] + alpha * pk;            // x_k
         p[3 * n + q] = pk;
         x[3 * n + q] = xk;
         rn[3 * n + q] = own_cur.r[q];

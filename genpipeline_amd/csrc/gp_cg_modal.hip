@@ -39,7 +39,7 @@ __global__ __launch_bounds__(TJ* TK) void k_cg_modal(Grid g, int NI, int ntj, in
                                                     DevScalars* sc, Parts parts, int k) {
   __shared__ CornerLds<T, TJ * TK> cv;
   body_cg_modal<T, TJ, TK>(g, NI, ntj, ntk, kc, E, rp, wp, sp, minv, rn, wn, sn, x, p, fixmask, sc,
-                           parts, k, blockIdx.x, gridDim.x, cv);
+                           parts, k, xcd_block(blockIdx.x, gridDim.x), gridDim.x, cv);
 }
 
 template <int TJ, int TK>
@@ -51,16 +51,17 @@ static void launch_cg_t(const Grid& g, int NI, const void* kc, int prec, const f
   const int ntk = (g.nz + 1 + (TK - 2)) / (TK - 1);
   const int nti = (g.nx + 1 + NI - 1) / NI;
   const dim3 grid(ntj * ntk * nti), block(TJ * TK);
-  if (prec == GP_F32)
+  if (prec == GP_F32) {
     hipLaunchKernelGGL((k_cg_modal<float, TJ, TK>), grid, block, 0, st, g, NI, ntj, ntk,
                        *reinterpret_cast<const ModalCoef<float>*>(kc), E, (const float*)rp,
                        (const float*)wp, (const float*)sp, (const float*)minv, (float*)rn,
                        (float*)wn, (float*)sn, (float*)x, (float*)p, fixmask, sc, parts, k);
-  else
+  } else {
     hipLaunchKernelGGL((k_cg_modal<double, TJ, TK>), grid, block, 0, st, g, NI, ntj, ntk,
                        *reinterpret_cast<const ModalCoef<double>*>(kc), E, (const double*)rp,
                        (const double*)wp, (const double*)sp, (const double*)minv, (double*)rn,
                        (double*)wn, (double*)sn, (double*)x, (double*)p, fixmask, sc, parts, k);
+  }
 }
 
 bool launch_cg_modal(const Grid& g, const KuTile& t, const void* kc, int prec, const float* E,

@@ -8,6 +8,7 @@
 namespace gp {
 
 constexpr int kBlock = 256;          // threads per block of the 1-D kernels
+constexpr int kNumXcd = 8;           // MI355X: workgroups go round-robin over 8 XCDs
 constexpr int kVecBlocks = 512;      // cap on blocks of reducing vector kernels
 constexpr int kMaxOffsets = 343;     // filter stencil cap: (2*3+1)^3, rmin <= 3
 
@@ -183,6 +184,16 @@ struct SchedTable {
 
 // Enqueues one tick with parity t & 1 (tj x tk must be 256 threads).
 bool sched_tick(const SchedTable& tab, int tj, int tk, int blocks, int t, hipStream_t st);
+
+// Logical block of hardware block b out of n. Workgroups are dispatched
+// round-robin over the XCDs, each with its own L2; this bijection gives every
+// XCD a contiguous range of logical blocks, so neighbouring tiles -- which
+// share halo nodes and partly used cache lines -- run under the same L2.
+__device__ inline int xcd_block(int b, int n) {
+  const int x = b % kNumXcd, i = b / kNumXcd;
+  const int q = n / kNumXcd, r = n - q * kNumXcd;
+  return x * q + (x < r ? x : r) + i;
+}
 
 __device__ inline double wave_sum64(double v) {
 #pragma unroll

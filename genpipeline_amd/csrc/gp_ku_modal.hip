@@ -33,7 +33,7 @@ __global__ __launch_bounds__(TJ* TK) void k_ku_modal(Grid g, int NI, int ntj, in
                                                     int nV) {
   __shared__ CornerLds<T, TJ * TK> cv;
   body_ku_modal<T, TJ, TK, MODE>(g, NI, ntj, ntk, kc, E, u, u2, y, pout, fixmask, sc, parts, it,
-                                 nV, blockIdx.x, cv);
+                                 nV, xcd_block(blockIdx.x, gridDim.x), cv);
 }
 
 template <int TJ, int TK>

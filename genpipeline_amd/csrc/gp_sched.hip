@@ -161,9 +161,10 @@ template <int TJ, int TK>
 __global__ __launch_bounds__(TJ* TK, 4) void k_tick(SchedTable tab, int t) {
   static_assert(TJ * TK == kBlock, "the scheduler runs every body with kBlock threads");
   __shared__ CornerLds<float, TJ * TK> cv;
-  const int b = tab.b2s[blockIdx.x];
+  const int gb = xcd_block(blockIdx.x, gridDim.x);
+  const int b = tab.b2s[gb];
   SchedSlot& S = tab.slots[b];
-  const int lb = blockIdx.x - tab.off[b];
+  const int lb = gb - tab.off[b];
   const SchedState& cur = S.st[t & 1];
   const BatchProb& P = S.P;
   const int phase = cur.phase;
