@@ -16,7 +16,7 @@ _lib = None
 GP_OK, GP_ERR_ARG, GP_ERR_OOM, GP_ERR_HIP, GP_BREAKDOWN, GP_NOT_CONVERGED, GP_ERR_NAN, GP_ERR_STATE = range(8)
 STATUS_NAMES = {0: "ok", 1: "bad argument", 2: "out of device memory", 3: "HIP error",
                 4: "breakdown", 5: "not converged", 6: "nan", 7: "bad state"}
-GP_F32, GP_F64, GP_F16 = 0, 1, 2
+GP_F32, GP_F64, GP_F16, GP_BF16 = 0, 1, 2, 3
 GP_LAW_CLAMP, GP_LAW_EMIN = 0, 1
 GP_OC_CPU, GP_OC_GPU = 0, 1
 GP_KE_HEX8, GP_KE_CUSTOM = 0, 1

@@ -147,6 +147,7 @@ struct gp_problem {
   hipEvent_t ev[2] = {nullptr, nullptr};
   // PCG workspace per precision
   void* ws32 = nullptr;         // PCG vectors (float, 8*ndof), layout in pcg_core
+  void* ws16 = nullptr;         // 16-bit PCG: M^-1 (float) + 8 vectors (layout in pcg_core_lp)
   void* ws64 = nullptr;         // PCG vectors (double, 8*ndof)
   double* r64 = nullptr;        // refinement residual
   double* y64 = nullptr;        // refinement K u
@@ -160,7 +161,7 @@ struct gp_problem {
   double* ce_sum = nullptr;
   // sampled iteration timing: events bracket every `prof_every`-th chunk of
   // PCG iteration launches
-  int iter_hint[2] = {0, 0};  // last PCG iteration count (fp32, fp64)
+  int iter_hint[3] = {0, 0, 0};  // last PCG iteration count (fp32, fp64, 16-bit)
   int prof_every = 0;
   long long prof_counter = 0;
   double prof_ms = 0.0;
@@ -344,6 +345,19 @@ int ensure_pcg_ws(gp_problem* P, int prec) {
   return GP_OK;
 }
 
+int ensure_lp_ws(gp_problem* P) {
+  const size_t nd = (size_t)P->g.ndof;
+  if (!P->ws16) ALLOC(P->ws16, 7 * nd * sizeof(float) + 2 * nd * sizeof(uint16_t));
+  return GP_OK;
+}
+
+// the 16-bit correction vector d of the refinement (after p in ws16)
+template <typename S>
+S* lp_correction(gp_problem* P) {
+  const size_t nd = (size_t)P->g.ndof;
+  return reinterpret_cast<S*>(reinterpret_cast<float*>(P->ws16) + 7 * nd) + nd;
+}
+
 int ensure_refine_ws(gp_problem* P) {
   const size_t nd = (size_t)P->g.ndof;
   if (!P->r64) ALLOC(P->r64, nd * sizeof(double));
@@ -381,45 +395,15 @@ int ku_dispatch(gp_problem* P, const float* E, const T* u, T* y, bool mask, bool
   return gather_blocks(P->g);
 }
 
-// Plain PCG in precision T on the free DOFs. u: T*, in x0 / out solution.
-template <typename T>
-int pcg_core(gp_problem* P, const float* E, const double* f, T* u, double tol, int max_iter,
-             bool use_x0, gp_pcg_stats* st, hipStream_t s) {
-  const Grid& g = P->g;
-  const size_t nd = (size_t)g.ndof;
-  T* base = reinterpret_cast<T*>(std::is_same<T, float>::value ? P->ws32 : P->ws64);
-  T* r = base;
-  T* z = base + nd;
-  T* p = base + 2 * nd;
-  T* Ap = base + 3 * nd;
-  T* minv = base + 4 * nd;
-  const bool modal = P->modal_ok && (std::is_same<T, float>::value ? P->use_modal32 : P->use_modal64);
-  const KuTile tile = std::is_same<T, float>::value ? P->tile32 : P->tile64;
-  const void* kc = std::is_same<T, float>::value ? (const void*)&P->mc32 : (const void*)&P->mc64;
-  const int prec = std::is_same<T, float>::value ? GP_F32 : GP_F64;
-  const T* ke = reinterpret_cast<const T*>(std::is_same<T, float>::value ? (const void*)P->ke32
-                                                                          : (const void*)P->ke64);
+// Issues PCG iteration launches after the set-up: chunked and speculative (one
+// chunk always queued behind the one whose status the host inspects), until
+// the device reports the solve finished or the cap is passed. issue_one(m)
+// enqueues iteration launch m. Keeps the sampled timing and the iteration
+// hint of slot hint_slot; fills iters / status / relres.
+template <typename IssueOne>
+int pcg_drive(gp_problem* P, int hint_slot, int max_iter, bool modal, IssueOne issue_one,
+              gp_pcg_stats* st, hipStream_t s) {
   DevScalars* sc = &P->sc[0];
-  // single-kernel (Chronopoulos-Gear) iteration buffers on the modal path:
-  // r, w, s ping-pong by parity; p in place
-  T* rb[2] = {base, base + nd};
-  T* wb[2] = {base + 2 * nd, base + 3 * nd};
-  T* sb[2] = {base + 5 * nd, base + 6 * nd};
-  T* pc = base + 7 * nd;
-  launch_diag_minv<T>(g, ke, E, P->fixmask, minv, true, s);
-  if (use_x0) {
-    launch_mask_fixed<T>(g, u, P->fixmask, s);
-    ku_dispatch<T>(P, E, u, modal ? wb[0] : Ap, true, false, s);
-  }
-  if (modal)
-    launch_cg_init<T>(g, f, wb[0], use_x0, u, rb[1], sb[0], wb[1], pc, minv, P->parts, sc, tol,
-                      max_iter, s);
-  else
-    launch_pcg_init<T>(g, f, Ap, use_x0, u, r, z, p, minv, P->parts, sc, tol, max_iter, s);
-  LAUNCHCHK();
-
-  // chunked, speculative issue: one chunk always queued behind the one the
-  // host is inspecting.
   auto take_event = [&]() {
     hipEvent_t e = nullptr;
     if (!P->prof_pool.empty()) {
@@ -439,17 +423,7 @@ int pcg_core(gp_problem* P, const float* E, const double* f, T* u, double tol, i
     hipEvent_t eb = sample ? take_event() : nullptr;
     const int first = launch_idx;
     if (ea && eb) hipEventRecord(ea, s);
-    for (int it = 0; it < n_it; ++it, ++launch_idx) {
-      if (modal) {
-        const int k = launch_idx, c = k & 1, o = c ^ 1;
-        launch_cg_modal(g, tile, kc, prec, E, rb[o], wb[o], sb[c], minv, rb[c], wb[c], sb[o], u,
-                        pc, P->fixmask, sc, P->parts, k, s);
-      } else {
-        const int nA = ku_dispatch<T>(P, E, p, Ap, true, true, s);
-        launch_pcg_b<T>(g, launch_idx, nA, u, r, z, p, Ap, minv, P->parts, sc, s);
-        launch_pcg_c<T>(g, launch_idx, p, z, P->parts, sc, s);
-      }
-    }
+    for (int it = 0; it < n_it; ++it, ++launch_idx) issue_one(launch_idx);
     if (ea && eb) {
       hipEventRecord(eb, s);
       P->prof_pending.push_back({ea, eb, first, n_it});
@@ -462,7 +436,7 @@ int pcg_core(gp_problem* P, const float* E, const double* f, T* u, double tol, i
   // one problem need similar counts) lets most launches go out in large
   // chunks; near that count, and without a hint, chunks stay small so few
   // launches are issued after convergence.
-  const int hint = P->iter_hint[prec == GP_F32 ? 0 : 1];
+  const int hint = P->iter_hint[hint_slot];
   int issued = 0;
   auto next_chunk = [&](int prev) {
     int c = std::min(prev * 2, 64);
@@ -513,12 +487,105 @@ int pcg_core(gp_problem* P, const float* E, const double* f, T* u, double tol, i
     P->prof_pool.push_back(pr.b);
   }
   P->prof_pending.clear();
-  P->iter_hint[prec == GP_F32 ? 0 : 1] = h.iter;
+  P->iter_hint[hint_slot] = h.iter;
   st->status = h.status;
   st->iters += h.iter;
   st->relres = h.bb > 0 ? std::sqrt(h.rr / h.bb) : 0.0;
   if (!h.done) st->status = GP_NOT_CONVERGED;
   return GP_OK;
+}
+
+
+// Plain PCG in precision T on the free DOFs. u: T*, in x0 / out solution.
+template <typename T>
+int pcg_core(gp_problem* P, const float* E, const double* f, T* u, double tol, int max_iter,
+             bool use_x0, gp_pcg_stats* st, hipStream_t s) {
+  const Grid& g = P->g;
+  const size_t nd = (size_t)g.ndof;
+  T* base = reinterpret_cast<T*>(std::is_same<T, float>::value ? P->ws32 : P->ws64);
+  T* r = base;
+  T* z = base + nd;
+  T* p = base + 2 * nd;
+  T* Ap = base + 3 * nd;
+  T* minv = base + 4 * nd;
+  const bool modal = P->modal_ok && (std::is_same<T, float>::value ? P->use_modal32 : P->use_modal64);
+  const KuTile tile = std::is_same<T, float>::value ? P->tile32 : P->tile64;
+  const void* kc = std::is_same<T, float>::value ? (const void*)&P->mc32 : (const void*)&P->mc64;
+  const int prec = std::is_same<T, float>::value ? GP_F32 : GP_F64;
+  const T* ke = reinterpret_cast<const T*>(std::is_same<T, float>::value ? (const void*)P->ke32
+                                                                          : (const void*)P->ke64);
+  DevScalars* sc = &P->sc[0];
+  // single-kernel (Chronopoulos-Gear) iteration buffers on the modal path:
+  // r, w, s ping-pong by parity; p in place
+  T* rb[2] = {base, base + nd};
+  T* wb[2] = {base + 2 * nd, base + 3 * nd};
+  T* sb[2] = {base + 5 * nd, base + 6 * nd};
+  T* pc = base + 7 * nd;
+  launch_diag_minv<T>(g, ke, E, P->fixmask, minv, true, s);
+  if (use_x0) {
+    launch_mask_fixed<T>(g, u, P->fixmask, s);
+    ku_dispatch<T>(P, E, u, modal ? wb[0] : Ap, true, false, s);
+  }
+  if (modal)
+    launch_cg_init<T>(g, f, wb[0], use_x0, u, rb[1], sb[0], wb[1], pc, minv, P->parts, sc, tol,
+                      max_iter, s);
+  else
+    launch_pcg_init<T>(g, f, Ap, use_x0, u, r, z, p, minv, P->parts, sc, tol, max_iter, s);
+  LAUNCHCHK();
+
+  const int hint_slot = std::is_same<T, float>::value ? 0 : 1;
+  if (modal)
+    return pcg_drive(
+        P, hint_slot, max_iter, true,
+        [&](int k) {
+          const int c = k & 1, o = c ^ 1;
+          launch_cg_modal(g, tile, kc, prec, E, rb[o], wb[o], sb[c], minv, rb[c], wb[c], sb[o], u,
+                          pc, P->fixmask, sc, P->parts, k, s);
+        },
+        st, s);
+  return pcg_drive(
+      P, hint_slot, max_iter, false,
+      [&](int k) {
+        const int nA = ku_dispatch<T>(P, E, p, Ap, true, true, s);
+        launch_pcg_b<T>(g, k, nA, u, r, z, p, Ap, minv, P->parts, sc, s);
+        launch_pcg_c<T>(g, k, p, z, P->parts, sc, s);
+      },
+      st, s);
+}
+
+// Inner solve of FP64 refinement with the iterate d and the search direction
+// held in 16 bits (S = bf16 or _Float16): K d = rhs from d = 0, float
+// arithmetic and float r, w, s, M^-1, FP64 dot products. Modal operator only.
+// The eight element coefficients stay in float: rounding them one by one
+// breaks the relations that give the element its rotational null space, and
+// the perturbed operator is then indefinite on soft regions.
+template <typename S>
+int pcg_core_lp(gp_problem* P, const float* E, const double* rhs, S* d, double tol, int max_iter,
+                gp_pcg_stats* st, hipStream_t s) {
+  const Grid& g = P->g;
+  const size_t nd = (size_t)g.ndof;
+  // ws16: 7 float vectors (M^-1, r, w, s ping-pong), then p and d in S
+  float* minv = reinterpret_cast<float*>(P->ws16);
+  float* rb[2] = {minv + nd, minv + 2 * nd};
+  float* wb[2] = {minv + 3 * nd, minv + 4 * nd};
+  float* sb[2] = {minv + 5 * nd, minv + 6 * nd};
+  S* pc = reinterpret_cast<S*>(minv + 7 * nd);
+  DevScalars* sc = &P->sc[0];
+  const ModalCoef<float> kc = P->mc32;
+  const int prec = std::is_same<S, bf16>::value ? GP_BF16 : GP_F16;
+  const KuTile tile = P->tile32.tj ? P->tile32 : KuTile{8, 32, 2};
+  launch_diag_minv<float>(g, P->ke32, E, P->fixmask, minv, true, s);
+  launch_cg_init<float, S>(g, rhs, nullptr, false, d, rb[1], sb[0], wb[1], pc, minv, P->parts, sc,
+                           tol, max_iter, s);
+  LAUNCHCHK();
+  return pcg_drive(
+      P, 2, max_iter, true,
+      [&](int k) {
+        const int c = k & 1, o = c ^ 1;
+        launch_cg_modal(g, tile, &kc, prec, E, rb[o], wb[o], sb[c], minv, rb[c], wb[c], sb[o], d,
+                        pc, P->fixmask, sc, P->parts, k, s);
+      },
+      st, s);
 }
 
 // FP64 iterative refinement: r = f - K u in FP64, correction solved in FP32.
@@ -527,7 +594,10 @@ int pcg_refine(gp_problem* P, const float* E, const double* f, double* u, const 
   const Grid& g = P->g;
   int rc = ensure_refine_ws(P);
   if (rc) return rc;
-  rc = ensure_pcg_ws(P, GP_F32);
+  const bool lp = o->precision == GP_F16 || o->precision == GP_BF16;
+  if (lp && !P->modal_ok)
+    return fail(GP_ERR_ARG, "16-bit PCG storage needs an isotropic hex8 element table");
+  rc = lp ? ensure_lp_ws(P) : ensure_pcg_ws(P, GP_F32);
   if (rc) return rc;
   DevScalars* sc2 = &P->sc[2];
   // ||f_free||^2
@@ -562,11 +632,26 @@ int pcg_refine(gp_problem* P, const float* E, const double* f, double* u, const 
       break;
     }
     gp_pcg_stats in{};
-    rc = pcg_core<float>(P, E, P->r64, P->d32, o->inner_tol > 0 ? o->inner_tol : 1e-3,
-                         o->max_iter, false, &in, s);
+    const double itol = o->inner_tol > 0 ? o->inner_tol : 1e-3;
+    // 16-bit inner solves see the residual rescaled to the norm of f, so the
+    // correction keeps the magnitude of the solution instead of shrinking
+    // with the residual into the 16-bit underflow range.
+    const double up = std::sqrt(bb / rr), down = std::sqrt(rr / bb);
+    if (lp) launch_scale64(g, P->r64, up, s);
+    if (o->precision == GP_BF16) {
+      bf16* d = lp_correction<bf16>(P);
+      rc = pcg_core_lp<bf16>(P, E, P->r64, d, itol, o->max_iter, &in, s);
+      if (!rc) launch_add_correction_lp<bf16>(g, u, d, down, s);
+    } else if (o->precision == GP_F16) {
+      _Float16* d = lp_correction<_Float16>(P);
+      rc = pcg_core_lp<_Float16>(P, E, P->r64, d, itol, o->max_iter, &in, s);
+      if (!rc) launch_add_correction_lp<_Float16>(g, u, d, down, s);
+    } else {
+      rc = pcg_core<float>(P, E, P->r64, P->d32, itol, o->max_iter, false, &in, s);
+      if (!rc) launch_add_correction<float>(g, u, P->d32, s);
+    }
     if (rc) return rc;
     st->iters += in.iters;
-    launch_add_correction<float>(g, u, P->d32, s);
     if (in.status == GP_BREAKDOWN) {
       st->status = GP_BREAKDOWN;
       // one more residual for the report
@@ -623,6 +708,7 @@ int gp_footprint_bytes(const gp_spec* spec, int32_t precision, size_t* bytes) {
              nd * 8 /*f*/ + 3 * sizeof(DevScalars) + kRegions * partial_cap(g) * 8;
   b += ne * (4 + 4 * 8) + 8;                                   // SIMP loop workspace
   if (precision == GP_F64) b += 8 * nd * 8;
+  else if (precision == GP_F16 || precision == GP_BF16) b += nd * (28 + 4) + nd * (8 + 8 + 4);
   else b += 8 * nd * 4 + nd * (8 + 8 + 4);                     // FP32 PCG + refinement
   *bytes = b;
   return GP_OK;
@@ -725,7 +811,7 @@ int gp_problem_destroy(gp_problem* P) {
   hipSetDevice(P->device);
   hipDeviceSynchronize();
   void* dev[] = {P->ke64, P->ke32, P->st, P->Hs, P->w, P->fixmask, P->pres, P->f,
-                 P->sc, P->partials, P->ws32, P->ws64, P->r64, P->y64, P->d32, P->E,
+                 P->sc, P->partials, P->ws32, P->ws64, P->ws16, P->r64, P->y64, P->d32, P->E,
                  P->dc, P->dcf, P->xnew, P->xphys_new, P->ce_sum};
   for (void* p : dev)
     if (p) hipFree(p);
@@ -886,8 +972,10 @@ int gp_pcg(gp_problem* P, const float* E, const double* f, void* u, const gp_pcg
   std::memset(st, 0, sizeof(*st));
   hipStream_t s = S(stream);
   int rc;
+  const bool lp = o->precision == GP_F16 || o->precision == GP_BF16;
+  if (lp && !o->refine)
+    return fail(GP_ERR_ARG, "16-bit PCG storage runs inside FP64 refinement (refine = 1)");
   if (o->refine) {
-    if (o->precision == GP_F16) return fail(GP_ERR_ARG, "GP_F16 storage is not available in this build");
     rc = pcg_refine(P, E, f, (double*)u, o, st, s);
   } else if (o->precision == GP_F64) {
     rc = ensure_pcg_ws(P, GP_F64);
@@ -972,7 +1060,8 @@ int gp_simp_run(gp_problem* P, const gp_simp_opts* o, double* x, double* xPhys, 
   if (rc) return rc;
   const Grid& g = P->g;
   const gp_spec& sp = P->spec;
-  const bool refine = o->pcg.refine != 0 || o->pcg.precision == GP_F64;
+  const bool refine = o->pcg.refine != 0 || o->pcg.precision == GP_F64 ||
+                      o->pcg.precision == GP_F16 || o->pcg.precision == GP_BF16;
   // u is float64 state; plain FP32 PCG works on a float copy
   float* u32 = nullptr;
   if (!refine) {

@@ -161,10 +161,11 @@ __device__ inline void body_diag(const Grid& g, const T* __restrict__ ke,
 
 // Single-kernel PCG set-up: r_{-1} = b - A x0 on free DOFs (or b), x masked,
 // s_{-2} = 0, w_{-1} = 0, p = 0; partials bb -> BB[lb], rr -> RR[lb].
-template <typename T>
+// S: storage of x and p (T for FP32/FP64; a 16-bit type with T = float).
+template <typename T, typename S = T>
 __device__ inline void body_cg_init(int nn, const double* __restrict__ f, const T* __restrict__ Ax,
-                                    int use_x0, T* __restrict__ x, T* __restrict__ r,
-                                    T* __restrict__ s0, T* __restrict__ w1, T* __restrict__ p,
+                                    int use_x0, S* __restrict__ x, T* __restrict__ r,
+                                    T* __restrict__ s0, T* __restrict__ w1, S* __restrict__ p,
                                     const T* __restrict__ minv, const Parts& parts, int lb,
                                     int nb) {
   double rr = 0.0, bb = 0.0;
@@ -172,13 +173,13 @@ __device__ inline void body_cg_init(int nn, const double* __restrict__ f, const 
        d += (long long)nb * kBlock) {
     const bool freed = minv[d] != T(0);
     const double bv = freed ? f[d] : 0.0;
-    const T xd = (use_x0 && freed) ? x[d] : T(0);
+    const S xd = (use_x0 && freed) ? x[d] : st_s<S>(T(0));
     const T rd = freed ? (T)(bv - (use_x0 ? (double)Ax[d] : 0.0)) : T(0);
     x[d] = xd;
     r[d] = rd;
     s0[d] = T(0);
     w1[d] = T(0);
-    p[d] = T(0);
+    p[d] = st_s<S>(T(0));
     rr += (double)rd * (double)rd;
     bb += bv * bv;
   }

@@ -54,14 +54,6 @@ __device__ inline void finish_node(const RawNode<T>& in, T alpha, T beta, NodeCG
   }
 }
 
-template <typename T>
-__device__ inline void node_cg(const Grid& g, const T* __restrict__ rp, const T* __restrict__ wp,
-                               const T* __restrict__ sp, const T* __restrict__ m, int ip, int jj,
-                               int kk, T alpha, T beta, NodeCG<T>& o) {
-  RawNode<T> raw;
-  load_node(g, rp, wp, sp, m, ip, jj, kk, raw);
-  finish_node(raw, alpha, beta, o);
-}
 
 // 2-D transform of z over the element column's four corners, read from the
 // plane's LDS tile zs[q][aa][bb] (aa, bb = node column - (J0-1, K0-1)).
@@ -114,13 +106,17 @@ __device__ inline void block_sum3_to(double v0, double v1, double v2, double* p0
 // their four corners from it.
 // Returns true when the solve is finished at this launch (converged, capped
 // or broken down at iteration k-1, or already finished): no work was done.
-template <typename T, int TJ, int TK>
+// T: arithmetic and storage of the recurrence vectors r, w, s (and M^-1);
+// S: storage of the iterate x and the search direction p. These two feed only
+// the x update, so a 16-bit S limits the accuracy of x (recovered by FP64
+// refinement) without disturbing the recurrence.
+template <typename T, int TJ, int TK, typename S = T>
 __device__ inline bool body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
                                      const ModalCoef<T>& kc, const float* __restrict__ E,
                                      const T* __restrict__ rp, const T* __restrict__ wp,
                                      const T* __restrict__ sp, const T* __restrict__ minv,
                                      T* __restrict__ rn, T* __restrict__ wn, T* __restrict__ sn,
-                                     T* __restrict__ x, T* __restrict__ p,
+                                     S* __restrict__ x, S* __restrict__ p,
                                      const uint8_t* __restrict__ fixmask, DevScalars* sc,
                                      const Parts& parts, int k, int lb, int nb,
                                      CornerLds<T, TJ * TK>& cv) {
@@ -226,8 +222,8 @@ __device__ inline bool body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
     if (owner) {
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
-        pv[q] = p[3 * nown + q];
-        xv[q] = x[3 * nown + q];
+        pv[q] = ld_s<T>(p[3 * nown + q]);
+        xv[q] = ld_s<T>(x[3 * nown + q]);
       }
     }
     if (ie < g.nx) {
@@ -264,8 +260,8 @@ __device__ inline bool body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
       for (int q = 0; q < 3; ++q) {
         const T pk = own_cur.zp[q] + beta * pv[q];  // p_{k-1}
         const T xk = xv[q] + alpha * pk;            // x_k
-        p[3 * n + q] = pk;
-        x[3 * n + q] = xk;
+        p[3 * n + q] = st_s<S>(pk);
+        x[3 * n + q] = st_s<S>(xk);
         rn[3 * n + q] = own_cur.r[q];
         sn[3 * n + q] = own_cur.s[q];
         wn[3 * n + q] = w[q];

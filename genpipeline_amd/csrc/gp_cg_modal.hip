@@ -20,11 +20,16 @@
 // p_{k-1} = z_{k-1} + beta p_{k-2} and x_k = x_{k-1} + alpha p_{k-1}.
 // r, w and s alternate between two buffers by iteration parity so that no
 // block reads a value another block has already overwritten.
+//
+// Storage: GP_F32 / GP_F64 keep every vector in the arithmetic type. GP_BF16
+// / GP_F16 store the iterate x and the search direction p in 16 bits (float
+// arithmetic; r, w, s and M^-1 in float; FP64 dot products); that variant
+// runs as the inner solve of FP64 iterative refinement.
 #include "gp_cg_body.hpp"
 
 namespace gp {
 
-template <typename T, int TJ, int TK>
+template <typename T, int TJ, int TK, typename S>
 __global__ __launch_bounds__(TJ* TK) void k_cg_modal(Grid g, int NI, int ntj, int ntk,
                                                     ModalCoef<T> kc,
                                                     const float* __restrict__ E,
@@ -33,13 +38,26 @@ __global__ __launch_bounds__(TJ* TK) void k_cg_modal(Grid g, int NI, int ntj, in
                                                     const T* __restrict__ sp,
                                                     const T* __restrict__ minv,
                                                     T* __restrict__ rn, T* __restrict__ wn,
-                                                    T* __restrict__ sn, T* __restrict__ x,
-                                                    T* __restrict__ p,
+                                                    T* __restrict__ sn, S* __restrict__ x,
+                                                    S* __restrict__ p,
                                                     const uint8_t* __restrict__ fixmask,
                                                     DevScalars* sc, Parts parts, int k) {
   __shared__ CornerLds<T, TJ * TK> cv;
-  body_cg_modal<T, TJ, TK>(g, NI, ntj, ntk, kc, E, rp, wp, sp, minv, rn, wn, sn, x, p, fixmask, sc,
-                           parts, k, xcd_block(blockIdx.x, gridDim.x), gridDim.x, cv);
+  body_cg_modal<T, TJ, TK, S>(g, NI, ntj, ntk, kc, E, rp, wp, sp, minv, rn, wn, sn, x, p,
+                              fixmask, sc, parts, k, xcd_block(blockIdx.x, gridDim.x), gridDim.x,
+                              cv);
+}
+
+template <int TJ, int TK, typename T, typename S>
+static void launch_cg_ts(const dim3& grid, int NI, int ntj, int ntk, const Grid& g, const void* kc,
+                         const float* E, const void* rp, const void* wp, const void* sp,
+                         const void* minv, void* rn, void* wn, void* sn, void* x, void* p,
+                         const uint8_t* fixmask, DevScalars* sc, Parts parts, int k,
+                         hipStream_t st) {
+  hipLaunchKernelGGL((k_cg_modal<T, TJ, TK, S>), grid, dim3(TJ * TK), 0, st, g, NI, ntj, ntk,
+                     *reinterpret_cast<const ModalCoef<T>*>(kc), E, (const T*)rp, (const T*)wp,
+                     (const T*)sp, (const T*)minv, (T*)rn, (T*)wn, (T*)sn, (S*)x, (S*)p, fixmask,
+                     sc, parts, k);
 }
 
 template <int TJ, int TK>
@@ -50,18 +68,17 @@ static void launch_cg_t(const Grid& g, int NI, const void* kc, int prec, const f
   const int ntj = (g.ny + 1 + (TJ - 2)) / (TJ - 1);
   const int ntk = (g.nz + 1 + (TK - 2)) / (TK - 1);
   const int nti = (g.nx + 1 + NI - 1) / NI;
-  const dim3 grid(ntj * ntk * nti), block(TJ * TK);
-  if (prec == GP_F32) {
-    hipLaunchKernelGGL((k_cg_modal<float, TJ, TK>), grid, block, 0, st, g, NI, ntj, ntk,
-                       *reinterpret_cast<const ModalCoef<float>*>(kc), E, (const float*)rp,
-                       (const float*)wp, (const float*)sp, (const float*)minv, (float*)rn,
-                       (float*)wn, (float*)sn, (float*)x, (float*)p, fixmask, sc, parts, k);
-  } else {
-    hipLaunchKernelGGL((k_cg_modal<double, TJ, TK>), grid, block, 0, st, g, NI, ntj, ntk,
-                       *reinterpret_cast<const ModalCoef<double>*>(kc), E, (const double*)rp,
-                       (const double*)wp, (const double*)sp, (const double*)minv, (double*)rn,
-                       (double*)wn, (double*)sn, (double*)x, (double*)p, fixmask, sc, parts, k);
+  const dim3 grid(ntj * ntk * nti);
+#define GP_ARGS \
+  grid, NI, ntj, ntk, g, kc, E, rp, wp, sp, minv, rn, wn, sn, x, p, fixmask, sc, parts, k, st
+  switch (prec) {
+    case GP_F32: launch_cg_ts<TJ, TK, float, float>(GP_ARGS); break;
+    case GP_F64: launch_cg_ts<TJ, TK, double, double>(GP_ARGS); break;
+    case GP_F16: launch_cg_ts<TJ, TK, float, _Float16>(GP_ARGS); break;
+    case GP_BF16: launch_cg_ts<TJ, TK, float, bf16>(GP_ARGS); break;
+    default: break;
   }
+#undef GP_ARGS
 }
 
 bool launch_cg_modal(const Grid& g, const KuTile& t, const void* kc, int prec, const float* E,
@@ -85,31 +102,34 @@ bool launch_cg_modal(const Grid& g, const KuTile& t, const void* kc, int prec, c
   return false;
 }
 
-template <typename T>
+template <typename T, typename S>
 __global__ __launch_bounds__(kBlock) void k_cg_init(int nn, const double* __restrict__ f,
                                                    const T* __restrict__ Ax, int use_x0,
-                                                   T* __restrict__ x, T* __restrict__ r,
+                                                   S* __restrict__ x, T* __restrict__ r,
                                                    T* __restrict__ s0, T* __restrict__ w1,
-                                                   T* __restrict__ p, const T* __restrict__ minv,
+                                                   S* __restrict__ p, const T* __restrict__ minv,
                                                    Parts parts) {
-  body_cg_init<T>(nn, f, Ax, use_x0, x, r, s0, w1, p, minv, parts, blockIdx.x, gridDim.x);
+  body_cg_init<T, S>(nn, f, Ax, use_x0, x, r, s0, w1, p, minv, parts, blockIdx.x, gridDim.x);
 }
 
-template <typename T>
-void launch_cg_init(const Grid& g, const double* f, const T* Ax, bool use_x0, T* x, T* r, T* s0,
-                    T* w1, T* p, const T* minv, Parts parts, DevScalars* sc, double tol,
+template <typename T, typename S>
+void launch_cg_init(const Grid& g, const double* f, const T* Ax, bool use_x0, S* x, T* r, T* s0,
+                    T* w1, S* p, const T* minv, Parts parts, DevScalars* sc, double tol,
                     int max_iter, hipStream_t st) {
   const int nV = vec_blocks(g);
-  hipLaunchKernelGGL((k_cg_init<T>), dim3(nV), dim3(kBlock), 0, st, g.nn, f, Ax, use_x0 ? 1 : 0, x,
-                     r, s0, w1, p, minv, parts);
+  hipLaunchKernelGGL((k_cg_init<T, S>), dim3(nV), dim3(kBlock), 0, st, g.nn, f, Ax,
+                     use_x0 ? 1 : 0, x, r, s0, w1, p, minv, parts);
   launch_pcg_init_fin(parts, nV, tol, max_iter, sc, st);
 }
 
-template void launch_cg_init<float>(const Grid&, const double*, const float*, bool, float*, float*,
-                                    float*, float*, float*, const float*, Parts, DevScalars*,
-                                    double, int, hipStream_t);
-template void launch_cg_init<double>(const Grid&, const double*, const double*, bool, double*,
-                                     double*, double*, double*, double*, const double*, Parts,
-                                     DevScalars*, double, int, hipStream_t);
+#define GP_INST(T, S)                                                                         \
+  template void launch_cg_init<T, S>(const Grid&, const double*, const T*, bool, S*, T*, T*, \
+                                     T*, S*, const T*, Parts, DevScalars*, double, int,      \
+                                     hipStream_t);
+GP_INST(float, float)
+GP_INST(double, double)
+GP_INST(float, _Float16)
+GP_INST(float, bf16)
+#undef GP_INST
 
 }  // namespace gp

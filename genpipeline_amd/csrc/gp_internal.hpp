@@ -1,6 +1,7 @@
 // Internal declarations shared by the kernel and API translation units.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
 #include <cstdint>
 #include <cstddef>
 #include "../../include/gp_simp.h"
@@ -185,6 +186,27 @@ struct SchedTable {
 // Enqueues one tick with parity t & 1 (tj x tk must be 256 threads).
 bool sched_tick(const SchedTable& tab, int tj, int tk, int blocks, int t, hipStream_t st);
 
+// ------------------------------------------------------ storage types
+// PCG vectors may be stored in a 16-bit type (bf16: FP32 exponent range;
+// _Float16: 11-bit significand, range to 65504) with FP32 arithmetic.
+using bf16 = __hip_bfloat16;
+template <typename T, typename S>
+__device__ __host__ inline T ld_s(S v) {
+  return (T)v;
+}
+template <>
+__device__ __host__ inline float ld_s<float, bf16>(bf16 v) {
+  return __bfloat162float(v);
+}
+template <typename S, typename T>
+__device__ __host__ inline S st_s(T v) {
+  return (S)v;
+}
+template <>
+__device__ __host__ inline bf16 st_s<bf16, float>(float v) {
+  return __float2bfloat16(v);
+}
+
 // Logical block of hardware block b out of n. Workgroups are dispatched
 // round-robin over the XCDs, each with its own L2; this bijection gives every
 // XCD a contiguous range of logical blocks, so neighbouring tiles -- which
@@ -227,11 +249,13 @@ void launch_pcg_init(const Grid& g, const double* f, const T* Ax, bool use_x0, T
                      int max_iter, hipStream_t st);
 void launch_pcg_init_fin(Parts parts, int nV, double tol, int max_iter, DevScalars* sc,
                          hipStream_t st);
-template <typename T>
-void launch_cg_init(const Grid& g, const double* f, const T* Ax, bool use_x0, T* x, T* r, T* s0,
-                    T* w1, T* p, const T* minv, Parts parts, DevScalars* sc, double tol,
+// S: storage of x and p (T, or _Float16 / bf16 with T = float)
+template <typename T, typename S = T>
+void launch_cg_init(const Grid& g, const double* f, const T* Ax, bool use_x0, S* x, T* r, T* s0,
+                    T* w1, S* p, const T* minv, Parts parts, DevScalars* sc, double tol,
                     int max_iter, hipStream_t st);
-// One Jacobi-PCG iteration k (Chronopoulos-Gear form) on the modal sweep:
+// One Jacobi-PCG iteration k (Chronopoulos-Gear form) on the modal sweep
+// (prec: GP_F32 / GP_F64, or GP_F16 / GP_BF16 16-bit storage of x and p):
 // reads r_{k-1} (rp), w_{k-1} (wp), s_{k-2} (sp); writes r_k, w_k, s_{k-1}
 // (rn, wn, sn); updates x and p in place.
 bool launch_cg_modal(const Grid& g, const KuTile& t, const void* kc, int prec, const float* E,
@@ -250,6 +274,11 @@ void launch_residual64(const Grid& g, const double* f, const double* Ku, const u
                        double* r, Parts parts, DevScalars* sc, hipStream_t st);
 template <typename T>
 void launch_add_correction(const Grid& g, double* u, const T* d, hipStream_t st);
+// u += scale * d for 16-bit stored corrections (S = _Float16 or bf16)
+template <typename S>
+void launch_add_correction_lp(const Grid& g, double* u, const S* d, double scale, hipStream_t st);
+// v *= a over all DOFs
+void launch_scale64(const Grid& g, double* v, double a, hipStream_t st);
 void launch_cast_f64_to_f32(const double* a, float* b, long long n, hipStream_t st);
 void launch_cast_f32_to_f64(const float* a, double* b, long long n, hipStream_t st);
 

@@ -332,6 +332,30 @@ void launch_add_correction(const Grid& g, double* u, const T* d, hipStream_t st)
                      g.ndof, u, d);
 }
 
+template <typename S>
+__global__ void k_add_correction_lp(long long n, double* __restrict__ u, const S* __restrict__ d,
+                                    double scale) {
+  const long long i = blockIdx.x * (long long)kBlock + threadIdx.x;
+  if (i < n) u[i] += scale * (double)ld_s<float>(d[i]);
+}
+template <typename S>
+void launch_add_correction_lp(const Grid& g, double* u, const S* d, double scale, hipStream_t st) {
+  hipLaunchKernelGGL((k_add_correction_lp<S>), dim3(grid_blocks(g.ndof)), dim3(kBlock), 0, st,
+                     g.ndof, u, d, scale);
+}
+template void launch_add_correction_lp<_Float16>(const Grid&, double*, const _Float16*, double,
+                                                 hipStream_t);
+template void launch_add_correction_lp<bf16>(const Grid&, double*, const bf16*, double,
+                                             hipStream_t);
+
+__global__ void k_scale64(long long n, double* __restrict__ v, double a) {
+  const long long i = blockIdx.x * (long long)kBlock + threadIdx.x;
+  if (i < n) v[i] *= a;
+}
+void launch_scale64(const Grid& g, double* v, double a, hipStream_t st) {
+  hipLaunchKernelGGL(k_scale64, dim3(grid_blocks(g.ndof)), dim3(kBlock), 0, st, g.ndof, v, a);
+}
+
 __global__ void k_cast64to32(const double* a, float* b, long long n) {
   const long long i = blockIdx.x * (long long)kBlock + threadIdx.x;
   if (i < n) b[i] = (float)a[i];

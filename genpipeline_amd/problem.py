@@ -18,7 +18,9 @@ from . import _lib as L
 
 LAWS = {"clamp": L.GP_LAW_CLAMP, "emin": L.GP_LAW_EMIN}
 OCS = {"cpu": L.GP_OC_CPU, "gpu": L.GP_OC_GPU}
-PRECISIONS = {"fp32": L.GP_F32, "fp64": L.GP_F64, "fp16": L.GP_F16}
+# "fp16" / "bf16": 16-bit PCG vector storage with float arithmetic, only as the
+# inner solve of FP64 iterative refinement (refine=True)
+PRECISIONS = {"fp32": L.GP_F32, "fp64": L.GP_F64, "fp16": L.GP_F16, "bf16": L.GP_BF16}
 
 
 @dataclass(frozen=True)

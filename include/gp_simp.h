@@ -37,8 +37,11 @@ extern "C" {
 /* ---- enums ----------------------------------------------------------- */
 #define GP_F32 0              /* PCG vectors float32, FP64 dots              */
 #define GP_F64 1              /* PCG vectors float64                          */
-#define GP_F16 2              /* p / KE coefficients stored fp16, fp32 math,  */
-                              /* FP64 iterative refinement around it          */
+#define GP_F16 2              /* PCG vectors and KE coefficients stored fp16, */
+                              /* fp32 math, FP64 dots; runs as the inner     */
+                              /* solve of FP64 iterative refinement          */
+#define GP_BF16 3             /* as GP_F16 with bfloat16 storage (the fp32   */
+                              /* exponent range: safe for Emin = 1e-9)        */
 
 #define GP_LAW_CLAMP 0        /* E = max(x,1e-3)^p (simp_solver.py:201-203) */
 #define GP_LAW_EMIN 1         /* E = Emin + x^p (E0-Emin)                   */
@@ -64,7 +67,7 @@ typedef struct gp_spec {
 typedef struct gp_problem gp_problem;   /* opaque */
 
 typedef struct gp_pcg_opts {
-  int32_t precision;          /* GP_F32 / GP_F64 / GP_F16                    */
+  int32_t precision;          /* GP_F32 / GP_F64 / GP_F16 / GP_BF16          */
   int32_t max_iter;           /* inner PCG iteration cap                     */
   double tol;                 /* ||r|| <= tol ||f_free||  (outer when refine) */
   int32_t use_x0;             /* 1: u holds the initial guess               */
