@@ -139,11 +139,26 @@ __device__ inline void reduce3_all(const double* a, const double* b, const doubl
                                    double (&out)[3]) {
   constexpr int NW = (NT + 63) / 64;
   __shared__ double sh[3][NW];
+  // thread t sums entries t, t+NT, t+2NT, ... in that order; each chunk of
+  // four is loaded before any is added, so the loads overlap
+  constexpr int U = 4;
   double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-  for (int i = threadIdx.x; i < n; i += NT) {
-    s0 += a[i];
-    s1 += b[i];
-    s2 += c[i];
+  for (int i0 = threadIdx.x; i0 < n; i0 += U * NT) {
+    double va[U], vb[U], vc[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = i0 + u * NT < n ? i0 + u * NT : i0;
+      va[u] = a[i];
+      vb[u] = b[i];
+      vc[u] = c[i];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (i0 + u * NT < n) {
+        s0 += va[u];
+        s1 += vb[u];
+        s2 += vc[u];
+      }
   }
   s0 = wave_sum64(s0);
   s1 = wave_sum64(s1);
