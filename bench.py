@@ -53,10 +53,13 @@ def parse():
     ap.add_argument("--calib-mib", type=int, default=0,
                     help="first run one elementwise add over a float32 buffer of this many MiB "
                          "(a known-size kernel for scaling HBM counters)")
-    ap.add_argument("--workload", choices=["single64", "batch48", "serve"], default="single64",
+    ap.add_argument("--workload", choices=["single64", "batch48", "serve", "mixed128"],
+                    default="single64",
                     help="single64: one 64^3 problem (default); batch48: 48 mixed-size problems "
                          "on the device scheduler against the same problems one by one; serve: "
-                         "open-loop request serving at 50/200/800 req/s")
+                         "open-loop request serving at 50/200/800 req/s; mixed128: 16-bit PCG "
+                         "storage under FP64 refinement on 128x64x64, errors against FP64")
+    ap.add_argument("--mixed-oc-iters", type=int, default=10)
     ap.add_argument("--batch-n", type=int, default=48)
     ap.add_argument("--batch-oc-iters", type=int, default=60)
     ap.add_argument("--serve-n", type=int, default=60, help="requests per arrival rate")
@@ -152,6 +155,55 @@ def bench_serve(a):
                  extra={"per_rate": per, "peak_problem_bytes": srv.peak_used})
 
 
+def bench_mixed128(a):
+    """C3: 128x64x64 (524,288 elements), distributed traction of total magnitude
+    1 on face x = max, density from seeded smoothed noise (sigma 2) clipped to
+    [0.01, 1] and rescaled to volfrac 0.25; SIMP from that density with FP64
+    refinement to an outer residual of 1e-6 around an inner PCG at 1e-3 whose
+    iterate and search direction are stored in 16 bits (bf16 and fp16), next to
+    the FP32-inner refinement, plain FP32 PCG and the FP64 reference run. The
+    compliance and density errors of each are reported against FP64."""
+    import torch
+    from genpipeline_amd import Problem
+    from genpipeline_amd.problems import mixed_128x64x64
+
+    w = mixed_128x64x64(seed=a.seed, n_iters=a.mixed_oc_iters)
+    dev = torch.device("cuda", 0)
+    P = Problem(w.spec, w.fixed, w.f, device=dev)
+    base = dict(x0=w.x0, xphys0=w.x0, max_iter=20000, max_outer=60, warm_start=True)
+    runs = [("fp64", dict(precision="fp64", tol=1e-10)),
+            ("bf16+ir", dict(precision="bf16", refine=True, tol=1e-6, inner_tol=1e-3)),
+            ("fp16+ir", dict(precision="fp16", refine=True, tol=1e-6, inner_tol=1e-3)),
+            ("fp32+ir", dict(precision="fp32", refine=True, tol=1e-6, inner_tol=1e-3)),
+            ("fp32", dict(precision="fp32", tol=1e-6))]
+    P.run(2, **base, **runs[1][1])  # warm-up of the 16-bit path
+    res = {}
+    for name, kw in runs:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = P.run(w.n_iters, **base, **kw)
+        torch.cuda.synchronize()
+        res[name] = (r, time.perf_counter() - t0)
+        print(f"[bench] mixed128 {name}: {res[name][1]:.2f} s, {r.pcg_iters} PCG its, "
+              f"C={r.compliance:.12g}", file=sys.stderr, flush=True)
+    ref = res["fp64"][0]
+    per = {}
+    for name, (r, dt) in res.items():
+        per[name] = {"seconds": dt, "solves_per_s": 1.0 / dt, "pcg_iters": r.pcg_iters,
+                     "status": r.status, "compliance": r.compliance,
+                     "compliance_rel_err_vs_fp64": abs(r.compliance - ref.compliance)
+                     / abs(ref.compliance),
+                     "density_max_abs_err_vs_fp64": float((r.xPhys - ref.xPhys).abs().max())}
+    P.close()
+    r, dt = res["bf16+ir"]
+    return _line("SIMP solves/sec, 16-bit PCG storage + FP64 refinement (128x64x64)", 1.0 / dt,
+                 "solves/s", 1, 1, dt * 1e3, "bf16-storage/f32-arith/f64-refine",
+                 {"workload": f"simp_mixed_128x64x64_seed{a.seed}", "grid": [128, 64, 64],
+                  "oc_iters": w.n_iters, "outer_tol": 1e-6, "inner_tol": 1e-3,
+                  "load": "traction, total 1, face x=max", "volfrac": 0.25},
+                 extra={"runs": per})
+
+
 def cpu_baseline(w, gpu_pcg_iters, gpu_oc_iters, n_sample):
     import numpy as np
     from oracle import cpu_ref, simp_oracle as O
@@ -204,6 +256,9 @@ def main():
         return
     if a.workload == "serve":
         print(json.dumps(bench_serve(a)), flush=True)
+        return
+    if a.workload == "mixed128":
+        print(json.dumps(bench_mixed128(a)), flush=True)
         return
     import torch
     from genpipeline_amd import Problem
