@@ -18,12 +18,16 @@ struct NodeCG {
   T r[3], s[3], z[3], zp[3];
 };
 
-// Raw inputs of one node (r_{k-1}, w_{k-1}, s_{k-2}, M), zero outside the grid.
+// Raw inputs of one node (r_{k-1}, w_{k-1}, s_{k-2}, M).
 template <typename T>
 struct RawNode {
   T r[3], w[3], s[3], m[3];
 };
 
+// A node outside the grid reads node 0 instead: such a node is a corner only
+// of elements outside the grid, whose modulus is exactly 0, and it is never
+// stored or summed, so its (finite) values do not reach any result. Leaving
+// the loaded values unselected lets all of a node's loads issue back to back.
 template <typename T>
 __device__ inline void load_node(const Grid& g, const T* __restrict__ rp, const T* __restrict__ wp,
                                  const T* __restrict__ sp, const T* __restrict__ m, int ip, int jj,
@@ -33,11 +37,10 @@ __device__ inline void load_node(const Grid& g, const T* __restrict__ rp, const 
   const long long n = ok ? (long long)ip * g.sI + (long long)jj * g.sJ + kk : 0;
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
-    const T vr = rp[3 * n + q], vw = wp[3 * n + q], vs = sp[3 * n + q], vm = m[3 * n + q];
-    o.r[q] = ok ? vr : T(0);
-    o.w[q] = ok ? vw : T(0);
-    o.s[q] = ok ? vs : T(0);
-    o.m[q] = ok ? vm : T(0);
+    o.r[q] = rp[3 * n + q];
+    o.w[q] = wp[3 * n + q];
+    o.s[q] = sp[3 * n + q];
+    o.m[q] = m[3 * n + q];
   }
 }
 
@@ -100,6 +103,14 @@ __device__ inline void block_sum3_to(double v0, double v1, double v2, double* p0
   }
 }
 
+// Inputs of one node plane that do not depend on alpha, beta.
+template <typename T>
+struct StageIn {
+  RawNode<T> ro, rh;  // owned node, halo node
+  T pv[3], xv[3];     // p, x of the owned node (owned planes only)
+  uint8_t fm;         // its support bits
+};
+
 // Node loading: thread (a,b) updates its own node column (J0+a, K0+b); the
 // TJ+TK+1 halo columns with aa = 0 or bb = 0 are spread over the first
 // threads. Each plane's z goes to an LDS tile; element columns then read
@@ -110,7 +121,7 @@ __device__ inline void block_sum3_to(double v0, double v1, double v2, double* p0
 // S: storage of the iterate x and the search direction p. These two feed only
 // the x update, so a 16-bit S limits the accuracy of x (recovered by FP64
 // refinement) without disturbing the recurrence.
-template <typename T, int TJ, int TK, typename S = T>
+template <typename T, int TJ, int TK, typename S = T, bool PF = false>
 __device__ inline bool body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
                                      const ModalCoef<T>& kc, const float* __restrict__ E,
                                      const T* __restrict__ rp, const T* __restrict__ wp,
@@ -177,31 +188,92 @@ __device__ inline bool body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
     const float v = E[e];
     return ok ? (T)v : T(0);
   };
-  // updates plane ip's own and halo nodes and publishes their z in zs[buf]
-  // updates plane ip's own and halo nodes and publishes their z in zs[buf];
-  // both nodes' loads are issued before either is used, so the halo threads
-  // (all in wave 0) wait for one memory latency per plane, not two
-  auto stage = [&](int ip, int buf, NodeCG<T>& own) {
-    RawNode<T> ro, rh;
-    load_node(g, rp, wp, sp, minv, ip, je + 1, ke + 1, ro);
-    if (has_h) load_node(g, rp, wp, sp, minv, ip, J0 - 1 + ha, K0 - 1 + hb, rh);
-    finish_node(ro, alpha, beta, own);
+  // Updates plane ip's own and halo nodes and publishes their z in zs[buf].
+  // When the plane is owned, the owner finishes its node's iteration here:
+  // p_{k-1} = z_{k-1} + beta p_{k-2}, x_k = x_{k-1} + alpha p_{k-1}, r_k and
+  // s_{k-1} are stored, and (r,z), (r,r) accumulated; only z (for (z,Az)) and
+  // the support bits are kept for the end of the step that owns the plane.
+  // The own and halo loads are issued before either is used, so the halo
+  // threads (all in wave 0) wait for one memory latency per plane, not two.
+  double gk = 0.0, dk = 0.0, rk = 0.0;
+  // Loads of one plane that do not depend on alpha, beta: the owned and halo
+  // node, and (when the plane is owned) p, x and the support bits.
+  auto load_stage = [&](int ip, bool own_plane, StageIn<T>& in) {
+    const bool fin = owner && own_plane;
+    const long long n = (long long)ip * g.sI + (long long)(je + 1) * g.sJ + (ke + 1);
+    load_node(g, rp, wp, sp, minv, ip, je + 1, ke + 1, in.ro);
+    if (has_h) load_node(g, rp, wp, sp, minv, ip, J0 - 1 + ha, K0 - 1 + hb, in.rh);
 #pragma unroll
-    for (int q = 0; q < 3; ++q) zs[buf][q][a + 1][b + 1] = own.z[q];
+    for (int q = 0; q < 3; ++q) in.pv[q] = in.xv[q] = T(0);
+    in.fm = 0;
+    if (fin) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        in.pv[q] = ld_s<T>(p[3 * n + q]);
+        in.xv[q] = ld_s<T>(x[3 * n + q]);
+      }
+      if (fixmask != nullptr) in.fm = fixmask[n];
+    }
+  };
+  // Updates plane ip's own and halo nodes and publishes their z in zs[buf].
+  // When the plane is owned, the owner finishes its node's iteration here:
+  // p_{k-1} = z_{k-1} + beta p_{k-2}, x_k = x_{k-1} + alpha p_{k-1}, r_k and
+  // s_{k-1} are stored, and (r,z), (r,r) accumulated; only z (for (z,Az)) and
+  // the support bits are kept for the end of the step that owns the plane.
+  auto finish_stage = [&](int ip, bool own_plane, const StageIn<T>& in, int buf, T (&zo)[3]) {
+    const bool fin = owner && own_plane;
+    const long long n = (long long)ip * g.sI + (long long)(je + 1) * g.sJ + (ke + 1);
+    NodeCG<T> o;
+    finish_node(in.ro, alpha, beta, o);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      zs[buf][q][a + 1][b + 1] = o.z[q];
+      zo[q] = o.z[q];
+    }
     if (has_h) {
       NodeCG<T> h;
-      finish_node(rh, alpha, beta, h);
+      finish_node(in.rh, alpha, beta, h);
 #pragma unroll
       for (int q = 0; q < 3; ++q) zs[buf][q][ha][hb] = h.z[q];
     }
+    if (fin) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const T pk = o.zp[q] + beta * in.pv[q];  // p_{k-1}
+        const T xk = in.xv[q] + alpha * pk;      // x_k
+        p[3 * n + q] = st_s<S>(pk);
+        x[3 * n + q] = st_s<S>(xk);
+        rn[3 * n + q] = o.r[q];
+        sn[3 * n + q] = o.s[q];
+        gk += (double)o.r[q] * (double)o.z[q];
+        rk += (double)o.r[q] * (double)o.r[q];
+      }
+    }
   };
 
+  const int iend = min(I0 + NI, g.nx + 1);
   Plane<T> Pold, Pnew, topp, bot, top;
-  NodeCG<T> own_cur, own_nxt;
+  T zc[3], zn[3];            // z of the owned node on the current / next plane
+  uint8_t fmc = 0, fmn = 0;  // its support bits
+  StageIn<T> nxt;            // PF: inputs of the next plane to stage
   {
-    NodeCG<T> own_m1;
-    stage(I0 - 1, 0, own_m1);
-    stage(I0, 1, own_cur);
+    T zm1[3];
+    if constexpr (PF) {
+      StageIn<T> i0, i1;
+      load_stage(I0 - 1, false, i0);
+      load_stage(I0, true, i1);
+      if (I0 < g.nx) load_stage(I0 + 1, I0 + 1 < iend, nxt);
+      finish_stage(I0 - 1, false, i0, 0, zm1);
+      finish_stage(I0, true, i1, 1, zc);
+      fmc = i1.fm;
+    } else {
+      StageIn<T> in;
+      load_stage(I0 - 1, false, in);
+      finish_stage(I0 - 1, false, in, 0, zm1);
+      load_stage(I0, true, in);
+      finish_stage(I0, true, in, 1, zc);
+      fmc = in.fm;
+    }
     __syncthreads();
     corners_transform<T, TJ, TK>(zs[0], a, b, Pold);
     corners_transform<T, TJ, TK>(zs[1], a, b, Pnew);
@@ -210,26 +282,21 @@ __device__ inline bool body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
     Pold = Pnew;
   }
 
-  double gk = 0.0, dk = 0.0, rk = 0.0;
-  const int iend = min(I0 + NI, g.nx + 1);
   for (int ie = I0, t = 0; ie < iend; ++ie, ++t) {
     const T Ee = Eat(ie);
     const int zb = t & 1;  // buffer of plane ie+1 (planes I0-1, I0 used 0, 1)
-    // the owned node's p and x are loaded first: in flight across the plane's
-    // loads, barriers and LDS work, used only at the end of the step
-    T pv[3] = {T(0), T(0), T(0)}, xv[3] = {T(0), T(0), T(0)};
-    const long long nown = (long long)ie * g.sI + (long long)(je + 1) * g.sJ + (ke + 1);
-    if (owner) {
-#pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        pv[q] = ld_s<T>(p[3 * nown + q]);
-        xv[q] = ld_s<T>(x[3 * nown + q]);
-      }
-    }
     if (ie < g.nx) {
       // zs[zb] was last read two steps earlier (or in the prologue), before a
       // barrier every thread has since passed
-      stage(ie + 1, zb, own_nxt);
+      StageIn<T> cur;
+      if constexpr (PF) {
+        cur = nxt;
+        if (ie + 1 < iend && ie + 1 < g.nx) load_stage(ie + 2, ie + 2 < iend, nxt);
+      } else {
+        load_stage(ie + 1, ie + 1 < iend, cur);
+      }
+      finish_stage(ie + 1, ie + 1 < iend, cur, zb, zn);
+      fmn = cur.fm;
       __syncthreads();
       corners_transform<T, TJ, TK>(zs[zb], a, b, Pnew);
       element(kc, Ee, Pold, Pnew, bot, top);
@@ -246,33 +313,24 @@ __device__ inline bool body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
     faces_to_corners<T, NT>(topp, bot, cv[par], tid);
     __syncthreads();
     if (owner) {
-      const long long n = nown;
+      const long long n = (long long)ie * g.sI + (long long)(je + 1) * g.sJ + (ke + 1);
       T w[3];
 #pragma unroll
       for (int q = 0; q < 3; ++q) w[q] = gather_node<T, TK, NT>(cv[par], q, tid);
-      if (fixmask != nullptr) {
-        const uint8_t fm = fixmask[n];
-        if (fm & 1) w[0] = 0;
-        if (fm & 2) w[1] = 0;
-        if (fm & 4) w[2] = 0;
-      }
+      if (fmc & 1) w[0] = 0;
+      if (fmc & 2) w[1] = 0;
+      if (fmc & 4) w[2] = 0;
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
-        const T pk = own_cur.zp[q] + beta * pv[q];  // p_{k-1}
-        const T xk = xv[q] + alpha * pk;            // x_k
-        p[3 * n + q] = st_s<S>(pk);
-        x[3 * n + q] = st_s<S>(xk);
-        rn[3 * n + q] = own_cur.r[q];
-        sn[3 * n + q] = own_cur.s[q];
         wn[3 * n + q] = w[q];
-        gk += (double)own_cur.r[q] * (double)own_cur.z[q];
-        dk += (double)own_cur.z[q] * (double)w[q];
-        rk += (double)own_cur.r[q] * (double)own_cur.r[q];
+        dk += (double)zc[q] * (double)w[q];
       }
     }
     topp = top;
     Pold = Pnew;
-    own_cur = own_nxt;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) zc[q] = zn[q];
+    fmc = fmn;
   }
   block_sum3_to<NT>(gk, dk, rk, parts.G[k & 1] + lb, parts.D[k & 1] + lb, parts.R[k & 1] + lb);
   return false;
