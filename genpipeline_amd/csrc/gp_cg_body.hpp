@@ -133,7 +133,6 @@ __device__ inline bool body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
                                      CornerLds<T, TJ * TK>& cv) {
   constexpr int NT = TJ * TK;
   constexpr int NH = TJ + TK + 1;  // halo node columns (aa = 0 or bb = 0)
-  if (sc->done) return true;
   __shared__ T zs[2][3][TJ + 1][TK + 1];
   const int tid = threadIdx.x;
   const int a = tid / TK, b = tid - (tid / TK) * TK;
@@ -152,49 +151,12 @@ __device__ inline bool body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
   const int hb = tid <= TK ? tid : 0;
 
   T alpha = T(0), beta = T(0);
-  if (k >= 1) {
-    const int q = (k - 1) & 1;
-    double s3[3];
-    reduce3_all<NT>(parts.G[q], parts.D[q], parts.R[q], nb, s3);  // gamma, delta, rho
-    const double gam = s3[0], del = s3[1], rho = s3[2];
-    const double bet = (k == 1) ? 0.0 : gam / sc->cgg[k & 1];
-    const double den = (k == 1) ? del : del - bet * gam / sc->cga[k & 1];
-    const bool conv = rho <= sc->tol2;
-    const bool cap = k - 1 >= sc->max_iter;
-    const bool brk = !(den > 0.0);
-    if (conv || cap || brk) {
-      if (lb == 0 && tid == 0) {
-        sc->iter = k - 1;
-        sc->rr = rho;
-        sc->status = conv ? GP_OK : (brk ? GP_BREAKDOWN : GP_NOT_CONVERGED);
-        sc->done = 1;
-      }
-      return true;
-    }
-    const double alp = gam / den;
-    if (lb == 0 && tid == 0) {
-      sc->cgg[(k - 1) & 1] = gam;
-      sc->cga[(k - 1) & 1] = alp;
-      sc->iter = k - 1;
-      sc->rr = rho;
-    }
-    alpha = (T)alp;
-    beta = (T)bet;
-  }
-
   auto Eat = [&](int ie) -> T {
     const bool ok = col_ok && (unsigned)ie < (unsigned)g.nx;
     const int e = ok ? ie * g.eI + je * g.eJ + ke : 0;
     const float v = E[e];
     return ok ? (T)v : T(0);
   };
-  // Updates plane ip's own and halo nodes and publishes their z in zs[buf].
-  // When the plane is owned, the owner finishes its node's iteration here:
-  // p_{k-1} = z_{k-1} + beta p_{k-2}, x_k = x_{k-1} + alpha p_{k-1}, r_k and
-  // s_{k-1} are stored, and (r,z), (r,r) accumulated; only z (for (z,Az)) and
-  // the support bits are kept for the end of the step that owns the plane.
-  // The own and halo loads are issued before either is used, so the halo
-  // threads (all in wave 0) wait for one memory latency per plane, not two.
   double gk = 0.0, dk = 0.0, rk = 0.0;
   // Loads of one plane that do not depend on alpha, beta: the owned and halo
   // node, and (when the plane is owned) p, x and the support bits.
@@ -252,6 +214,43 @@ __device__ inline bool body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
   };
 
   const int iend = min(I0 + NI, g.nx + 1);
+  // The first two planes' loads do not depend on the previous iteration's
+  // reduction: they are issued before it (and before the finished-flag
+  // check), so their latency overlaps that of the partials.
+  StageIn<T> i0, i1;
+  load_stage(I0 - 1, false, i0);
+  load_stage(I0, true, i1);
+  if (sc->done) return true;
+  if (k >= 1) {
+    const int q = (k - 1) & 1;
+    double s3[3];
+    reduce3_all<NT>(parts.G[q], parts.D[q], parts.R[q], nb, s3);  // gamma, delta, rho
+    const double gam = s3[0], del = s3[1], rho = s3[2];
+    const double bet = (k == 1) ? 0.0 : gam / sc->cgg[k & 1];
+    const double den = (k == 1) ? del : del - bet * gam / sc->cga[k & 1];
+    const bool conv = rho <= sc->tol2;
+    const bool cap = k - 1 >= sc->max_iter;
+    const bool brk = !(den > 0.0);
+    if (conv || cap || brk) {
+      if (lb == 0 && tid == 0) {
+        sc->iter = k - 1;
+        sc->rr = rho;
+        sc->status = conv ? GP_OK : (brk ? GP_BREAKDOWN : GP_NOT_CONVERGED);
+        sc->done = 1;
+      }
+      return true;
+    }
+    const double alp = gam / den;
+    if (lb == 0 && tid == 0) {
+      sc->cgg[(k - 1) & 1] = gam;
+      sc->cga[(k - 1) & 1] = alp;
+      sc->iter = k - 1;
+      sc->rr = rho;
+    }
+    alpha = (T)alp;
+    beta = (T)bet;
+  }
+
   Plane<T> Pold, Pnew, topp, bot, top;
   T zc[3], zn[3];            // z of the owned node on the current / next plane
   uint8_t fmc = 0, fmn = 0;  // its support bits
@@ -259,21 +258,11 @@ __device__ inline bool body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
   {
     T zm1[3];
     if constexpr (PF) {
-      StageIn<T> i0, i1;
-      load_stage(I0 - 1, false, i0);
-      load_stage(I0, true, i1);
       if (I0 < g.nx) load_stage(I0 + 1, I0 + 1 < iend, nxt);
-      finish_stage(I0 - 1, false, i0, 0, zm1);
-      finish_stage(I0, true, i1, 1, zc);
-      fmc = i1.fm;
-    } else {
-      StageIn<T> in;
-      load_stage(I0 - 1, false, in);
-      finish_stage(I0 - 1, false, in, 0, zm1);
-      load_stage(I0, true, in);
-      finish_stage(I0, true, in, 1, zc);
-      fmc = in.fm;
     }
+    finish_stage(I0 - 1, false, i0, 0, zm1);
+    finish_stage(I0, true, i1, 1, zc);
+    fmc = i1.fm;
     __syncthreads();
     corners_transform<T, TJ, TK>(zs[0], a, b, Pold);
     corners_transform<T, TJ, TK>(zs[1], a, b, Pnew);
