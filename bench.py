@@ -53,12 +53,15 @@ def parse():
     ap.add_argument("--calib-mib", type=int, default=0,
                     help="first run one elementwise add over a float32 buffer of this many MiB "
                          "(a known-size kernel for scaling HBM counters)")
-    ap.add_argument("--workload", choices=["single64", "batch48", "serve", "mixed128"],
+    ap.add_argument("--workload", choices=["single64", "batch48", "serve", "mixed128", "ku"],
                     default="single64",
                     help="single64: one 64^3 problem (default); batch48: 48 mixed-size problems "
                          "on the device scheduler against the same problems one by one; serve: "
                          "open-loop request serving at 50/200/800 req/s; mixed128: 16-bit PCG "
-                         "storage under FP64 refinement on 128x64x64, errors against FP64")
+                         "storage under FP64 refinement on 128x64x64, errors against FP64; ku: "
+                         "K.u and one PCG iteration on a grid larger than the Infinity Cache, "
+                         "beside a device copy")
+    ap.add_argument("--ku-grid", type=int, nargs=3, default=[256, 256, 256])
     ap.add_argument("--mixed-oc-iters", type=int, default=10)
     ap.add_argument("--batch-n", type=int, default=48)
     ap.add_argument("--batch-oc-iters", type=int, default=60)
@@ -206,6 +209,80 @@ def bench_mixed128(a):
                  extra={"runs": per})
 
 
+def _event_us(fn, reps):
+    """Average microseconds of fn() over reps back-to-back calls, from HIP
+    events on the current stream (fn launches on that stream)."""
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        fn()
+    b.record()
+    b.synchronize()
+    return a.elapsed_time(b) * 1e3 / reps
+
+
+def bench_ku(a):
+    """K.u on a grid whose operands do not fit the 256 MB Infinity Cache (default
+    256^3: u and y are 204 MB each in FP32, E 67 MB), so it streams from HBM.
+    Reported beside one fused PCG iteration on the same grid and a 1 GiB device
+    copy (the bandwidth HBM delivers to a plain read + write stream). Every
+    modal tile and sweep depths 4/8/16 are tried; the best is reported."""
+    import numpy as np
+    import torch
+    from genpipeline_amd import Problem, ProblemSpec, autotune
+    from genpipeline_amd.bcs import clamp_x0
+
+    dev = torch.device("cuda", 0)
+    n = (1 << 30) // 4
+    src = torch.ones(n, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    copy_us = _event_us(lambda: dst.copy_(src), 20)
+    copy_gbs = 2 * 4 * n / (copy_us * 1e-6) / 1e9
+    del src, dst
+    nx, ny, nz = a.ku_grid
+    spec = ProblemSpec(nx, ny, nz)
+    f = np.zeros(spec.n_dof)
+    f[-1] = -1.0
+    P = Problem(spec, clamp_x0(nx, ny, nz), f, device=dev)
+    rng = np.random.default_rng(0)
+    E = torch.from_numpy(rng.uniform(1e-3, 1.0, spec.n_elem).astype(np.float32)).to(dev)
+    u = torch.from_numpy(rng.standard_normal(spec.n_dof).astype(np.float32)).to(dev)
+    ku_bytes = 24 * spec.n_nodes + 4 * spec.n_elem
+    it_bytes = 120 * spec.n_nodes + 4 * spec.n_elem
+    ku_t, it_t = {}, {}
+    for tj, tk in autotune.TILES:
+        for ni in (4, 8, 16):
+            P.set_ku_tile(tj, tk, ni, precision="fp32")
+            ku_t[(tj, tk, ni)] = _event_us(lambda: P.ku(E, u), 10)
+            it_t[(tj, tk, ni)] = autotune.time_pcg(P, E, (tj, tk, ni), "fp32", iters=60, repeats=1)
+    kb = min(ku_t, key=ku_t.get)
+    ib = min(it_t, key=it_t.get)
+    P.close()
+    ku_gbs = ku_bytes / (ku_t[kb] * 1e-6) / 1e9
+    it_gbs = it_bytes / (it_t[ib] * 1e-6) / 1e9
+    print(f"[bench] ku {nx}x{ny}x{nz}: copy {copy_gbs:.0f} GB/s; K.u {kb} {ku_t[kb]:.1f} us "
+          f"{ku_gbs:.0f} GB/s; PCG iteration {ib} {it_t[ib]:.1f} us {it_gbs:.0f} GB/s",
+          file=sys.stderr, flush=True)
+    return _line("K·u GB/s vs HBM peak (HBM-resident grid)", ku_gbs, "GB/s", 10, 1,
+                 ku_t[kb] * 1e-3, "f32",
+                 {"workload": f"ku_{nx}x{ny}x{nz}_modal", "grid": [nx, ny, nz],
+                  "ku_tile": list(kb), "pcg_tile": list(ib)},
+                 roofline={"bound": "hbm", "kernel": "k_ku_modal<float> (y = K(E) u, fixed DOFs "
+                           "zeroed)", "achieved": ku_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": ku_gbs / HBM_PEAK_GBS, "traffic": None,
+                           "algorithmic_bytes_per_launch": ku_bytes, "avg_launch_us": ku_t[kb]},
+                 extra={"device_copy_gbs": copy_gbs, "device_copy_frac_of_peak":
+                        copy_gbs / HBM_PEAK_GBS, "ku_frac_of_copy": ku_gbs / copy_gbs,
+                        "pcg_iteration_us": it_t[ib], "pcg_iteration_gbs": it_gbs,
+                        "pcg_iteration_frac_of_peak": it_gbs / HBM_PEAK_GBS,
+                        "pcg_algorithmic_bytes": it_bytes,
+                        "ku_us_by_tile": {f"{k[0]}x{k[1]}x{k[2]}": v for k, v in ku_t.items()},
+                        "pcg_us_by_tile": {f"{k[0]}x{k[1]}x{k[2]}": v for k, v in it_t.items()}})
+
+
 def cpu_baseline(w, gpu_pcg_iters, gpu_oc_iters, n_sample):
     import numpy as np
     from oracle import cpu_ref, simp_oracle as O
@@ -261,6 +338,9 @@ def main():
         return
     if a.workload == "mixed128":
         print(json.dumps(bench_mixed128(a)), flush=True)
+        return
+    if a.workload == "ku":
+        print(json.dumps(bench_ku(a)), flush=True)
         return
     import torch
     from genpipeline_amd import Problem

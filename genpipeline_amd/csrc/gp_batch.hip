@@ -45,8 +45,8 @@ __global__ __launch_bounds__(TJ* TK) void k_b_ku(BatchLaunch bl) {
   GP_BATCH_BLOCK(off_cg)
   (void)nb;
   __shared__ CornerLds<float, TJ * TK> cv;
-  body_ku_modal<float, TJ, TK, 0>(P.g, P.ni, P.ntj, P.ntk, P.mc, P.E, P.u, nullptr, P.wb[0],
-                                  nullptr, P.fixmask, P.sc, P.parts, 0, 0, lb, cv);
+  body_ku_sweep<float, TJ, TK>(P.g, P.ni, P.ntj, P.ntk, P.mc, P.E, P.u, P.wb[0], P.fixmask, lb,
+                               cv);
 }
 
 template <int TJ, int TK>

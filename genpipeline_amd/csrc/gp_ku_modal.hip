@@ -32,8 +32,12 @@ __global__ __launch_bounds__(TJ* TK) void k_ku_modal(Grid g, int NI, int ntj, in
                                                     DevScalars* sc, Parts parts, int it,
                                                     int nV) {
   __shared__ CornerLds<T, TJ * TK> cv;
-  body_ku_modal<T, TJ, TK, MODE>(g, NI, ntj, ntk, kc, E, u, u2, y, pout, fixmask, sc, parts, it,
-                                 nV, xcd_block(blockIdx.x, gridDim.x), cv);
+  if constexpr (MODE == 0)
+    body_ku_sweep<T, TJ, TK>(g, NI, ntj, ntk, kc, E, u, y, fixmask,
+                             xcd_block(blockIdx.x, gridDim.x), cv);
+  else
+    body_ku_modal<T, TJ, TK, MODE>(g, NI, ntj, ntk, kc, E, u, u2, y, pout, fixmask, sc, parts, it,
+                                   nV, xcd_block(blockIdx.x, gridDim.x), cv);
 }
 
 template <int TJ, int TK>

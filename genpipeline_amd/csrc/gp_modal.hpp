@@ -28,6 +28,10 @@ struct Raw {
   T b[PCG ? 2 : 1][2][3];
 };
 
+// A corner outside the grid reads node 0 instead: it is a corner only of
+// elements outside the grid, whose modulus is exactly 0, and it is never
+// stored or summed (PCG mode writes and sums the owned node only). Leaving
+// the loaded values unselected lets the corner loads issue back to back.
 template <typename T, bool PCG>
 __device__ inline void load_raw(const Grid& g, const T* __restrict__ u, const T* __restrict__ u2,
                                 int ip, int je, int ke, Raw<T, PCG>& out) {
@@ -41,12 +45,8 @@ __device__ inline void load_raw(const Grid& g, const T* __restrict__ u, const T*
       const long long n = ok ? (long long)ip * g.sI + (long long)jj * g.sJ + kk : 0;
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
-        const T v = u[3 * n + q];
-        out.a[cj][ck][q] = ok ? v : T(0);
-        if (PCG) {
-          const T w = u2[3 * n + q];
-          out.b[cj & (PCG ? 1 : 0)][ck][q] = ok ? w : T(0);
-        }
+        out.a[cj][ck][q] = u[3 * n + q];
+        if (PCG) out.b[cj & (PCG ? 1 : 0)][ck][q] = u2[3 * n + q];
       }
     }
 }
@@ -350,6 +350,129 @@ __device__ inline void body_ku_modal(const Grid& g, int NI, int ntj, int ntk,
       for (int w = 0; w < (NT + 63) / 64; ++w) s += red[w];
       parts.A[lb] = s;
     }
+  }
+}
+
+// y = K(E) u (fixed DOFs zeroed when fixmask is given) with one load per node:
+// thread (a,b) loads its own node column (J0+a, K0+b) and the TJ+TK+1 halo
+// columns with aa = 0 or bb = 0 go to the first threads; element columns read
+// their four corners from an LDS tile of the plane. The arithmetic, and each
+// node's summation order, are those of body_ku_modal in MODE 0, so the result
+// is the same bit for bit; each node is read once per plane instead of once
+// per element column that touches it.
+template <typename T, int TJ, int TK>
+__device__ inline void body_ku_sweep(const Grid& g, int NI, int ntj, int ntk,
+                                     const ModalCoef<T>& kc, const float* __restrict__ E,
+                                     const T* __restrict__ u, T* __restrict__ y,
+                                     const uint8_t* __restrict__ fixmask, int lb,
+                                     CornerLds<T, TJ * TK>& cv) {
+  constexpr int NT = TJ * TK;
+  constexpr int NH = TJ + TK + 1;
+  __shared__ T zs[2][3][TJ + 1][TK + 1];
+  const int tid = threadIdx.x;
+  const int a = tid / TK, b = tid - (tid / TK) * TK;
+  int bid = lb;
+  const int tk_ = bid % ntk;
+  bid /= ntk;
+  const int tj_ = bid % ntj;
+  const int ti_ = bid / ntj;
+  const int J0 = tj_ * (TJ - 1), K0 = tk_ * (TK - 1), I0 = ti_ * NI;
+  const int je = J0 - 1 + a, ke = K0 - 1 + b;
+  const bool col_ok = (unsigned)je < (unsigned)g.ny && (unsigned)ke < (unsigned)g.nz;
+  const bool owner = a < TJ - 1 && b < TK - 1 && (je + 1) <= g.ny && (ke + 1) <= g.nz;
+  const bool has_h = tid < NH;
+  const int ha = tid <= TK ? 0 : tid - TK;
+  const int hb = tid <= TK ? tid : 0;
+
+  auto Eat = [&](int ie) -> T {
+    const bool ok = col_ok && (unsigned)ie < (unsigned)g.nx;
+    const int e = ok ? ie * g.eI + je * g.eJ + ke : 0;
+    const float v = E[e];
+    return ok ? (T)v : T(0);
+  };
+  // a node outside the grid reads node 0: it is a corner only of elements
+  // outside the grid, whose modulus is exactly 0
+  auto node = [&](int ip, int jj, int kk, T (&v)[3]) {
+    const bool ok = (unsigned)ip <= (unsigned)g.nx && (unsigned)jj <= (unsigned)g.ny &&
+                    (unsigned)kk <= (unsigned)g.nz;
+    const long long n = ok ? (long long)ip * g.sI + (long long)jj * g.sJ + kk : 0;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) v[q] = u[3 * n + q];
+  };
+  auto stage = [&](int ip, int buf) {
+    T vo[3], vh[3];
+    node(ip, je + 1, ke + 1, vo);
+    if (has_h) node(ip, J0 - 1 + ha, K0 - 1 + hb, vh);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) zs[buf][q][a + 1][b + 1] = vo[q];
+    if (has_h) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) zs[buf][q][ha][hb] = vh[q];
+    }
+  };
+  // 2-D transform of the element column's four corners from the LDS tile
+  auto corners = [&](int buf, Plane<T>& out) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const T c00 = zs[buf][q][a][b], c01 = zs[buf][q][a][b + 1];
+      const T c10 = zs[buf][q][a + 1][b], c11 = zs[buf][q][a + 1][b + 1];
+      const T a0 = c00 + c01, d0 = c01 - c00;
+      const T a1 = c10 + c11, d1 = c11 - c10;
+      out.P[q][0] = a0 + a1;
+      out.P[q][1] = d0 + d1;
+      out.P[q][2] = a1 - a0;
+      out.P[q][3] = d1 - d0;
+    }
+  };
+
+  Plane<T> Pold, Pnew, topp, bot, top;
+  stage(I0 - 1, 0);
+  stage(I0, 1);
+  __syncthreads();
+  corners(0, Pold);
+  corners(1, Pnew);
+  __syncthreads();  // step 0 rewrites zs[0]
+  element(kc, Eat(I0 - 1), Pold, Pnew, bot, topp);
+  Pold = Pnew;
+  const int iend = min(I0 + NI, g.nx + 1);
+  for (int ie = I0, t = 0; ie < iend; ++ie, ++t) {
+    const T Ee = Eat(ie);
+    const int zb = t & 1;
+    if (ie < g.nx) {
+      // zs[zb] was last read two steps earlier (or in the prologue), before a
+      // barrier every thread has since passed
+      stage(ie + 1, zb);
+      __syncthreads();
+      corners(zb, Pnew);
+      element(kc, Ee, Pold, Pnew, bot, top);
+    } else {
+#pragma unroll
+      for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+          bot.P[q][m] = 0;
+          top.P[q][m] = 0;
+        }
+    }
+    const int par = t & 1;
+    faces_to_corners<T, NT>(topp, bot, cv[par], tid);
+    __syncthreads();
+    if (owner) {
+      const long long n = (long long)ie * g.sI + (long long)(je + 1) * g.sJ + (ke + 1);
+      T o[3];
+#pragma unroll
+      for (int q = 0; q < 3; ++q) o[q] = gather_node<T, TK, NT>(cv[par], q, tid);
+      if (fixmask != nullptr) {
+        const uint8_t fm = fixmask[n];
+        if (fm & 1) o[0] = 0;
+        if (fm & 2) o[1] = 0;
+        if (fm & 4) o[2] = 0;
+      }
+#pragma unroll
+      for (int q = 0; q < 3; ++q) y[3 * n + q] = o[q];
+    }
+    topp = top;
+    Pold = Pnew;
   }
 }
 

@@ -184,8 +184,8 @@ __global__ __launch_bounds__(TJ* TK, 4) void k_tick(SchedTable tab, int t) {
       body_diag<float>(P.g, P.ke32, P.E, P.fixmask, P.minv, 1, lb, nb);
       break;
     case PH_KX0:
-      body_ku_modal<float, TJ, TK, 0>(P.g, P.ni, P.ntj, P.ntk, P.mc, P.E, P.u, nullptr, P.wb[0],
-                                      nullptr, P.fixmask, P.sc, P.parts, 0, 0, lb, cv);
+      body_ku_sweep<float, TJ, TK>(P.g, P.ni, P.ntj, P.ntk, P.mc, P.E, P.u, P.wb[0], P.fixmask,
+                                   lb, cv);
       break;
     case PH_CGINIT: {
       const int x0 = sched_use_x0(cur, S.cfg) ? 1 : 0;
