@@ -42,8 +42,6 @@ __device__ inline void sched_after_update(SchedState& nx, const SchedCfg& c) {
     nx.solve = nx.oc_it < c.n_iters ? SK_OC : SK_FINAL;
 }
 
-// Occupancy floor of 4 waves per SIMD: the 128-register budget the PCG sweep
-// fits in, kept for every phase of the kernel.
 // Next state of a slot after tick work; run by the slot's lead thread only.
 // fin: the PCG solve (PH_CG) or the bisection (PH_OCSTEP) finished this tick.
 __device__ inline SchedState sched_next(const SchedState& cur, const SchedCfg& C,
@@ -156,7 +154,8 @@ __device__ inline SchedState sched_next(const SchedState& cur, const SchedCfg& C
 }
 
 // Occupancy floor of 4 waves per SIMD: the 128-register budget the PCG sweep
-// fits in, kept for every phase of the kernel.
+// fits in, kept for every phase of the kernel (the FP64 sensitivity phase
+// spills a little; a fourth wave per SIMD is worth more to the sweep).
 template <int TJ, int TK>
 __global__ __launch_bounds__(TJ* TK, 4) void k_tick(SchedTable tab, int t) {
   static_assert(TJ * TK == kBlock, "the scheduler runs every body with kBlock threads");
