@@ -67,6 +67,8 @@ def parse():
     ap.add_argument("--batch-oc-iters", type=int, default=60)
     ap.add_argument("--serve-n", type=int, default=60, help="requests per arrival rate")
     ap.add_argument("--serve-ni", type=int, default=8, help="sweep depth of the serving slots")
+    ap.add_argument("--serve-policy", choices=["fifo", "sjf"], default="sjf",
+                    help="admission order: arrival (fifo) or single solves first (sjf)")
     return ap.parse_args()
 
 
@@ -131,9 +133,10 @@ def bench_serve(a):
     with x_init); continuous batching on the device scheduler."""
     import torch
     from genpipeline_amd.problems import serving_requests
-    from genpipeline_amd.serve import Server, latency_summary
+    from genpipeline_amd.serve import Server, ServeConfig, latency_summary
 
-    srv = Server({(32, 32, 32): 16, (64, 64, 64): 8}, tile=(16, 16), ni=a.serve_ni, chunk_ticks=64)
+    srv = Server({(32, 32, 32): 16, (64, 64, 64): 8}, cfg=ServeConfig(policy=a.serve_policy),
+                 tile=(16, 16), ni=a.serve_ni, chunk_ticks=64)
     srv.run(serving_requests(1000, 6, seed=99))  # warm-up
     per = {}
     total_n, total_t = 0, 0.0
@@ -156,7 +159,7 @@ def bench_serve(a):
                  "solves/s", len(per), 1, total_t / len(per) * 1e3, "f32",
                  {"workload": "serve_c5_32cube_64cube", "requests_per_rate": a.serve_n,
                   "rates_per_s": [50, 200, 800], "slots": {"32^3": 16, "64^3": 8},
-                  "scheduler": f"16x16 tile, ni {a.serve_ni}"},
+                  "scheduler": f"16x16 tile, ni {a.serve_ni}", "admission": a.serve_policy},
                  extra={"per_rate": per, "peak_problem_bytes": srv.peak_used})
 
 

@@ -69,6 +69,10 @@ class ServeConfig:
     simp_iters: int = 30
     solve_tol: float = 1e-6
     max_iter: int = 20000
+    # admission order among queued requests that fit: "fifo" (arrival order)
+    # or "sjf" (single solves before SIMP runs, each kind in arrival order;
+    # a SIMP run costs about twenty solves)
+    policy: str = "fifo"
     bcs: dict = field(default_factory=lambda: {"fixed_face": "x_min", "load_face": "x_max",
                                                "load_dof": 2})
 
@@ -120,6 +124,8 @@ class Server:
     def __init__(self, slots: dict, cfg: ServeConfig | None = None, hbm_budget: int | None = None,
                  tile=(16, 16), ni: int = 4, graph_ticks: int = 32, chunk_ticks: int = 64,
                  device=None):
+        if cfg is not None and cfg.policy not in ("fifo", "sjf"):
+            raise ValueError(f"unknown admission policy {cfg.policy!r}")
         if not torch.cuda.is_available():
             raise RuntimeError("Server needs a ROCm GPU (torch.cuda.is_available() is False)")
         d = torch.device("cuda") if device is None else torch.device(device)
@@ -185,8 +191,11 @@ class Server:
             while nxt < len(reqs) and reqs[nxt].t_arrive * time_scale <= now:
                 queue.append(reqs[nxt])
                 nxt += 1
-            # admission: first come first served among requests that fit
-            for r in list(queue):
+            # admission: in policy order among requests that fit
+            order = list(queue)
+            if self.cfg.policy == "sjf":
+                order.sort(key=lambda q: (q.kind != "solve", q.t_arrive))
+            for r in order:
                 shp = tuple(r.shape)
                 if not free[shp]:
                     continue

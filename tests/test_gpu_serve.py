@@ -25,9 +25,10 @@ def _same(a, b):
         assert torch.equal(a.xPhys, b.xPhys)
 
 
-def test_each_result_same_alone_and_in_mix(gpu):
+@pytest.mark.parametrize("policy", ["fifo", "sjf"])
+def test_each_result_same_alone_and_in_mix(gpu, policy):
     from genpipeline_amd.serve import Server, ServeConfig, run_alone
-    cfg = ServeConfig(simp_iters=6)
+    cfg = ServeConfig(simp_iters=6, policy=policy)
     reqs = _requests(14)
     assert {r.kind for r in reqs} == {"solve", "simp"} and {r.shape for r in reqs} == set(SHAPES)
     srv = Server({SHAPES[0]: 3, SHAPES[1]: 2}, cfg=cfg, tile=(16, 16), ni=4, chunk_ticks=32)
