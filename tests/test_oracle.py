@@ -394,6 +394,21 @@ def test_c_oracle_matches_numpy_oracle():
     assert np.abs(u - ud).max() <= 1e-9 * np.abs(ud).max()
 
 
+def test_c_oracle_simp_run_matches_numpy_oracle():
+    """The C restatement's whole SIMP driver (solve, sensitivity, filter, CPU
+    OC bisection, final compliance) against the numpy oracle."""
+    from oracle import cpu_ref
+    nx, ny, nz = 8, 4, 4
+    f, fixed = O.get_bcs_cpu({"load_dof": 2}, nx, ny, nz, 1.0)
+    P = O.OracleProblem(nx, ny, nz, f, fixed, volfrac=0.3, law="emin", oc="cpu")
+    xp_ref, c_ref, _ = P.run(6, "pcg", 1e-12)
+    _, xp, _, c, its = cpu_ref.simp_run((nx, ny, nz), O.ke_hex8(0.3), f, fixed, 6, 0.3, 1e-12,
+                                        law="emin", threads=2)
+    assert its > 0
+    assert np.abs(xp - xp_ref).max() <= 1e-9
+    assert c == pytest.approx(c_ref, rel=1e-10)
+
+
 def test_library_hex8_ke_matches_oracle():
     """The library's host-side KE (used by every device kernel)."""
     from genpipeline_amd import _lib
