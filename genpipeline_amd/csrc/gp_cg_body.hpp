@@ -121,7 +121,7 @@ struct StageIn {
 // S: storage of the iterate x and the search direction p. These two feed only
 // the x update, so a 16-bit S limits the accuracy of x (recovered by FP64
 // refinement) without disturbing the recurrence.
-template <typename T, int TJ, int TK, typename S = T, bool PF = false>
+template <typename T, int TJ, int TK, typename S = T>
 __device__ inline bool body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
                                      const ModalCoef<T>& kc, const float* __restrict__ E,
                                      const T* __restrict__ rp, const T* __restrict__ wp,
@@ -254,12 +254,8 @@ __device__ inline bool body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
   Plane<T> Pold, Pnew, topp, bot, top;
   T zc[3], zn[3];            // z of the owned node on the current / next plane
   uint8_t fmc = 0, fmn = 0;  // its support bits
-  StageIn<T> nxt;            // PF: inputs of the next plane to stage
   {
     T zm1[3];
-    if constexpr (PF) {
-      if (I0 < g.nx) load_stage(I0 + 1, I0 + 1 < iend, nxt);
-    }
     finish_stage(I0 - 1, false, i0, 0, zm1);
     finish_stage(I0, true, i1, 1, zc);
     fmc = i1.fm;
@@ -278,12 +274,7 @@ __device__ inline bool body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
       // zs[zb] was last read two steps earlier (or in the prologue), before a
       // barrier every thread has since passed
       StageIn<T> cur;
-      if constexpr (PF) {
-        cur = nxt;
-        if (ie + 1 < iend && ie + 1 < g.nx) load_stage(ie + 2, ie + 2 < iend, nxt);
-      } else {
-        load_stage(ie + 1, ie + 1 < iend, cur);
-      }
+      load_stage(ie + 1, ie + 1 < iend, cur);
       finish_stage(ie + 1, ie + 1 < iend, cur, zb, zn);
       fmn = cur.fm;
       __syncthreads();
