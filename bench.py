@@ -94,10 +94,10 @@ def bench_batch48(a):
     ws = batch_specs(seed=0, n=a.batch_n, n_iters=a.batch_oc_iters)
     dev = torch.device("cuda", 0)
     Ps = [Problem(w.spec, w.fixed, w.f, device=dev) for w in ws]
-    run_batch(Ps[:4], 2, tile=(16, 16), ni=8, graph_ticks=32)  # warm-up
+    run_batch(Ps[:4], 2, tile=(14, 18), ni=8, graph_ticks=32)  # warm-up
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    res, ticks = run_batch(Ps, a.batch_oc_iters, tile=(16, 16), ni=8, graph_ticks=32, tol=1e-6,
+    res, ticks = run_batch(Ps, a.batch_oc_iters, tile=(14, 18), ni=8, graph_ticks=32, tol=1e-6,
                            max_iter=20000)
     torch.cuda.synchronize()
     t_b = time.perf_counter() - t0
@@ -119,7 +119,7 @@ def bench_batch48(a):
                  {"workload": "simp_batch48_c4_seed0", "problems": n, "oc_iters": a.batch_oc_iters,
                   "grids": "32^3 / 48x24x24 / 64x32x32 / 64^3 by seed",
                   "pcg": "jacobi, fp32 vectors, fp64 dots, rel tol 1e-6, warm start",
-                  "scheduler": "16x16 tile, ni 8, 32-tick HIP graph"},
+                  "scheduler": "14x18 tile in 256-lane blocks, ni 8, 32-tick HIP graph"},
                  extra={"one_by_one_solves_per_s": n / t_s, "speedup_vs_one_by_one": t_s / t_b,
                         "ticks": ticks, "us_per_tick": t_b / ticks * 1e6, "pcg_iters_total": its,
                         "pcg_algorithmic_gbs": pcg_bytes / t_b / 1e9,
@@ -136,7 +136,7 @@ def bench_serve(a):
     from genpipeline_amd.serve import Server, ServeConfig, latency_summary
 
     srv = Server({(32, 32, 32): 16, (64, 64, 64): 8}, cfg=ServeConfig(policy=a.serve_policy),
-                 tile=(16, 16), ni=a.serve_ni, chunk_ticks=64)
+                 tile=(14, 18), ni=a.serve_ni, chunk_ticks=64)
     srv.run(serving_requests(1000, 6, seed=99))  # warm-up
     per = {}
     total_n, total_t = 0, 0.0
@@ -159,7 +159,7 @@ def bench_serve(a):
                  "solves/s", len(per), 1, total_t / len(per) * 1e3, "f32",
                  {"workload": "serve_c5_32cube_64cube", "requests_per_rate": a.serve_n,
                   "rates_per_s": [50, 200, 800], "slots": {"32^3": 16, "64^3": 8},
-                  "scheduler": f"16x16 tile, ni {a.serve_ni}", "admission": a.serve_policy},
+                  "scheduler": f"14x18 tile, ni {a.serve_ni}", "admission": a.serve_policy},
                  extra={"per_rate": per, "peak_problem_bytes": srv.peak_used})
 
 

@@ -1627,8 +1627,9 @@ extern "C" {
 int gp_sched_create(int32_t n_slots, const int32_t* slot_dims, int32_t tj, int32_t tk,
                     gp_sched** out) {
   if (n_slots < 1 || !slot_dims || !out) return fail(GP_ERR_ARG, "bad argument");
-  if (!((tj == 8 && tk == 32) || (tj == 16 && tk == 16)))
-    return fail(GP_ERR_ARG, "scheduler tiles are 8x32 or 16x16 (256 threads), got %dx%d", tj, tk);
+  if (!((tj == 8 && tk == 32) || (tj == 16 && tk == 16) || (tj == 14 && tk == 18) ||
+        (tj == 6 && tk == 34)))
+    return fail(GP_ERR_ARG, "scheduler tiles are 8x32, 16x16, 14x18 or 6x34, got %dx%d", tj, tk);
   *out = nullptr;
   int dev = 0;
   HIPCHK(hipGetDevice(&dev));

@@ -121,7 +121,10 @@ struct StageIn {
 // S: storage of the iterate x and the search direction p. These two feed only
 // the x update, so a 16-bit S limits the accuracy of x (recovered by FP64
 // refinement) without disturbing the recurrence.
-template <typename T, int TJ, int TK, typename S = T>
+// NB: threads per block; NT = TJ x TK of them are active and the rest only
+// join the barriers and add zero to the block sums (a padded block gives the
+// same results as one of exactly NT threads).
+template <typename T, int TJ, int TK, typename S = T, int NB = TJ * TK>
 __device__ inline bool body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
                                      const ModalCoef<T>& kc, const float* __restrict__ E,
                                      const T* __restrict__ rp, const T* __restrict__ wp,
@@ -133,9 +136,12 @@ __device__ inline bool body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
                                      CornerLds<T, TJ * TK>& cv) {
   constexpr int NT = TJ * TK;
   constexpr int NH = TJ + TK + 1;  // halo node columns (aa = 0 or bb = 0)
+  static_assert(NB >= NT && (NB + 63) / 64 == (NT + 63) / 64, "padding within the last wave");
   __shared__ T zs[2][3][TJ + 1][TK + 1];
   const int tid = threadIdx.x;
-  const int a = tid / TK, b = tid - (tid / TK) * TK;
+  const bool act = NB == NT || tid < NT;
+  const int tl = act ? tid : 0;
+  const int a = tl / TK, b = tl - (tl / TK) * TK;
   int bid = lb;
   const int tk_ = bid % ntk;
   bid /= ntk;
@@ -144,7 +150,8 @@ __device__ inline bool body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
   const int J0 = tj_ * (TJ - 1), K0 = tk_ * (TK - 1), I0 = ti_ * NI;
   const int je = J0 - 1 + a, ke = K0 - 1 + b;
   const bool col_ok = (unsigned)je < (unsigned)g.ny && (unsigned)ke < (unsigned)g.nz;
-  const bool owner = a < TJ - 1 && b < TK - 1 && (je + 1) <= g.ny && (ke + 1) <= g.nz;
+  const bool owner =
+      act && a < TJ - 1 && b < TK - 1 && (je + 1) <= g.ny && (ke + 1) <= g.nz;
   // halo assignment
   const bool has_h = tid < NH;
   const int ha = tid <= TK ? 0 : tid - TK;
@@ -188,9 +195,10 @@ __device__ inline bool body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
     NodeCG<T> o;
     finish_node(in.ro, alpha, beta, o);
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      zs[buf][q][a + 1][b + 1] = o.z[q];
-      zo[q] = o.z[q];
+    for (int q = 0; q < 3; ++q) zo[q] = o.z[q];
+    if (act) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) zs[buf][q][a + 1][b + 1] = o.z[q];
     }
     if (has_h) {
       NodeCG<T> h;
@@ -290,7 +298,7 @@ __device__ inline bool body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
         }
     }
     const int par = t & 1;
-    faces_to_corners<T, NT>(topp, bot, cv[par], tid);
+    if (act) faces_to_corners<T, NT>(topp, bot, cv[par], tid);
     __syncthreads();
     if (owner) {
       const long long n = (long long)ie * g.sI + (long long)(je + 1) * g.sJ + (ke + 1);

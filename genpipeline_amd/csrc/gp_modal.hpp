@@ -143,7 +143,8 @@ __device__ inline void reduce3_all(const double* a, const double* b, const doubl
   // four is loaded before any is added, so the loads overlap
   constexpr int U = 4;
   double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-  for (int i0 = threadIdx.x; i0 < n; i0 += U * NT) {
+  // threads past NT (a block padded to more lanes than its tile) add nothing
+  for (int i0 = threadIdx.x; threadIdx.x < NT && i0 < n; i0 += U * NT) {
     double va[U], vb[U], vc[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -360,7 +361,9 @@ __device__ inline void body_ku_modal(const Grid& g, int NI, int ntj, int ntk,
 // node's summation order, are those of body_ku_modal in MODE 0, so the result
 // is the same bit for bit; each node is read once per plane instead of once
 // per element column that touches it.
-template <typename T, int TJ, int TK>
+// NB: threads per block, NT = TJ x TK of them active (the rest only join the
+// barriers).
+template <typename T, int TJ, int TK, int NB = TJ * TK>
 __device__ inline void body_ku_sweep(const Grid& g, int NI, int ntj, int ntk,
                                      const ModalCoef<T>& kc, const float* __restrict__ E,
                                      const T* __restrict__ u, T* __restrict__ y,
@@ -368,9 +371,12 @@ __device__ inline void body_ku_sweep(const Grid& g, int NI, int ntj, int ntk,
                                      CornerLds<T, TJ * TK>& cv) {
   constexpr int NT = TJ * TK;
   constexpr int NH = TJ + TK + 1;
+  static_assert(NB >= NT && (NB + 63) / 64 == (NT + 63) / 64, "padding within the last wave");
   __shared__ T zs[2][3][TJ + 1][TK + 1];
   const int tid = threadIdx.x;
-  const int a = tid / TK, b = tid - (tid / TK) * TK;
+  const bool act = NB == NT || tid < NT;
+  const int tl = act ? tid : 0;
+  const int a = tl / TK, b = tl - (tl / TK) * TK;
   int bid = lb;
   const int tk_ = bid % ntk;
   bid /= ntk;
@@ -379,7 +385,8 @@ __device__ inline void body_ku_sweep(const Grid& g, int NI, int ntj, int ntk,
   const int J0 = tj_ * (TJ - 1), K0 = tk_ * (TK - 1), I0 = ti_ * NI;
   const int je = J0 - 1 + a, ke = K0 - 1 + b;
   const bool col_ok = (unsigned)je < (unsigned)g.ny && (unsigned)ke < (unsigned)g.nz;
-  const bool owner = a < TJ - 1 && b < TK - 1 && (je + 1) <= g.ny && (ke + 1) <= g.nz;
+  const bool owner =
+      act && a < TJ - 1 && b < TK - 1 && (je + 1) <= g.ny && (ke + 1) <= g.nz;
   const bool has_h = tid < NH;
   const int ha = tid <= TK ? 0 : tid - TK;
   const int hb = tid <= TK ? tid : 0;
@@ -403,8 +410,10 @@ __device__ inline void body_ku_sweep(const Grid& g, int NI, int ntj, int ntk,
     T vo[3], vh[3];
     node(ip, je + 1, ke + 1, vo);
     if (has_h) node(ip, J0 - 1 + ha, K0 - 1 + hb, vh);
+    if (act) {
 #pragma unroll
-    for (int q = 0; q < 3; ++q) zs[buf][q][a + 1][b + 1] = vo[q];
+      for (int q = 0; q < 3; ++q) zs[buf][q][a + 1][b + 1] = vo[q];
+    }
     if (has_h) {
 #pragma unroll
       for (int q = 0; q < 3; ++q) zs[buf][q][ha][hb] = vh[q];
@@ -455,7 +464,7 @@ __device__ inline void body_ku_sweep(const Grid& g, int NI, int ntj, int ntk,
         }
     }
     const int par = t & 1;
-    faces_to_corners<T, NT>(topp, bot, cv[par], tid);
+    if (act) faces_to_corners<T, NT>(topp, bot, cv[par], tid);
     __syncthreads();
     if (owner) {
       const long long n = (long long)ie * g.sI + (long long)(je + 1) * g.sJ + (ke + 1);

@@ -157,8 +157,8 @@ __device__ inline SchedState sched_next(const SchedState& cur, const SchedCfg& C
 // fits in, kept for every phase of the kernel (the FP64 sensitivity phase
 // spills a little; a fourth wave per SIMD is worth more to the sweep).
 template <int TJ, int TK>
-__global__ __launch_bounds__(TJ* TK, 4) void k_tick(SchedTable tab, int t) {
-  static_assert(TJ * TK == kBlock, "the scheduler runs every body with kBlock threads");
+__global__ __launch_bounds__(kBlock, 4) void k_tick(SchedTable tab, int t) {
+  static_assert(TJ * TK <= kBlock, "the scheduler runs every body with kBlock threads");
   __shared__ CornerLds<float, TJ * TK> cv;
   const int gb = xcd_block(blockIdx.x, gridDim.x);
   const int b = tab.b2s[gb];
@@ -183,8 +183,8 @@ __global__ __launch_bounds__(TJ* TK, 4) void k_tick(SchedTable tab, int t) {
       body_diag<float>(P.g, P.ke32, P.E, P.fixmask, P.minv, 1, lb, nb);
       break;
     case PH_KX0:
-      body_ku_sweep<float, TJ, TK>(P.g, P.ni, P.ntj, P.ntk, P.mc, P.E, P.u, P.wb[0], P.fixmask,
-                                   lb, cv);
+      body_ku_sweep<float, TJ, TK, kBlock>(P.g, P.ni, P.ntj, P.ntk, P.mc, P.E, P.u, P.wb[0],
+                                           P.fixmask, lb, cv);
       break;
     case PH_CGINIT: {
       const int x0 = sched_use_x0(cur, S.cfg) ? 1 : 0;
@@ -200,9 +200,9 @@ __global__ __launch_bounds__(TJ* TK, 4) void k_tick(SchedTable tab, int t) {
       break;
     case PH_CG: {
       const int k = cur.k, c = k & 1, o = c ^ 1;
-      fin = body_cg_modal<float, TJ, TK>(P.g, P.ni, P.ntj, P.ntk, P.mc, P.E, P.rb[o], P.wb[o],
-                                         P.sb[c], P.minv, P.rb[c], P.wb[c], P.sb[o], P.u, P.pc,
-                                         P.fixmask, P.sc, P.parts, k, lb, nb, cv);
+      fin = body_cg_modal<float, TJ, TK, float, kBlock>(
+          P.g, P.ni, P.ntj, P.ntk, P.mc, P.E, P.rb[o], P.wb[o], P.sb[c], P.minv, P.rb[c], P.wb[c],
+          P.sb[o], P.u, P.pc, P.fixmask, P.sc, P.parts, k, lb, nb, cv);
       break;
     }
     case PH_SENS:
@@ -262,15 +262,19 @@ __global__ __launch_bounds__(TJ* TK, 4) void k_tick(SchedTable tab, int t) {
   }
 }
 
+// Tiles of up to kBlock lanes; a tile of fewer lanes (14 x 18 = 252, 6 x 34 =
+// 204) runs in kBlock-thread blocks whose extra lanes idle in the sweeps.
 bool sched_tick(const SchedTable& tab, int tj, int tk, int blocks, int t, hipStream_t st) {
-  if (tj == 8 && tk == 32) {
-    hipLaunchKernelGGL((k_tick<8, 32>), dim3(blocks), dim3(kBlock), 0, st, tab, t);
-    return true;
+#define GP_TILE(TJ_, TK_)                                                                   \
+  if (tj == TJ_ && tk == TK_) {                                                             \
+    hipLaunchKernelGGL((k_tick<TJ_, TK_>), dim3(blocks), dim3(kBlock), 0, st, tab, t);      \
+    return true;                                                                            \
   }
-  if (tj == 16 && tk == 16) {
-    hipLaunchKernelGGL((k_tick<16, 16>), dim3(blocks), dim3(kBlock), 0, st, tab, t);
-    return true;
-  }
+  GP_TILE(8, 32)
+  GP_TILE(16, 16)
+  GP_TILE(14, 18)
+  GP_TILE(6, 34)
+#undef GP_TILE
   return false;
 }
 

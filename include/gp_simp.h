@@ -260,8 +260,8 @@ typedef struct gp_slot_status {
 } gp_slot_status;
 
 /* slot_dims: 4 ints per slot (nx, ny, nz, ni): the largest grid the slot
- * accepts and the modal sweep depth its jobs use. (tj, tk) = (8, 32) or
- * (16, 16). The device is the current one.                               */
+ * accepts and the modal sweep depth its jobs use. (tj, tk) = (8, 32),
+ * (16, 16), (14, 18) or (6, 34). The device is the current one.          */
 int gp_sched_create(int32_t n_slots, const int32_t* slot_dims, int32_t tj, int32_t tk,
                     gp_sched** out);
 int gp_sched_destroy(gp_sched* s);
