@@ -17,7 +17,7 @@ import numpy as np
 import torch
 
 CACHE = Path(__file__).resolve().parent / "tuned_tiles.json"
-TILES = [(8, 32), (16, 16), (14, 18), (6, 34), (14, 34), (4, 64), (10, 26)]
+TILES = [(8, 32), (16, 16), (14, 18), (6, 34), (14, 34), (4, 64), (10, 26), (7, 34)]
 NIS = [1, 2, 3, 4, 6, 8, 12, 16, 24, 32]
 _lock = threading.Lock()
 _mem = None

@@ -192,7 +192,7 @@ int alloc(void** p, size_t bytes) {
   } while (0)
 
 const KuTile kTiles[] = {{8, 32, 1}, {16, 16, 1}, {14, 18, 1}, {6, 34, 1},
-                         {14, 34, 1}, {4, 64, 1}, {10, 26, 1}};
+                         {14, 34, 1}, {4, 64, 1}, {10, 26, 1}, {7, 34, 1}};
 
 // entries per partials region: enough for every reducing launch shape
 size_t partial_cap(const Grid& g) {
@@ -1628,8 +1628,9 @@ int gp_sched_create(int32_t n_slots, const int32_t* slot_dims, int32_t tj, int32
                     gp_sched** out) {
   if (n_slots < 1 || !slot_dims || !out) return fail(GP_ERR_ARG, "bad argument");
   if (!((tj == 8 && tk == 32) || (tj == 16 && tk == 16) || (tj == 14 && tk == 18) ||
-        (tj == 6 && tk == 34)))
-    return fail(GP_ERR_ARG, "scheduler tiles are 8x32, 16x16, 14x18 or 6x34, got %dx%d", tj, tk);
+        (tj == 6 && tk == 34) || (tj == 7 && tk == 34)))
+    return fail(GP_ERR_ARG, "scheduler tiles are 8x32, 16x16, 14x18, 7x34 or 6x34, got %dx%d", tj,
+                tk);
   *out = nullptr;
   int dev = 0;
   HIPCHK(hipGetDevice(&dev));

@@ -166,7 +166,7 @@ void batch_pcg_setup(const BatchLaunch& bl, bool use_x0, double tol, int max_ite
 #define GP_BT(TJ_, TK_)                                                                     \
   if (bl.tj == TJ_ && bl.tk == TK_)                                                         \
     hipLaunchKernelGGL((k_b_ku<TJ_, TK_>), G(bl.tot_cg), dim3(TJ_ * TK_), 0, st, bl);
-    GP_BT(8, 32) GP_BT(16, 16) GP_BT(14, 18) GP_BT(6, 34) GP_BT(14, 34) GP_BT(4, 64) GP_BT(10, 26)
+    GP_BT(8, 32) GP_BT(16, 16) GP_BT(14, 18) GP_BT(6, 34) GP_BT(14, 34) GP_BT(4, 64) GP_BT(10, 26) GP_BT(7, 34)
 #undef GP_BT
   }
   hipLaunchKernelGGL(k_b_cg_init, G(bl.tot_vec), dim3(kBlock), 0, st, bl, use_x0 ? 1 : 0);
@@ -176,7 +176,7 @@ void batch_cg(const BatchLaunch& bl, int k, hipStream_t st) {
 #define GP_BT(TJ_, TK_)                                                                     \
   if (bl.tj == TJ_ && bl.tk == TK_)                                                         \
     hipLaunchKernelGGL((k_b_cg<TJ_, TK_>), G(bl.tot_cg), dim3(TJ_ * TK_), 0, st, bl, k);
-  GP_BT(8, 32) GP_BT(16, 16) GP_BT(14, 18) GP_BT(6, 34) GP_BT(14, 34) GP_BT(4, 64) GP_BT(10, 26)
+  GP_BT(8, 32) GP_BT(16, 16) GP_BT(14, 18) GP_BT(6, 34) GP_BT(14, 34) GP_BT(4, 64) GP_BT(10, 26) GP_BT(7, 34)
 #undef GP_BT
 }
 void batch_post_solve(const BatchLaunch& bl, int oc_max_steps, hipStream_t st) {

@@ -98,6 +98,7 @@ bool launch_cg_modal(const Grid& g, const KuTile& t, const void* kc, int prec, c
   GP_TILE(14, 34)
   GP_TILE(4, 64)
   GP_TILE(10, 26)
+  GP_TILE(7, 34)
 #undef GP_TILE
   return false;
 }

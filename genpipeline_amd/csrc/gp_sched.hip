@@ -262,8 +262,9 @@ __global__ __launch_bounds__(kBlock, 4) void k_tick(SchedTable tab, int t) {
   }
 }
 
-// Tiles of up to kBlock lanes; a tile of fewer lanes (14 x 18 = 252, 6 x 34 =
-// 204) runs in kBlock-thread blocks whose extra lanes idle in the sweeps.
+// Tiles of up to kBlock lanes; a tile of fewer lanes (14 x 18 = 252, 7 x 34 =
+// 238, 6 x 34 = 204) runs in kBlock-thread blocks whose extra lanes idle in
+// the sweeps.
 bool sched_tick(const SchedTable& tab, int tj, int tk, int blocks, int t, hipStream_t st) {
 #define GP_TILE(TJ_, TK_)                                                                   \
   if (tj == TJ_ && tk == TK_) {                                                             \
@@ -274,6 +275,7 @@ bool sched_tick(const SchedTable& tab, int tj, int tk, int blocks, int t, hipStr
   GP_TILE(16, 16)
   GP_TILE(14, 18)
   GP_TILE(6, 34)
+  GP_TILE(7, 34)
 #undef GP_TILE
   return false;
 }

@@ -205,7 +205,7 @@ class Problem:
         L.call("gp_problem_get_profile", self._h, C.byref(ms), C.byref(n), 1 if reset else 0)
         return (ms.value / n.value if n.value else float("nan")), int(n.value)
 
-    KU_TILES = [(8, 32), (16, 16), (14, 18), (6, 34), (14, 34), (4, 64), (10, 26)]
+    KU_TILES = [(8, 32), (16, 16), (14, 18), (6, 34), (14, 34), (4, 64), (10, 26), (7, 34)]
 
     def set_ku_tile(self, tj: int, tk: int = 0, ni: int = 1, precision: str = "fp32"):
         """Modal K.u workgroup shape (tj x tk element columns, ni planes);

@@ -23,7 +23,7 @@ import torch
 from . import _lib as L
 from .problem import Problem, SimpResult, _stream
 
-TILES = ((8, 32), (16, 16), (14, 18), (6, 34))
+TILES = ((8, 32), (16, 16), (14, 18), (6, 34), (7, 34))
 
 
 @dataclass

@@ -148,7 +148,7 @@ int gp_problem_set_profiling(gp_problem* p, int32_t every);
 int gp_problem_get_profile(gp_problem* p, double* ku_ms_sum, int64_t* ku_samples,
                            int32_t reset);
 /* Workgroup shape of the modal K.u sweep for one precision (GP_F32/GP_F64):
- * tj x tk element columns (one of 8x32 16x16 14x18 6x34 14x34 4x64 10x26),
+ * tj x tk element columns (one of 8x32 16x16 14x18 6x34 14x34 4x64 10x26 7x34),
  * ni node planes per workgroup; tj = 0 selects the dense gather kernel.
  * Used by the autotuner. No counterpart in the reference.                */
 int gp_problem_set_ku_tile(gp_problem* p, int32_t precision, int32_t tj, int32_t tk,
@@ -261,7 +261,7 @@ typedef struct gp_slot_status {
 
 /* slot_dims: 4 ints per slot (nx, ny, nz, ni): the largest grid the slot
  * accepts and the modal sweep depth its jobs use. (tj, tk) = (8, 32),
- * (16, 16), (14, 18) or (6, 34). The device is the current one.          */
+ * (16, 16), (14, 18), (7, 34) or (6, 34). The device is the current one. */
 int gp_sched_create(int32_t n_slots, const int32_t* slot_dims, int32_t tj, int32_t tk,
                     gp_sched** out);
 int gp_sched_destroy(gp_sched* s);

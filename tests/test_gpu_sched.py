@@ -36,7 +36,7 @@ def _same(a, b):
     assert a.iters_done == b.iters_done and a.status == b.status
 
 
-@pytest.mark.parametrize("tile", [(8, 32), (16, 16), (14, 18), (6, 34)])
+@pytest.mark.parametrize("tile", [(8, 32), (16, 16), (14, 18), (6, 34), (7, 34)])
 def test_single_slot_equals_problem_run(gpu, tile):
     from genpipeline_amd.sched import Scheduler
     P = _problem(gpu, 12, 6, 5, 1, pres=True)
