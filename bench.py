@@ -53,22 +53,28 @@ def parse():
     ap.add_argument("--calib-mib", type=int, default=0,
                     help="first run one elementwise add over a float32 buffer of this many MiB "
                          "(a known-size kernel for scaling HBM counters)")
-    ap.add_argument("--workload", choices=["single64", "batch48", "serve", "mixed128", "ku"],
+    ap.add_argument("--workload",
+                    choices=["single64", "batch48", "serve", "mixed128", "ku", "evaluator"],
                     default="single64",
                     help="single64: one 64^3 problem (default); batch48: 48 mixed-size problems "
                          "on the device scheduler against the same problems one by one; serve: "
                          "open-loop request serving at 50/200/800 req/s; mixed128: 16-bit PCG "
                          "storage under FP64 refinement on 128x64x64, errors against FP64; ku: "
                          "K.u and one PCG iteration on a grid larger than the Infinity Cache, "
-                         "beside a device copy")
+                         "beside a device copy; evaluator: a BO batch of latent vectors through "
+                         "VoxelFEMEvaluator.evaluate_batch against evaluate() one by one")
     ap.add_argument("--ku-grid", type=int, nargs=3, default=[256, 256, 256])
     ap.add_argument("--mixed-oc-iters", type=int, default=10)
     ap.add_argument("--batch-n", type=int, default=48)
     ap.add_argument("--batch-oc-iters", type=int, default=60)
     ap.add_argument("--serve-n", type=int, default=60, help="requests per arrival rate")
     ap.add_argument("--serve-ni", type=int, default=8, help="sweep depth of the serving slots")
+    ap.add_argument("--serve-slots", type=int, nargs=2, default=[8, 4],
+                    help="serving slots for 32^3 and for 64^3 requests")
     ap.add_argument("--serve-policy", choices=["fifo", "sjf"], default="sjf",
                     help="admission order: arrival (fifo) or single solves first (sjf)")
+    ap.add_argument("--eval-q", type=int, default=32, help="latent vectors per BO batch")
+    ap.add_argument("--eval-reps", type=int, default=5, help="timed BO batches")
     return ap.parse_args()
 
 
@@ -135,7 +141,8 @@ def bench_serve(a):
     from genpipeline_amd.problems import serving_requests
     from genpipeline_amd.serve import Server, ServeConfig, latency_summary
 
-    srv = Server({(32, 32, 32): 16, (64, 64, 64): 8}, cfg=ServeConfig(policy=a.serve_policy),
+    n32, n64 = a.serve_slots
+    srv = Server({(32, 32, 32): n32, (64, 64, 64): n64}, cfg=ServeConfig(policy=a.serve_policy),
                  tile=(14, 18), ni=a.serve_ni, chunk_ticks=64)
     srv.run(serving_requests(1000, 6, seed=99))  # warm-up
     per = {}
@@ -158,9 +165,97 @@ def bench_serve(a):
     return _line("SIMP/FEM request serving: solves/sec with p50/p99 latency", total_n / total_t,
                  "solves/s", len(per), 1, total_t / len(per) * 1e3, "f32",
                  {"workload": "serve_c5_32cube_64cube", "requests_per_rate": a.serve_n,
-                  "rates_per_s": [50, 200, 800], "slots": {"32^3": 16, "64^3": 8},
+                  "rates_per_s": [50, 200, 800], "slots": {"32^3": n32, "64^3": n64},
                   "scheduler": f"14x18 tile, ni {a.serve_ni}", "admission": a.serve_policy},
                  extra={"per_rate": per, "peak_problem_bytes": srv.peak_used})
+
+
+def bench_evaluator(a):
+    """BO evaluator batch: q latent vectors (N(0,1), 32-dim) decoded to 64^3
+    fields by a seeded stand-in decoder (no VAE checkpoint exists here),
+    down-sampled and smoothed to 32^3 on the device, and solved with
+    GPUVoxelFEM semantics (x_min clamp, 1000 N over the x_max face in -z) as
+    one scheduler batch; timed beside evaluate() per vector (device
+    pre-processing, FP64-refined GPUVoxelFEM solve), and the device
+    pre-processing beside the reference's SciPy chain on the host."""
+    import torch
+    from genpipeline_amd.fem import VoxelFEMEvaluator
+    from genpipeline_amd.fem.voxel_fem import decode_batch
+    from genpipeline_amd.fem.voxel_prep import prep_voxels
+    from genpipeline_amd.pipeline_utils import smooth_voxels
+    from genpipeline_amd.problems import SyntheticDecoder, bo_latents
+    from scipy.ndimage import zoom
+
+    dev = torch.device("cuda", 0)
+    dec = SyntheticDecoder(seed=0, device=dev)
+    q = a.eval_q
+    ev = VoxelFEMEvaluator(vae_model=dec, use_gpu=True)
+    ev.batch_slots = min(q, 32)
+    ev.evaluate_latents(bo_latents(q, seed=99))  # warm-up: scheduler, graphs, problems
+
+    def timed(refine):
+        ev.batch_refine = refine
+        times, outs = [], []
+        for rep in range(a.eval_reps):
+            zs = bo_latents(q, seed=rep)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            outs.append(ev.evaluate_latents(zs))
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+            print(f"[bench] evaluator batch {rep} (refine {refine}): {times[-1] * 1e3:.1f} ms",
+                  file=sys.stderr, flush=True)
+        return times, outs
+
+    times_u, outs_u = timed(False)
+    times, outs = timed(True)
+    t_b = sum(times)
+    # one by one: the reference's per-vector evaluate() on the same first batch
+    single = VoxelFEMEvaluator(vae_model=dec, use_gpu=True)
+    zs = bo_latents(q, seed=0)
+    single.evaluate(zs[0], dec)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ref = [single.evaluate(z, dec) for z in zs]
+    torch.cuda.synchronize()
+    t_s = time.perf_counter() - t0
+    rel = max(abs(o["compliance"] - r["compliance"]) / r["compliance"]
+              for o, r in zip(outs[0], ref))
+    rel_u = max(abs(o["compliance"] - r["compliance"]) / r["compliance"]
+                for o, r in zip(outs_u[0], ref))
+    # pre-processing alone: device kernels for the batch vs SciPy per field
+    raw = decode_batch(list(zs), dec)
+    prep_voxels(raw)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(10):
+        prep_voxels(raw)
+    torch.cuda.synchronize()
+    t_prep = (time.perf_counter() - t0) / 10
+    host = raw.cpu().numpy()
+    n_cpu = min(q, 8)
+    t0 = time.perf_counter()
+    for b in range(n_cpu):
+        smooth_voxels(zoom(host[b], 0.5, order=1))
+    t_prep_cpu = (time.perf_counter() - t0) / n_cpu * q
+    ev.close()
+    single.close()
+    n = q * a.eval_reps
+    return _line("FEM evaluations/sec (BO batch through VoxelFEMEvaluator.evaluate_batch)",
+                 n / t_b, "evaluations/s", a.eval_reps, 1, t_b / a.eval_reps * 1e3, "f32",
+                 {"workload": f"bo_evaluator_q{q}_64cube_decode_32cube_fem",
+                  "latent_dim": 32, "decoded": [64, 64, 64], "fem_grid": [32, 32, 32],
+                  "solve": "GPUVoxelFEM semantics and refinement: FP32 Jacobi-PCG (FP64 dots) "
+                           "batches at 1e-3, FP64 residual passes to 1e-8; "
+                           f"{ev.batch_slots} scheduler slots, 14x18 tile, ni 8"},
+                 extra={"one_by_one_evaluations_per_s": q / t_s,
+                        "speedup_vs_one_by_one": (n / t_b) / (q / t_s),
+                        "max_rel_compliance_vs_one_by_one": rel,
+                        "unrefined_evaluations_per_s": n / sum(times_u),
+                        "unrefined_max_rel_compliance_vs_one_by_one": rel_u,
+                        "prep_device_ms_per_batch": t_prep * 1e3,
+                        "prep_scipy_host_ms_per_batch": t_prep_cpu * 1e3,
+                        "batch_ms": [t * 1e3 for t in times]})
 
 
 def bench_mixed128(a):
@@ -344,6 +439,9 @@ def main():
         return
     if a.workload == "ku":
         print(json.dumps(bench_ku(a)), flush=True)
+        return
+    if a.workload == "evaluator":
+        print(json.dumps(bench_evaluator(a)), flush=True)
         return
     import torch
     from genpipeline_amd import Problem

@@ -104,6 +104,7 @@ SIGNATURES = {
     "gp_sched_blocks": [P, I32, C.POINTER(I32)],
     "gp_sched_admit": [P, I32, P, C.POINTER(GpJob), P, P, P, P],
     "gp_sched_release": [P, I32, P],
+    "gp_sched_set_rhs": [P, I32, P, P],
     "gp_sched_set_graph": [P, I32],
     "gp_sched_tick": [P, I64, P],
     "gp_sched_status": [P, P, P],
@@ -111,6 +112,7 @@ SIGNATURES = {
     "gp_generic_sens": [P, P, P, P, I64, F64, P, P],
     "gp_generic_spmv": [P, P, P, P, I64, I64, F64, P, P],
     "gp_csr_pcg": [P, P, P, P, P, P, P, I64, I64, I32, F64, C.POINTER(GpPcgStats), P],
+    "gp_voxel_prep": [P, I32, I32, I32, I32, I32, I32, I32, P, I32, F64, P, P, P],
 }
 EXTRA = {"gp_last_error": ([], C.c_char_p), "gp_version": ([], C.c_int)}
 

@@ -1197,6 +1197,25 @@ int gp_generic_spmv(const float* xPhys, const float* u, const float* ke, const i
   return GP_OK;
 }
 
+// Evaluator pre-processing (voxel_fem.py:452-462, pipeline_utils.py:55-76).
+int gp_voxel_prep(const float* in, int32_t batch, int32_t d, int32_t h, int32_t w, int32_t od,
+                  int32_t oh, int32_t ow, const double* weights, int32_t radius,
+                  double sharpness, float* work, float* out, void* stream) {
+  if (!in || !weights || !work || !out || batch < 0 || d < 1 || h < 1 || w < 1 || od < 1 ||
+      oh < 1 || ow < 1)
+    return fail(GP_ERR_ARG, "bad argument");
+  if (radius < 0 || 2 * radius + 1 > kMaxTaps)
+    return fail(GP_ERR_ARG, "filter radius %d outside [0, %d]", radius, (kMaxTaps - 1) / 2);
+  for (int q = 1; q <= radius; ++q)
+    if (weights[radius - q] != weights[radius + q])
+      return fail(GP_ERR_ARG, "filter weights are not symmetric");
+  if (batch == 0) return GP_OK;
+  launch_voxel_prep(in, batch, d, h, w, od, oh, ow, weights, radius, sharpness, work, out,
+                    S(stream));
+  LAUNCHCHK();
+  return GP_OK;
+}
+
 // simp_pcg_cuda.cu:66-124 restated: x starts at x0 but r starts at b (the
 // reference never forms b - A x0); z = r / diagonal; ||b|| is the unmasked
 // norm; |p^T A p| < 1e-30 stops the loop. Host-synchronous per iteration,
@@ -1762,6 +1781,18 @@ int gp_sched_admit(gp_sched* Q, int32_t slot, gp_problem* P, const gp_job* job, 
   HIPCHK(hipMemcpyAsync(Q->d_slots + slot, &q, sizeof(SchedSlot), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(Q->d_status + slot, &q.st[0], sizeof(SchedState), hipMemcpyHostToDevice, s));
   Q->busy[slot] = 1;
+  return GP_OK;
+}
+
+int gp_sched_set_rhs(gp_sched* Q, int32_t slot, const double* f, void* stream) {
+  if (!Q || !f || slot < 0 || slot >= Q->n) return fail(GP_ERR_ARG, "bad argument");
+  if (!Q->busy[slot]) return fail(GP_ERR_STATE, "slot %d holds no job", slot);
+  SchedSlot& q = Q->h_slots[slot];
+  if (q.cfg.n_iters != 0) return fail(GP_ERR_ARG, "a right-hand side applies to single solves only");
+  HIPCHK(hipSetDevice(Q->device));
+  q.P.f = f;
+  HIPCHK(hipMemcpyAsync(&Q->d_slots[slot].P.f, &q.P.f, sizeof(q.P.f), hipMemcpyHostToDevice,
+                        S(stream)));
   return GP_OK;
 }
 

@@ -12,6 +12,7 @@ constexpr int kBlock = 256;          // threads per block of the 1-D kernels
 constexpr int kNumXcd = 8;           // MI355X: workgroups go round-robin over 8 XCDs
 constexpr int kVecBlocks = 512;      // cap on blocks of reducing vector kernels
 constexpr int kMaxOffsets = 343;     // filter stencil cap: (2*3+1)^3, rmin <= 3
+constexpr int kMaxTaps = 33;         // 1-D voxel filter cap: radius <= 16
 
 // Structured hex8 grid. Node strides: sJ = nz+1, sI = (ny+1)(nz+1).
 struct Grid {
@@ -313,6 +314,12 @@ void launch_vec(float* y, const float* x, const float* d, float a, long long n, 
                 hipStream_t st);
 
 // generic-table kernels (arbitrary edof)
+// Evaluator pre-processing of decoded fields: order-1 zoom to od x oh x ow,
+// a separable symmetric (2 radius + 1)-tap filter with reflect boundaries
+// along each axis, logistic sharpening. work and out: batch*od*oh*ow floats.
+void launch_voxel_prep(const float* in, int batch, int d, int h, int w, int od, int oh, int ow,
+                       const double* weights, int radius, double sharpness, float* work,
+                       float* out, hipStream_t st);
 void launch_generic_sens(const float* xPhys, const float* u, const float* ke, const int32_t* edof,
                          long long ne, double penal, float* dc, hipStream_t st);
 void launch_generic_spmv(const float* xPhys, const float* u, const float* ke, const int32_t* edof,

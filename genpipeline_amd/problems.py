@@ -15,6 +15,9 @@ order, so a (generator, seed) pair names one exact problem.
 * ``batch_specs``          the uneven 48-problem batch: size class from
   {32^3, 48x24x24, 64x32x32, 64^3}, volfrac U[0.2,0.4], 1-4 point loads.
 * ``serving_requests``     Poisson arrivals of FEM/SIMP evaluation requests.
+* ``SyntheticDecoder``     stand-in for the design VAE's decoder: seeded,
+  smooth random solids of the decoder's output shape.
+* ``bo_latents``           a batch of seeded latent vectors (BO candidates).
 """
 from __future__ import annotations
 
@@ -124,9 +127,11 @@ class Request:
     rid: int
     t_arrive: float
     shape: tuple
-    density: np.ndarray
+    density: np.ndarray  # or a device tensor of the grid's elements
     kind: str           # "solve" | "simp"
     volfrac: float
+    rhs: object = None   # "solve" only: device load vector replacing the problem's
+    tol: float = None    # "solve" only: PCG tolerance replacing the server's
 
 
 def serving_requests(lam: float, n: int, seed=0, p_small=0.6, p_solve=0.5,
@@ -150,3 +155,37 @@ def serving_requests(lam: float, n: int, seed=0, p_small=0.6, p_solve=0.5,
         kind = "solve" if rng.uniform() < p_solve else "simp"
         out.append(Request(r, t, shape, dens, kind, phi))
     return out
+
+
+class SyntheticDecoder:
+    """Stands in for DesignVAE.decode (genpipeline/vae_design_model.py:190-212:
+    sigmoid of the decoder logits, (B, 1, res, res, res) float32) where no
+    trained checkpoint exists. decode(z) = sigmoid(gain (h + bias)) with h a
+    fixed seeded linear map of z onto a coarse^3 grid, trilinearly up-sampled
+    to res^3: smooth random solids, about half solid at bias 0."""
+
+    def __init__(self, latent_dim: int = 32, res: int = 64, coarse: int = 8, seed: int = 0,
+                 gain: float = 6.0, bias: float = 0.0, device=None):
+        import torch
+        g = torch.Generator().manual_seed(int(seed))
+        w = torch.randn(latent_dim, coarse ** 3, generator=g) / float(latent_dim) ** 0.5
+        self._w = torch.nn.Parameter(w.to(device) if device is not None else w,
+                                     requires_grad=False)
+        self.latent_dim, self.res, self.coarse = latent_dim, res, coarse
+        self.gain, self.bias = float(gain), float(bias)
+
+    def parameters(self):
+        return iter([self._w])
+
+    def decode(self, z):
+        import torch
+        c = self.coarse
+        h = (z.float().to(self._w.device) @ self._w).reshape(-1, 1, c, c, c)
+        h = torch.nn.functional.interpolate(h, size=(self.res,) * 3, mode="trilinear",
+                                            align_corners=True)
+        return torch.sigmoid(self.gain * (h + self.bias))
+
+
+def bo_latents(n: int, latent_dim: int = 32, seed: int = 0) -> np.ndarray:
+    """n latent vectors drawn N(0, 1): rng.standard_normal((n, latent_dim))."""
+    return np.random.default_rng(seed).standard_normal((n, latent_dim))

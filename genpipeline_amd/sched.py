@@ -45,6 +45,7 @@ class Job:
     u: torch.Tensor
     n_iters: int
     volfrac: float
+    rhs: Optional[torch.Tensor] = None
 
 
 def _status(s: L.GpSlotStatus) -> SlotStatus:
@@ -112,11 +113,13 @@ class Scheduler:
     def admit(self, slot: int, problem: Problem, n_iters: int = 0, volfrac: float | None = None,
               tol: float = 1e-6, max_iter: int = 20000, tol_schedule: bool = False,
               early_stop: bool = False, compliance_each_iter: bool = False,
-              warm_start: bool = True, x0=None, xphys0=None, u0=None) -> Job:
+              warm_start: bool = True, x0=None, xphys0=None, u0=None, rhs=None) -> Job:
         """Start a job in a free slot (takes effect at the next tick).
 
         n_iters = 0 runs one solve at the given density (``xphys0``, else
-        ``x0``, else volfrac) and reports its compliance."""
+        ``x0``, else volfrac) and reports its compliance. ``rhs`` (n_dof,
+        single solves only) replaces the problem's load: the job solves
+        K d = rhs from d = 0 (``warm_start`` must be False)."""
         sp = problem.spec
         vf = sp.volfrac if volfrac is None else float(volfrac)
         dev = self.device
@@ -145,7 +148,18 @@ class Scheduler:
             L.call("gp_sched_admit", self._h, int(slot), problem.handle, C.byref(j),
                    C.c_void_p(x.data_ptr()), C.c_void_p(xp.data_ptr()), C.c_void_p(u.data_ptr()),
                    _stream(dev))
-        job = Job(problem, x, xp, u, int(n_iters), vf)
+        f = None
+        if rhs is not None:
+            if n_iters != 0 or warm_start:
+                raise ValueError("rhs applies to single solves from zero (n_iters=0, "
+                                 "warm_start=False)")
+            f = torch.as_tensor(rhs, dtype=torch.float64, device=dev).reshape(-1).contiguous()
+            if f.numel() != sp.n_dof:
+                raise ValueError("rhs length does not match the problem")
+            with torch.cuda.device(dev):
+                L.call("gp_sched_set_rhs", self._h, int(slot), C.c_void_p(f.data_ptr()),
+                       _stream(dev))
+        job = Job(problem, x, xp, u, int(n_iters), vf, f)
         self.jobs[slot] = job
         return job
 

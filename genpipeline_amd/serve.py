@@ -56,6 +56,7 @@ class ServeResult:
     t_admit: float
     t_done: float
     xPhys: Optional[torch.Tensor] = field(default=None, repr=False)
+    u: Optional[torch.Tensor] = field(default=None, repr=False)
 
     @property
     def latency(self) -> float:
@@ -84,12 +85,23 @@ def _spec(shape, kind) -> ProblemSpec:
     return ProblemSpec(nx, ny, nz, penal=3.0, law="clamp", oc="gpu")
 
 
+def _flat64(density):
+    """A request's density as a flat float64 array (a device tensor stays on
+    its device)."""
+    if isinstance(density, torch.Tensor):
+        return density.reshape(-1).to(torch.float64)
+    return np.asarray(density, np.float64).ravel()
+
+
 def job_args(req: ServeRequest, cfg: ServeConfig) -> dict:
     """Scheduler job arguments of a request (also used to run it alone)."""
-    dens = np.asarray(req.density, np.float64).ravel()
+    dens = _flat64(req.density)
     if req.kind == "solve":
-        return dict(n_iters=0, x0=dens, xphys0=dens, tol=cfg.solve_tol, max_iter=cfg.max_iter,
-                    warm_start=False)
+        tol = cfg.solve_tol if getattr(req, "tol", None) is None else float(req.tol)
+        return dict(n_iters=0, x0=dens, xphys0=dens, tol=tol, max_iter=cfg.max_iter,
+                    warm_start=False, rhs=getattr(req, "rhs", None))
+    if isinstance(dens, torch.Tensor):
+        dens = dens.cpu().numpy()
     x = np.clip(dens * (req.volfrac / max(dens.mean(), 1e-6)), 1e-3, 1.0)
     return dict(n_iters=cfg.simp_iters, volfrac=req.volfrac, x0=x, xphys0=x, tol=1e-6,
                 max_iter=cfg.max_iter, tol_schedule=True, early_stop=True,
@@ -109,7 +121,10 @@ def fem_outputs(compliance: float, u: torch.Tensor, density: np.ndarray):
     """(compliance, stress_max, displacement_max) with GPUVoxelFEM's proxies and
     sentinels (gpu_fem_solver.py:96-133)."""
     disp = float(u.view(-1, 3).norm(dim=1).max().item())
-    n_solid = int((np.asarray(density) > 0.5).sum())
+    if isinstance(density, torch.Tensor):
+        n_solid = int((density > 0.5).sum().item())
+    else:
+        n_solid = int((np.asarray(density) > 0.5).sum())
     if n_solid < 50 or compliance < 1.0:
         return 1e6, 1e4, 1e3
     return (float(np.clip(compliance, 0, 1e8)),
@@ -171,9 +186,10 @@ class Server:
         return make_problem(shape, kind, self.cfg, self.device)
 
     def run(self, requests: Sequence[ServeRequest], time_scale: float = 1.0,
-            keep_xphys: bool = False) -> list[ServeResult]:
+            keep_xphys: bool = False, keep_u: bool = False) -> list[ServeResult]:
         """Serve requests arriving at t_arrive * time_scale seconds after the
-        call; returns one result per request (request order)."""
+        call; returns one result per request (request order). keep_xphys /
+        keep_u keep each job's density / displacement (device tensors)."""
         reqs = sorted(requests, key=lambda r: r.t_arrive)
         results: dict[int, ServeResult] = {}
         queue: deque = deque()
@@ -229,7 +245,8 @@ class Server:
                 results[r.rid] = ServeResult(r.rid, r.kind, tuple(r.shape), c, sm, dm,
                                              s.iters_done, s.pcg_iters, s.status, r.t_arrive * time_scale,
                                              t_admit, t_now,
-                                             job.xPhys if keep_xphys else None)
+                                             job.xPhys if keep_xphys else None,
+                                             job.u if keep_u else None)
                 self.S.release(slot)
                 free[tuple(r.shape)].append(slot)
                 self.pool[(tuple(r.shape), r.kind)].append(P)

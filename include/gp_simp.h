@@ -273,6 +273,15 @@ int gp_sched_blocks(gp_sched* s, int32_t slot, int32_t* blocks);
  * released. Stream-ordered: takes effect at the next tick on `stream`.   */
 int gp_sched_admit(gp_sched* s, int32_t slot, gp_problem* p, const gp_job* job, double* x,
                    double* xPhys, double* u, void* stream);
+/* Replaces the load of a single-solve job (n_iters = 0) just admitted to
+ * `slot` with the device vector f (n_dof float64): the job then solves
+ * K d = f from d = 0, and its reported compliance is f.d. No counterpart in
+ * the reference, which solves each evaluator field alone
+ * (GPUVoxelFEM.solve, genpipeline/fem/gpu_fem_solver.py:76-133): it carries
+ * the residual right-hand sides of the batched evaluator's FP64 correction
+ * passes. Call it before the next tick; f must stay valid until the slot is
+ * released. Stream-ordered.                                               */
+int gp_sched_set_rhs(gp_sched* s, int32_t slot, const double* f, void* stream);
 /* Frees a slot (its job need not be done). Stream-ordered.               */
 int gp_sched_release(gp_sched* s, int32_t slot, void* stream);
 /* Captures `ticks` (even) tick launches in a HIP graph that later ticks
@@ -308,6 +317,22 @@ int gp_csr_pcg(const int64_t* row_ptr, const int64_t* col_idx, const float* valu
                const float* diagonal, const float* b, float* x, const int64_t* fixed,
                int64_t n_fixed, int64_t n, int32_t max_iter, double tol,
                gp_pcg_stats* stats, void* stream);
+
+/* ---- evaluator pre-processing ------------------------------------------ *
+ * Decoded fields -> FEM voxel densities, on the device. Replaces the CPU
+ * chain of VoxelFEMEvaluator.evaluate (genpipeline/fem/voxel_fem.py:452-462):
+ * scipy.ndimage.zoom(v, 0.5, order=1), then smooth_voxels
+ * (genpipeline/pipeline_utils.py:55-76): gaussian_filter(v, sigma) and
+ * 1 / (1 + exp(-sharpness (v - 0.5))). in: batch fields of d x h x w float32
+ * (C order); out and work: batch * od * oh * ow float32 (od = round(d * zoom)
+ * as scipy rounds it). weights: host array of 2 radius + 1 symmetric taps
+ * (the Gaussian of gaussian_filter), radius <= 16. Order-1 zoom and the
+ * filter follow scipy's operation order and rounding (float32 after the zoom
+ * and after each axis); the logistic is float arithmetic as numpy does it on
+ * float32.                                                                  */
+int gp_voxel_prep(const float* in, int32_t batch, int32_t d, int32_t h, int32_t w, int32_t od,
+                  int32_t oh, int32_t ow, const double* weights, int32_t radius,
+                  double sharpness, float* work, float* out, void* stream);
 
 #ifdef __cplusplus
 }

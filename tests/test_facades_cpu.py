@@ -117,7 +117,7 @@ class _TinyDecoder:
         return iter([self._p])
 
     def decode(self, z):
-        return self.field[None, None]
+        return self.field[None, None].expand(z.shape[0], 1, *self.field.shape)
 
 
 def test_evaluator_penalty_path_without_gpu_solver(tmp_path):

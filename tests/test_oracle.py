@@ -414,3 +414,64 @@ def test_library_hex8_ke_matches_oracle():
     from genpipeline_amd import _lib
     for nu in (0.0, 0.3, 0.45):
         assert np.abs(_lib.hex8_ke(nu) - O.ke_hex8(nu)).max() < 1e-15
+
+
+# ------------------------------------------------- evaluator pre-processing
+def test_evaluator_zoom_restated_is_scipy_bit_for_bit():
+    from scipy.ndimage import zoom
+    from oracle import evaluator_oracle as V
+    rng = np.random.default_rng(3)
+    shapes = [(64, 64, 64), (64, 32, 16), (33, 20, 9), (5, 4, 3), (2, 9, 17), (31, 48, 63)]
+    for shp in shapes:
+        a = rng.random(shp, dtype=np.float32)
+        ref = zoom(a, 0.5, order=1)
+        assert np.array_equal(V.zoom1_restated(a, ref.shape), ref), shp
+    # 32 -> 16: 15 * (31/15) rounds to 31.000000000000004 > 31, and the
+    # 'constant' mode then writes cval on that whole output plane
+    a = rng.random((64, 32, 16), dtype=np.float32) + 1.0
+    ref = zoom(a, 0.5, order=1)
+    assert (ref[:, -1, :] == 0).all() and (ref[:, :-1, :] > 0).all()
+
+
+def test_evaluator_zoom_reproduces_affine_field():
+    from oracle import evaluator_oracle as V
+    n, m = (17, 9, 33), (9, 5, 17)   # exact coordinate maps (n-1)/(m-1) = 2
+    i, j, k = np.meshgrid(*[np.arange(q, dtype=np.float64) for q in n], indexing="ij")
+    a = 0.25 * i - 1.5 * j + 0.125 * k + 3.0
+    out = V.zoom1_restated(a, m)
+    oi, oj, ok = np.meshgrid(*[np.arange(q) * 2.0 for q in m], indexing="ij")
+    assert np.allclose(out, 0.25 * oi - 1.5 * oj + 0.125 * ok + 3.0, rtol=0, atol=1e-12)
+
+
+def test_evaluator_gaussian_restated_is_scipy_and_closed_forms():
+    from scipy.ndimage import gaussian_filter, gaussian_filter1d, zoom
+    from oracle import evaluator_oracle as V
+    from genpipeline_amd.fem.voxel_prep import gaussian_taps, zoom_shape
+    rng = np.random.default_rng(4)
+    for shp in [(32, 32, 32), (16, 10, 4), (2, 2, 2), (1, 3, 7)]:
+        a = rng.random(shp, dtype=np.float32)
+        assert np.array_equal(V.gaussian_restated(a), gaussian_filter(a, 0.5)), shp
+    c = np.full((6, 5, 4), 0.375, np.float32)
+    assert np.array_equal(V.gaussian_restated(c), c)
+    imp = np.zeros((9, 9, 9))
+    imp[4, 4, 4] = 1.0
+    w = gaussian_taps(0.5)
+    assert np.allclose(V.gaussian_restated(imp)[2:7, 2:7, 2:7],
+                       w[:, None, None] * w[None, :, None] * w[None, None, :], rtol=0, atol=1e-16)
+    # the host-side taps are the weights gaussian_filter1d correlates with
+    line = np.zeros(11)
+    line[5] = 1.0
+    assert np.array_equal(gaussian_filter1d(line, 0.5)[3:8], w)
+    assert len(gaussian_taps(1.3)) == 2 * int(4 * 1.3 + 0.5) + 1
+    for shp in [(64, 64, 64), (33, 20, 9), (5, 4, 3), (7, 1, 2)]:
+        assert zoom_shape(shp) == zoom(np.zeros(shp, np.float32), 0.5, order=1).shape
+
+
+def test_evaluator_chain_logistic_fixed_point():
+    from oracle import evaluator_oracle as V
+    v = V.evaluator_voxels(np.full((8, 8, 8), 0.5, np.float32))
+    assert v.dtype == np.float32 and np.array_equal(v, np.full((4, 4, 4), 0.5, np.float32))
+    a = np.random.default_rng(6).random((16, 12, 10), dtype=np.float32)
+    v1, v2 = V.evaluator_voxels(a), V.evaluator_voxels(a, exp="double")
+    assert v2.dtype == np.float32
+    assert np.abs(v1.astype(np.float64) - v2).max() <= 2 ** -23
