@@ -50,7 +50,7 @@ __global__ __launch_bounds__(TJ* TK) void k_b_ku(BatchLaunch bl) {
 }
 
 template <int TJ, int TK>
-__global__ __launch_bounds__(TJ* TK) void k_b_cg(BatchLaunch bl, int k) {
+__global__ __launch_bounds__(whole_waves(TJ* TK)) void k_b_cg(BatchLaunch bl, int k) {
   GP_BATCH_BLOCK(off_cg)
   const int c = k & 1, o = c ^ 1;
   __shared__ CornerLds<float, TJ * TK> cv;
@@ -175,7 +175,8 @@ void batch_pcg_setup(const BatchLaunch& bl, bool use_x0, double tol, int max_ite
 void batch_cg(const BatchLaunch& bl, int k, hipStream_t st) {
 #define GP_BT(TJ_, TK_)                                                                     \
   if (bl.tj == TJ_ && bl.tk == TK_)                                                         \
-    hipLaunchKernelGGL((k_b_cg<TJ_, TK_>), G(bl.tot_cg), dim3(TJ_ * TK_), 0, st, bl, k);
+    hipLaunchKernelGGL((k_b_cg<TJ_, TK_>), G(bl.tot_cg), dim3(whole_waves(TJ_ * TK_)), 0, st, \
+                       bl, k);
   GP_BT(8, 32) GP_BT(16, 16) GP_BT(14, 18) GP_BT(6, 34) GP_BT(14, 34) GP_BT(4, 64) GP_BT(10, 26) GP_BT(7, 34)
 #undef GP_BT
 }

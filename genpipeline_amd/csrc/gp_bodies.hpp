@@ -29,7 +29,7 @@ __device__ inline double wave_max(double v) {
 // Block sum (blockDim == kBlock); result in thread 0.
 __device__ inline double block_sum0(double v) {
   __shared__ double sh[kBlock / 64];
-  v = wave_sum64(v);
+  v = wave_sum_nb<kBlock>(v);
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
   __syncthreads();
   double s = 0.0;
@@ -44,7 +44,7 @@ __device__ inline double reduce_all(const double* __restrict__ p, int n) {
   __shared__ double sh[kBlock / 64];
   double s = 0.0;
   for (int i = threadIdx.x; i < n; i += kBlock) s += p[i];
-  s = wave_sum64(s);
+  s = wave_sum_nb<kBlock>(s);
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
   __syncthreads();
   double t = 0.0;

@@ -76,14 +76,16 @@ __device__ inline void corners_transform(const T (&zs)[3][TJ + 1][TK + 1], int a
   }
 }
 
-template <int NT>
+// Block sums of three values (NB threads, NT of them carrying values; the
+// others pass 0) into *p0, *p1, *p2: wave sums, then waves in order.
+template <int NT, int NB = NT>
 __device__ inline void block_sum3_to(double v0, double v1, double v2, double* p0, double* p1,
                                      double* p2) {
   constexpr int NW = (NT + 63) / 64;
   __shared__ double red[3][NW];
-  v0 = wave_sum64(v0);
-  v1 = wave_sum64(v1);
-  v2 = wave_sum64(v2);
+  v0 = wave_sum_nb<NB>(v0);
+  v1 = wave_sum_nb<NB>(v1);
+  v2 = wave_sum_nb<NB>(v2);
   if ((threadIdx.x & 63) == 0) {
     red[0][threadIdx.x >> 6] = v0;
     red[1][threadIdx.x >> 6] = v1;
@@ -124,7 +126,7 @@ struct StageIn {
 // NB: threads per block; NT = TJ x TK of them are active and the rest only
 // join the barriers and add zero to the block sums (a padded block gives the
 // same results as one of exactly NT threads).
-template <typename T, int TJ, int TK, typename S = T, int NB = TJ * TK>
+template <typename T, int TJ, int TK, typename S = T, int NB = whole_waves(TJ * TK)>
 __device__ inline bool body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
                                      const ModalCoef<T>& kc, const float* __restrict__ E,
                                      const T* __restrict__ rp, const T* __restrict__ wp,
@@ -137,6 +139,7 @@ __device__ inline bool body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
   constexpr int NT = TJ * TK;
   constexpr int NH = TJ + TK + 1;  // halo node columns (aa = 0 or bb = 0)
   static_assert(NB >= NT && (NB + 63) / 64 == (NT + 63) / 64, "padding within the last wave");
+  static_assert(NB % 64 == 0, "whole waves: the block sums use the DPP wave sum");
   __shared__ T zs[2][3][TJ + 1][TK + 1];
   const int tid = threadIdx.x;
   const bool act = NB == NT || tid < NT;
@@ -232,7 +235,7 @@ __device__ inline bool body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
   if (k >= 1) {
     const int q = (k - 1) & 1;
     double s3[3];
-    reduce3_all<NT>(parts.G[q], parts.D[q], parts.R[q], nb, s3);  // gamma, delta, rho
+    reduce3_all<NT, NB>(parts.G[q], parts.D[q], parts.R[q], nb, s3);  // gamma, delta, rho
     const double gam = s3[0], del = s3[1], rho = s3[2];
     const double bet = (k == 1) ? 0.0 : gam / sc->cgg[k & 1];
     const double den = (k == 1) ? del : del - bet * gam / sc->cga[k & 1];
@@ -320,7 +323,8 @@ __device__ inline bool body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
     for (int q = 0; q < 3; ++q) zc[q] = zn[q];
     fmc = fmn;
   }
-  block_sum3_to<NT>(gk, dk, rk, parts.G[k & 1] + lb, parts.D[k & 1] + lb, parts.R[k & 1] + lb);
+  block_sum3_to<NT, NB>(gk, dk, rk, parts.G[k & 1] + lb, parts.D[k & 1] + lb,
+                        parts.R[k & 1] + lb);
   return false;
 }
 

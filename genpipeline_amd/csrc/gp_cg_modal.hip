@@ -29,8 +29,10 @@
 
 namespace gp {
 
+// One workgroup of TJ x TK lanes rounded up to whole waves (the block sums
+// use the DPP wave sum); the extra lanes idle.
 template <typename T, int TJ, int TK, typename S>
-__global__ __launch_bounds__(TJ* TK) void k_cg_modal(Grid g, int NI, int ntj, int ntk,
+__global__ __launch_bounds__(whole_waves(TJ* TK)) void k_cg_modal(Grid g, int NI, int ntj, int ntk,
                                                     ModalCoef<T> kc,
                                                     const float* __restrict__ E,
                                                     const T* __restrict__ rp,
@@ -54,7 +56,8 @@ static void launch_cg_ts(const dim3& grid, int NI, int ntj, int ntk, const Grid&
                          const void* minv, void* rn, void* wn, void* sn, void* x, void* p,
                          const uint8_t* fixmask, DevScalars* sc, Parts parts, int k,
                          hipStream_t st) {
-  hipLaunchKernelGGL((k_cg_modal<T, TJ, TK, S>), grid, dim3(TJ * TK), 0, st, g, NI, ntj, ntk,
+  hipLaunchKernelGGL((k_cg_modal<T, TJ, TK, S>), grid, dim3(whole_waves(TJ * TK)), 0, st, g, NI,
+                     ntj, ntk,
                      *reinterpret_cast<const ModalCoef<T>*>(kc), E, (const T*)rp, (const T*)wp,
                      (const T*)sp, (const T*)minv, (T*)rn, (T*)wn, (T*)sn, (S*)x, (S*)p, fixmask,
                      sc, parts, k);

@@ -218,10 +218,49 @@ __device__ inline int xcd_block(int b, int n) {
   return x * q + (x < r ? x : r) + i;
 }
 
+// Sum over a wave by an xor butterfly of lane shuffles; every lane gets the
+// same bits. Lanes missing from a partial wave read 0.
 __device__ inline double wave_sum64(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
+}
+
+// v from the lane DPP control CTRL names (row_mask RM), 0 where there is none.
+template <int CTRL, int RM>
+__device__ inline double dpp_f64(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)__double2loint(v), CTRL, RM, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)__double2hiint(v), CTRL, RM, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Sum over a wave whose 64 lanes are all active, returned to every lane: a
+// running sum within each 16-lane row (shifts by 1, 2, 4, 8), then row 0 into
+// row 1 and row 2 into row 3 (row_bcast:15), then rows 0-1 into rows 2-3
+// (row_bcast:31); lane 63 holds the total and is broadcast. Fixed order, and
+// no LDS traffic: six DPP moves and adds instead of six shuffle round trips.
+__device__ inline double wave_sum64_full(double x) {
+  x = x + dpp_f64<0x111, 0xf>(x);  // row_shr:1
+  x = x + dpp_f64<0x112, 0xf>(x);  // row_shr:2
+  x = x + dpp_f64<0x114, 0xf>(x);  // row_shr:4
+  x = x + dpp_f64<0x118, 0xf>(x);  // row_shr:8
+  x = x + dpp_f64<0x142, 0xa>(x);  // row_bcast:15 into rows 1 and 3
+  x = x + dpp_f64<0x143, 0xc>(x);  // row_bcast:31 into rows 2 and 3
+  const int lo = __builtin_amdgcn_readlane((int)__double2loint(x), 63);
+  const int hi = __builtin_amdgcn_readlane((int)__double2hiint(x), 63);
+  return __hiloint2double(hi, lo);
+}
+
+// Threads of a block of n lanes rounded up to whole waves.
+constexpr int whole_waves(int n) { return (n + 63) / 64 * 64; }
+
+// Wave sum for a block of NB threads: the DPP form when every wave is full.
+template <int NB>
+__device__ inline double wave_sum_nb(double v) {
+  if constexpr (NB % 64 == 0)
+    return wave_sum64_full(v);
+  else
+    return wave_sum64(v);
 }
 
 // ---------------------------------------------------------------- launchers

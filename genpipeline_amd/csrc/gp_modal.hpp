@@ -133,8 +133,9 @@ __device__ inline void element(const ModalCoef<T>& k, T Ee, const Plane<T>& lo, 
 }
 
 // Fixed-order block sums of three partial arrays, returned to every thread
-// (any block size that is a multiple of the wave size or not).
-template <int NT>
+// (NB threads per block, the first NT of them loading; any NB, with the DPP
+// wave sum when NB is a multiple of the wave size).
+template <int NT, int NB = NT>
 __device__ inline void reduce3_all(const double* a, const double* b, const double* c, int n,
                                    double (&out)[3]) {
   constexpr int NW = (NT + 63) / 64;
@@ -161,9 +162,9 @@ __device__ inline void reduce3_all(const double* a, const double* b, const doubl
         s2 += vc[u];
       }
   }
-  s0 = wave_sum64(s0);
-  s1 = wave_sum64(s1);
-  s2 = wave_sum64(s2);
+  s0 = wave_sum_nb<NB>(s0);
+  s1 = wave_sum_nb<NB>(s1);
+  s2 = wave_sum_nb<NB>(s2);
   if ((threadIdx.x & 63) == 0) {
     sh[0][threadIdx.x >> 6] = s0;
     sh[1][threadIdx.x >> 6] = s1;
