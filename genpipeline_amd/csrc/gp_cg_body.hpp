@@ -263,7 +263,7 @@ __device__ inline bool body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
   }
 
   Plane<T> Pold, Pnew, topp, bot, top;
-  T zc[3], zn[3];            // z of the owned node on the current / next plane
+  T zc[3], zn[3] = {};       // z of the owned node on the current / next plane
   uint8_t fmc = 0, fmn = 0;  // its support bits
   {
     T zm1[3];
@@ -278,51 +278,60 @@ __device__ inline bool body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
     Pold = Pnew;
   }
 
-  for (int ie = I0, t = 0; ie < iend; ++ie, ++t) {
+  // One element plane ie (step t): the owned node's plane ie+1 is staged and
+  // its corners transformed into pnew, the element between pold and pnew
+  // gives bot and top, and bot plus the previous step's top (topp) complete
+  // node plane ie. The caller passes the planes of consecutive steps with
+  // their roles swapped (pnew becomes pold, top becomes topp, zn becomes zc),
+  // so nothing is copied between steps.
+  auto step = [&](int ie, int t, Plane<T>& pold, Plane<T>& pnew, Plane<T>& topp_,
+                  Plane<T>& top_, T(&zc_)[3], T(&zn_)[3], uint8_t& fmc_, uint8_t& fmn_) {
     const T Ee = Eat(ie);
     const int zb = t & 1;  // buffer of plane ie+1 (planes I0-1, I0 used 0, 1)
+    Plane<T> bot_;
     if (ie < g.nx) {
       // zs[zb] was last read two steps earlier (or in the prologue), before a
       // barrier every thread has since passed
       StageIn<T> cur;
       load_stage(ie + 1, ie + 1 < iend, cur);
-      finish_stage(ie + 1, ie + 1 < iend, cur, zb, zn);
-      fmn = cur.fm;
+      finish_stage(ie + 1, ie + 1 < iend, cur, zb, zn_);
+      fmn_ = cur.fm;
       __syncthreads();
-      corners_transform<T, TJ, TK>(zs[zb], a, b, Pnew);
-      element(kc, Ee, Pold, Pnew, bot, top);
+      corners_transform<T, TJ, TK>(zs[zb], a, b, pnew);
+      element(kc, Ee, pold, pnew, bot_, top_);
     } else {
 #pragma unroll
       for (int q = 0; q < 3; ++q)
 #pragma unroll
         for (int m = 0; m < 4; ++m) {
-          bot.P[q][m] = 0;
-          top.P[q][m] = 0;
+          bot_.P[q][m] = 0;
+          top_.P[q][m] = 0;
         }
     }
     const int par = t & 1;
-    if (act) faces_to_corners<T, NT>(topp, bot, cv[par], tid);
+    if (act) faces_to_corners<T, NT>(topp_, bot_, cv[par], tid);
     __syncthreads();
     if (owner) {
       const long long n = (long long)ie * g.sI + (long long)(je + 1) * g.sJ + (ke + 1);
       T w[3];
 #pragma unroll
       for (int q = 0; q < 3; ++q) w[q] = gather_node<T, TK, NT>(cv[par], q, tid);
-      if (fmc & 1) w[0] = 0;
-      if (fmc & 2) w[1] = 0;
-      if (fmc & 4) w[2] = 0;
+      if (fmc_ & 1) w[0] = 0;
+      if (fmc_ & 2) w[1] = 0;
+      if (fmc_ & 4) w[2] = 0;
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
         wn[3 * n + q] = w[q];
-        dk += (double)zc[q] * (double)w[q];
+        dk += (double)zc_[q] * (double)w[q];
       }
     }
-    topp = top;
-    Pold = Pnew;
-#pragma unroll
-    for (int q = 0; q < 3; ++q) zc[q] = zn[q];
-    fmc = fmn;
+  };
+  int ie = I0, t = 0;
+  for (; ie + 1 < iend; ie += 2, t += 2) {
+    step(ie, t, Pold, Pnew, topp, top, zc, zn, fmc, fmn);
+    step(ie + 1, t + 1, Pnew, Pold, top, topp, zn, zc, fmn, fmc);
   }
+  if (ie < iend) step(ie, t, Pold, Pnew, topp, top, zc, zn, fmc, fmn);
   block_sum3_to<NT, NB>(gk, dk, rk, parts.G[k & 1] + lb, parts.D[k & 1] + lb,
                         parts.R[k & 1] + lb);
   return false;

@@ -226,26 +226,29 @@ __device__ inline double wave_sum64(double v) {
   return v;
 }
 
-// v from the lane DPP control CTRL names (row_mask RM), 0 where there is none.
-template <int CTRL, int RM>
+// v from the lane that DPP control CTRL names, 0 where there is none.
+template <int CTRL>
 __device__ inline double dpp_f64(double v) {
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)__double2loint(v), CTRL, RM, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)__double2hiint(v), CTRL, RM, 0xf, false);
+  const int lo = __builtin_amdgcn_mov_dpp((int)__double2loint(v), CTRL, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)__double2hiint(v), CTRL, 0xf, 0xf, true);
   return __hiloint2double(hi, lo);
 }
 
 // Sum over a wave whose 64 lanes are all active, returned to every lane: a
-// running sum within each 16-lane row (shifts by 1, 2, 4, 8), then row 0 into
-// row 1 and row 2 into row 3 (row_bcast:15), then rows 0-1 into rows 2-3
-// (row_bcast:31); lane 63 holds the total and is broadcast. Fixed order, and
-// no LDS traffic: six DPP moves and adds instead of six shuffle round trips.
+// running sum within each 16-lane row (shifts by 1, 2, 4, 8), so lane 16r+15
+// holds row r; then lane 15 of each row into the next row (row_bcast:15),
+// so lane 63 holds rows 3 + 2 and lane 31 rows 1 + 0; then lane 31 into rows
+// 2 and 3 (row_bcast:31): lane 63 holds (row3 + row2) + (row1 + row0) and is
+// broadcast. Other lanes end with partial sums that are not read. Fixed
+// order, and no LDS traffic: six DPP moves and adds instead of six shuffle
+// round trips.
 __device__ inline double wave_sum64_full(double x) {
-  x = x + dpp_f64<0x111, 0xf>(x);  // row_shr:1
-  x = x + dpp_f64<0x112, 0xf>(x);  // row_shr:2
-  x = x + dpp_f64<0x114, 0xf>(x);  // row_shr:4
-  x = x + dpp_f64<0x118, 0xf>(x);  // row_shr:8
-  x = x + dpp_f64<0x142, 0xa>(x);  // row_bcast:15 into rows 1 and 3
-  x = x + dpp_f64<0x143, 0xc>(x);  // row_bcast:31 into rows 2 and 3
+  x = x + dpp_f64<0x111>(x);  // row_shr:1
+  x = x + dpp_f64<0x112>(x);  // row_shr:2
+  x = x + dpp_f64<0x114>(x);  // row_shr:4
+  x = x + dpp_f64<0x118>(x);  // row_shr:8
+  x = x + dpp_f64<0x142>(x);  // row_bcast:15
+  x = x + dpp_f64<0x143>(x);  // row_bcast:31
   const int lo = __builtin_amdgcn_readlane((int)__double2loint(x), 63);
   const int hi = __builtin_amdgcn_readlane((int)__double2hiint(x), 63);
   return __hiloint2double(hi, lo);
