@@ -126,7 +126,8 @@ struct StageIn {
 // NB: threads per block; NT = TJ x TK of them are active and the rest only
 // join the barriers and add zero to the block sums (a padded block gives the
 // same results as one of exactly NT threads).
-template <typename T, int TJ, int TK, typename S = T, int NB = whole_waves(TJ * TK)>
+template <typename T, int TJ, int TK, typename S = T, int NB = whole_waves(TJ * TK),
+          bool UNROLL2 = true>
 __device__ inline bool body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
                                      const ModalCoef<T>& kc, const float* __restrict__ E,
                                      const T* __restrict__ rp, const T* __restrict__ wp,
@@ -263,7 +264,7 @@ __device__ inline bool body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
   }
 
   Plane<T> Pold, Pnew, topp, bot, top;
-  T zc[3], zn[3] = {};       // z of the owned node on the current / next plane
+  T zc[3], zn[3];            // z of the owned node on the current / next plane
   uint8_t fmc = 0, fmn = 0;  // its support bits
   {
     T zm1[3];
@@ -326,12 +327,62 @@ __device__ inline bool body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
       }
     }
   };
-  int ie = I0, t = 0;
-  for (; ie + 1 < iend; ie += 2, t += 2) {
-    step(ie, t, Pold, Pnew, topp, top, zc, zn, fmc, fmn);
-    step(ie + 1, t + 1, Pnew, Pold, top, topp, zn, zc, fmn, fmc);
+  if constexpr (UNROLL2) {
+    int ie = I0, t = 0;
+    for (; ie + 1 < iend; ie += 2, t += 2) {
+      step(ie, t, Pold, Pnew, topp, top, zc, zn, fmc, fmn);
+      step(ie + 1, t + 1, Pnew, Pold, top, topp, zn, zc, fmn, fmc);
+    }
+    if (ie < iend) step(ie, t, Pold, Pnew, topp, top, zc, zn, fmc, fmn);
+  } else {
+    // the same steps as one loop with the plane roles rotated by copies (the
+    // form that compiles best under a 128-register budget)
+    for (int ie = I0, t = 0; ie < iend; ++ie, ++t) {
+      const T Ee = Eat(ie);
+      const int zb = t & 1;  // buffer of plane ie+1 (planes I0-1, I0 used 0, 1)
+      if (ie < g.nx) {
+        // zs[zb] was last read two steps earlier (or in the prologue), before a
+        // barrier every thread has since passed
+        StageIn<T> cur;
+        load_stage(ie + 1, ie + 1 < iend, cur);
+        finish_stage(ie + 1, ie + 1 < iend, cur, zb, zn);
+        fmn = cur.fm;
+        __syncthreads();
+        corners_transform<T, TJ, TK>(zs[zb], a, b, Pnew);
+        element(kc, Ee, Pold, Pnew, bot, top);
+      } else {
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+#pragma unroll
+          for (int m = 0; m < 4; ++m) {
+            bot.P[q][m] = 0;
+            top.P[q][m] = 0;
+          }
+      }
+      const int par = t & 1;
+      if (act) faces_to_corners<T, NT>(topp, bot, cv[par], tid);
+      __syncthreads();
+      if (owner) {
+        const long long n = (long long)ie * g.sI + (long long)(je + 1) * g.sJ + (ke + 1);
+        T w[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) w[q] = gather_node<T, TK, NT>(cv[par], q, tid);
+        if (fmc & 1) w[0] = 0;
+        if (fmc & 2) w[1] = 0;
+        if (fmc & 4) w[2] = 0;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          wn[3 * n + q] = w[q];
+          dk += (double)zc[q] * (double)w[q];
+        }
+      }
+      topp = top;
+      Pold = Pnew;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) zc[q] = zn[q];
+      fmc = fmn;
+    }
   }
-  if (ie < iend) step(ie, t, Pold, Pnew, topp, top, zc, zn, fmc, fmn);
   block_sum3_to<NT, NB>(gk, dk, rk, parts.G[k & 1] + lb, parts.D[k & 1] + lb,
                         parts.R[k & 1] + lb);
   return false;

@@ -200,7 +200,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_tick(SchedTable tab, int t) {
       break;
     case PH_CG: {
       const int k = cur.k, c = k & 1, o = c ^ 1;
-      fin = body_cg_modal<float, TJ, TK, float, kBlock>(
+      fin = body_cg_modal<float, TJ, TK, float, kBlock, false>(
           P.g, P.ni, P.ntj, P.ntk, P.mc, P.E, P.rb[o], P.wb[o], P.sb[c], P.minv, P.rb[c], P.wb[c],
           P.sb[o], P.u, P.pc, P.fixmask, P.sc, P.parts, k, lb, nb, cv);
       break;
