@@ -1197,6 +1197,61 @@ int gp_generic_spmv(const float* xPhys, const float* u, const float* ke, const i
   return GP_OK;
 }
 
+// Isosurface triangles of a density field (mesh_export.py:7-67).
+static int mc_check(int32_t nx, int32_t ny, int32_t nz) {
+  if (nx < 2 || ny < 2 || nz < 2) return fail(GP_ERR_ARG, "grid %dx%dx%d has no cube", nx, ny, nz);
+  if ((long long)(nx - 1) * (ny - 1) * (nz - 1) * 5 >= (1LL << 31))
+    return fail(GP_ERR_ARG, "grid %dx%dx%d too large for 32-bit triangle offsets", nx, ny, nz);
+  return GP_OK;
+}
+
+int gp_mc_workspace(int32_t nx, int32_t ny, int32_t nz, int64_t* n_ints) {
+  if (!n_ints) return fail(GP_ERR_ARG, "bad argument");
+  int rc = mc_check(nx, ny, nz);
+  if (rc) return rc;
+  *n_ints = (int64_t)mc_blocks(nx, ny, nz) + 1;
+  return GP_OK;
+}
+
+int gp_mc_count(const double* density, int32_t nx, int32_t ny, int32_t nz, double level,
+                int32_t* workspace, int64_t* n_tri, void* stream) {
+  if (!density || !workspace || !n_tri) return fail(GP_ERR_ARG, "bad argument");
+  int rc = mc_check(nx, ny, nz);
+  if (rc) return rc;
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (const char* e = mc_prepare(dev)) return fail(GP_ERR_HIP, "%s", e);
+  hipStream_t s = S(stream);
+  launch_mc_count(density, nx, ny, nz, level, workspace, s);
+  LAUNCHCHK();
+  int32_t total = 0;
+  HIPCHK(hipMemcpyAsync(&total, workspace + mc_blocks(nx, ny, nz), sizeof(total),
+                        hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  *n_tri = total;
+  return GP_OK;
+}
+
+int gp_mc_emit(const double* density, int32_t nx, int32_t ny, int32_t nz, double level,
+               double sx, double sy, double sz, const int32_t* workspace, float* tris,
+               void* stream) {
+  if (!density || !workspace || !tris) return fail(GP_ERR_ARG, "bad argument");
+  int rc = mc_check(nx, ny, nz);
+  if (rc) return rc;
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (const char* e = mc_prepare(dev)) return fail(GP_ERR_HIP, "%s", e);
+  launch_mc_emit(density, nx, ny, nz, level, sx, sy, sz, workspace, tris, S(stream));
+  LAUNCHCHK();
+  return GP_OK;
+}
+
+int gp_mc_tables(int8_t* ntri, int8_t* tri) {
+  if (!ntri || !tri) return fail(GP_ERR_ARG, "bad argument");
+  mc_tables_host(ntri, tri);
+  return GP_OK;
+}
+
 // Evaluator pre-processing (voxel_fem.py:452-462, pipeline_utils.py:55-76).
 int gp_voxel_prep(const float* in, int32_t batch, int32_t d, int32_t h, int32_t w, int32_t od,
                   int32_t oh, int32_t ow, const double* weights, int32_t radius,

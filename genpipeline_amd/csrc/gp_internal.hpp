@@ -356,6 +356,20 @@ void launch_vec(float* y, const float* x, const float* d, float a, long long n, 
                 hipStream_t st);
 
 // generic-table kernels (arbitrary edof)
+// Marching cubes over an nx x ny x nz float64 sample grid (cubes in C order,
+// 256 a block): mc_prepare copies the generated case table to the current
+// device once (error text or null); launch_mc_count leaves per-block
+// triangle offsets in off[0..nb) and the total in off[nb]; launch_mc_emit
+// writes 12 floats (unit normal, 3 vertices scaled by sx, sy, sz) per
+// triangle. mc_tables_host gives the table (ntri[256], tri[256][15]).
+int mc_blocks(int nx, int ny, int nz);
+const char* mc_prepare(int device);
+void mc_tables_host(int8_t* ntri, int8_t* tri);
+void launch_mc_count(const double* d, int nx, int ny, int nz, double level, int* off,
+                     hipStream_t st);
+void launch_mc_emit(const double* d, int nx, int ny, int nz, double level, double sx, double sy,
+                    double sz, const int* off, float* out, hipStream_t st);
+
 // Evaluator pre-processing of decoded fields: order-1 zoom to od x oh x ow,
 // a separable symmetric (2 radius + 1)-tap filter with reflect boundaries
 // along each axis, logistic sharpening. work and out: batch*od*oh*ow floats.

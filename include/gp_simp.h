@@ -318,6 +318,28 @@ int gp_csr_pcg(const int64_t* row_ptr, const int64_t* col_idx, const float* valu
                int64_t n_fixed, int64_t n, int32_t max_iter, double tol,
                gp_pcg_stats* stats, void* stream);
 
+/* ---- isosurface ---------------------------------------------------------- *
+ * Triangles of the surface density = level of an nx x ny x nz float64 field
+ * (C order), the marching-cubes step of density_to_stl
+ * (genpipeline/topology/mesh_export.py:7-67, which calls
+ * skimage.measure.marching_cubes and writes the faces with numpy-stl).
+ * A sample is inside when > level; each 2x2x2 cube's triangles come from a
+ * generated 256-case table (inside on the normal's back side); a vertex is
+ * the linear level crossing of its grid edge. Record per triangle: 12
+ * float32 (unit normal, then 3 vertices in index units times sx, sy, sz),
+ * in C order of the cubes.
+ * gp_mc_workspace: int32 entries of device workspace the other two need.
+ * gp_mc_count: fills the workspace, returns the triangle count (synchronises
+ *   the stream); gp_mc_emit: writes n_tri * 12 floats to tris.
+ * gp_mc_tables: the generated table (ntri[256], tri[256][15], -1 padded). */
+int gp_mc_workspace(int32_t nx, int32_t ny, int32_t nz, int64_t* n_ints);
+int gp_mc_count(const double* density, int32_t nx, int32_t ny, int32_t nz, double level,
+                int32_t* workspace, int64_t* n_tri, void* stream);
+int gp_mc_emit(const double* density, int32_t nx, int32_t ny, int32_t nz, double level,
+               double sx, double sy, double sz, const int32_t* workspace, float* tris,
+               void* stream);
+int gp_mc_tables(int8_t* ntri, int8_t* tri);
+
 /* ---- evaluator pre-processing ------------------------------------------ *
  * Decoded fields -> FEM voxel densities, on the device. Replaces the CPU
  * chain of VoxelFEMEvaluator.evaluate (genpipeline/fem/voxel_fem.py:452-462):

@@ -475,3 +475,93 @@ def test_evaluator_chain_logistic_fixed_point():
     v1, v2 = V.evaluator_voxels(a), V.evaluator_voxels(a, exp="double")
     assert v2.dtype == np.float32
     assert np.abs(v1.astype(np.float64) - v2).max() <= 2 ** -23
+
+
+# ------------------------------------------------------------ isosurface
+def _mesh_topology(T):
+    V, idx = np.unique(T.reshape(-1, 3), axis=0, return_inverse=True)
+    idx = idx.reshape(-1, 3)
+    de = {}
+    for a, b, c in idx:
+        for u, v in ((a, b), (b, c), (c, a)):
+            de[(u, v)] = de.get((u, v), 0) + 1
+    closed = all(n == 1 and de.get((v, u), 0) == 1 for (u, v), n in de.items())
+    n_edges = len({tuple(sorted(k)) for k in de})
+    vol = np.einsum("ij,ij->i", T[:, 0], np.cross(T[:, 1], T[:, 2])).sum() / 6
+    return len(V) - n_edges + len(T), closed, vol
+
+
+def test_mesh_table_single_corner_cases_point_outward():
+    from oracle import mesh_oracle as M
+    ntri, tri = M.tables()
+    assert ntri[0] == 0 and ntri[255] == 0 and ntri.max() == 5
+    for c in range(8):
+        for case, sign in ((1 << c, 1.0), (255 ^ (1 << c), -1.0)):
+            assert ntri[case] == 1
+            e = set(tri[case, :3].tolist())
+            assert e == {i for i, (a, b) in enumerate(M.EDGES) if c in (a, b)}
+            # the triangle's normal points away from an inside corner c and
+            # toward an outside one
+            d = np.where(np.arange(8) == c, 1.0, 0.0) if sign > 0 else np.where(np.arange(8) == c, 0.0, 1.0)
+            field = np.zeros((2, 2, 2))
+            for k in range(8):
+                field[tuple(M.CORNERS[k])] = d[k]
+            T = M.marching_cubes(field, 0.5)
+            n = np.cross(T[0, 1] - T[0, 0], T[0, 2] - T[0, 0])
+            assert sign * (n @ (T[0].mean(axis=0) - M.CORNERS[c])) > 0
+
+
+def test_mesh_closed_shapes_topology_and_volume():
+    from oracle import mesh_oracle as M
+    n = 22
+    x = np.arange(n) - (n - 1) / 2
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    r0 = 7.6
+    chi, closed, vol = _mesh_topology(M.marching_cubes(r0 - np.sqrt(X**2 + Y**2 + Z**2), 0.0))
+    assert closed and chi == 2
+    assert abs(vol - 4 / 3 * np.pi * r0**3) < 0.03 * 4 / 3 * np.pi * r0**3
+    R, rr = 6.5, 2.5
+    chi, closed, vol = _mesh_topology(
+        M.marching_cubes(rr - np.sqrt((np.sqrt(X**2 + Y**2) - R) ** 2 + Z**2), 0.0))
+    assert closed and chi == 0
+    assert abs(vol - 2 * np.pi**2 * R * rr**2) < 0.06 * 2 * np.pi**2 * R * rr**2
+    # random blobs away from the border: closed and consistently oriented,
+    # including the faces with two diagonal inside corners
+    from scipy.ndimage import gaussian_filter
+    d = np.zeros((16, 15, 14))
+    d[2:-2, 2:-2, 2:-2] = gaussian_filter(np.random.default_rng(1).standard_normal((12, 11, 10)), 1.0)
+    chi, closed, vol = _mesh_topology(M.marching_cubes(d, 0.0))
+    assert closed and vol > 0
+
+
+def test_mesh_vertices_are_edge_crossings():
+    from oracle import mesh_oracle as M
+    rng = np.random.default_rng(2)
+    d = rng.random((5, 4, 6))
+    T = M.marching_cubes(d, 0.5, scale=(1.0, 1.0, 1.0))
+    assert len(T)
+    for p in T.reshape(-1, 3):
+        fl = np.floor(p).astype(int)
+        ax = int(np.argmax(p - fl))  # one coordinate is fractional
+        q = fl.copy()
+        q[ax] += 1
+        v0, v1 = d[tuple(fl)], d[tuple(q)]
+        assert (v0 > 0.5) != (v1 > 0.5)
+        assert p[ax] == fl[ax] + (0.5 - v0) / (v1 - v0)  # from the lower end
+        assert np.array_equal(np.delete(p, ax), np.delete(fl, ax).astype(float))
+
+
+def test_density_to_stl_level_rules_and_stl_format(tmp_path):
+    from genpipeline_amd.topology.mesh_export import _level, read_stl, write_stl
+    assert _level(np.full((3, 3, 3), 0.005), 0.5) is None
+    assert _level(np.linspace(0, 1, 27).reshape(3, 3, 3), 0.5) == 0.5
+    d = np.linspace(0.6, 0.9, 27).reshape(3, 3, 3)  # threshold below min
+    assert _level(d, 0.5) == pytest.approx(0.6 + 0.1 * 0.3)
+    d = np.linspace(0.02, 0.3, 27).reshape(3, 3, 3)  # above max
+    assert _level(d, 0.5) == pytest.approx(0.3 - 0.1 * 0.28)
+    d = np.linspace(0.02, 0.05, 27).reshape(3, 3, 3)  # at least 0.1
+    assert _level(d, 0.5) == 0.1
+    rec = np.random.default_rng(3).standard_normal((7, 12)).astype(np.float32)
+    write_stl(str(tmp_path / "m.stl"), rec)
+    assert (tmp_path / "m.stl").stat().st_size == 80 + 4 + 50 * 7
+    assert np.array_equal(read_stl(str(tmp_path / "m.stl")), rec)
