@@ -139,3 +139,40 @@ def test_evaluator_penalty_path_without_gpu_solver(tmp_path):
     assert h[0]["result"]["stress_max"] == FEM_SENTINEL
     with pytest.raises(RuntimeError):
         VoxelFEMEvaluator().evaluate_batch(params)
+
+
+def test_topo_data_generator_interface_and_draws(tmp_path):
+    """Constructor defaults and the random draws of generate_single
+    (topo_data_gen.py:129-264): geometry by weight (bridge, arch, 3dload
+    twice as likely), grid by size class, volfrac and force in the organic
+    ranges, all from NumPy's global generator in that order."""
+    from genpipeline_amd.topology.topo_data_gen import TopoDataGenerator
+    assert _defaults(TopoDataGenerator.__init__) == dict(
+        output_dir="./fem_data", n_workers=2, use_gpu=True, n_iters=30, grid_size="medium",
+        penal=3.5, organic_mode=True)
+    assert _defaults(TopoDataGenerator.generate) == dict(n_samples=100)
+    assert _defaults(TopoDataGenerator.generate_scaled) == dict(n_samples=1000, batch_size=100)
+    g = TopoDataGenerator(output_dir=str(tmp_path / "d"))
+    assert len(g.GEOM_BCS) == 17 and g.GEOM_BCS["spoke"]["fixed_face"] == "center"
+    geoms = list(g.GEOM_BCS)
+    w = np.array([g.GEOM_WEIGHTS.get(x, 1) for x in geoms], float)
+    assert w.sum() == 20 and w[geoms.index("bridge")] == 2 and w[geoms.index("portal")] == 1
+    np.random.seed(7)
+    got = [g.draw_sample() for _ in range(40)]
+    np.random.seed(7)
+    for s in got:
+        geom = np.random.choice(geoms, p=w / w.sum())
+        assert s.geom == geom
+        if geom in ("lbracket", "bridge", "arch", "center"):
+            lo_hi = ((16, 24), (6, 10), (12, 18))
+        elif geom in ("3dload", "offset"):
+            lo_hi = ((22, 32), (8, 12), (8, 12))
+        elif geom in ("branch", "y_tree", "network"):
+            lo_hi = ((22, 32), (10, 16), (10, 16))
+        else:
+            lo_hi = ((20, 32), (6, 10), (6, 10))
+        assert (s.nx, s.ny, s.nz) == tuple(np.random.randint(a, b) for a, b in lo_hi)
+        assert s.volfrac == np.random.uniform(0.12, 0.35)
+        assert s.force_mag == np.random.uniform(200, 5000)
+    g.grid_size = "small"
+    assert g._get_grid_dims("bridge") == (16, 8, 8)
