@@ -54,7 +54,8 @@ def parse():
                     help="first run one elementwise add over a float32 buffer of this many MiB "
                          "(a known-size kernel for scaling HBM counters)")
     ap.add_argument("--workload",
-                    choices=["single64", "batch48", "serve", "mixed128", "ku", "evaluator"],
+                    choices=["single64", "batch48", "serve", "mixed128", "ku", "evaluator",
+                             "datagen"],
                     default="single64",
                     help="single64: one 64^3 problem (default); batch48: 48 mixed-size problems "
                          "on the device scheduler against the same problems one by one; serve: "
@@ -62,7 +63,9 @@ def parse():
                          "storage under FP64 refinement on 128x64x64, errors against FP64; ku: "
                          "K.u and one PCG iteration on a grid larger than the Infinity Cache, "
                          "beside a device copy; evaluator: a BO batch of latent vectors through "
-                         "VoxelFEMEvaluator.evaluate_batch against evaluate() one by one")
+                         "VoxelFEMEvaluator.evaluate_batch against evaluate() one by one; "
+                         "datagen: TopoDataGenerator samples (SIMP + STL + JSON) as one "
+                         "device batch against generate_single one by one")
     ap.add_argument("--ku-grid", type=int, nargs=3, default=[256, 256, 256])
     ap.add_argument("--mixed-oc-iters", type=int, default=10)
     ap.add_argument("--batch-n", type=int, default=48)
@@ -75,6 +78,7 @@ def parse():
                     help="admission order: arrival (fifo) or single solves first (sjf)")
     ap.add_argument("--eval-q", type=int, default=32, help="latent vectors per BO batch")
     ap.add_argument("--eval-reps", type=int, default=5, help="timed BO batches")
+    ap.add_argument("--datagen-n", type=int, default=64, help="generated samples per batch")
     return ap.parse_args()
 
 
@@ -256,6 +260,50 @@ def bench_evaluator(a):
                         "prep_device_ms_per_batch": t_prep * 1e3,
                         "prep_scipy_host_ms_per_batch": t_prep_cpu * 1e3,
                         "batch_ms": [t * 1e3 for t in times]})
+
+
+def bench_datagen(a):
+    """Data generation: TopoDataGenerator samples with the reference's defaults
+    (medium grids, organic geometry mix, 30 OC iterations, penal 3.5,
+    DenseGPUSolver semantics), drawn from numpy seed 0: SIMP, density_to_stl
+    and the JSON record, as one device batch (generate) against the same
+    number of samples through generate_single one after another."""
+    import tempfile
+
+    import numpy as np
+    import torch
+    from genpipeline_amd.topology.topo_data_gen import TopoDataGenerator
+
+    n = a.datagen_n
+    with tempfile.TemporaryDirectory() as tmp:
+        g = TopoDataGenerator(output_dir=tmp)
+        np.random.seed(99)
+        g.generate(4)  # warm-up
+        np.random.seed(0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        stems = g.generate(n)
+        torch.cuda.synchronize()
+        t_b = time.perf_counter() - t0
+        n_one = min(n, 16)
+        np.random.seed(0)
+        t0 = time.perf_counter()
+        for i in range(n_one):
+            try:
+                g.generate_single(i, n_one)
+            except ValueError:
+                pass
+        torch.cuda.synchronize()
+        t_s = (time.perf_counter() - t0) / n_one * n
+    print(f"[bench] datagen batch {t_b:.2f} s for {n}; one by one {t_s:.2f} s (from {n_one})",
+          file=sys.stderr, flush=True)
+    return _line("SIMP dataset samples/sec (TopoDataGenerator.generate, one device batch)",
+                 n / t_b, "samples/s", 1, 1, t_b * 1e3, "f32",
+                 {"workload": f"topo_datagen_{n}_medium_organic_seed0", "samples": n,
+                  "oc_iters": 30, "semantics": "DenseGPUSolver (reference default use_gpu=True)",
+                  "outputs": "density_to_stl STL + JSON per sample"},
+                 extra={"one_by_one_samples_per_s": n / t_s, "speedup_vs_one_by_one": t_s / t_b,
+                        "written": len(stems), "one_by_one_timed_on": n_one})
 
 
 def bench_mixed128(a):
@@ -442,6 +490,9 @@ def main():
         return
     if a.workload == "evaluator":
         print(json.dumps(bench_evaluator(a)), flush=True)
+        return
+    if a.workload == "datagen":
+        print(json.dumps(bench_datagen(a)), flush=True)
         return
     import torch
     from genpipeline_amd import Problem
