@@ -1646,7 +1646,16 @@ struct gp_sched {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int graph_ticks = 0;
-  SchedTable table() const { return SchedTable{d_slots, n, d_off, d_b2s, d_status}; }
+  // Blocks with work: while every slot runs a job that fills its range the
+  // launch covers all blocks (nact = -1) and ticks replay from the graph;
+  // otherwise only the first nb_cg blocks of each running slot are launched,
+  // listed in d_act (a finished or empty slot's blocks would only idle).
+  int* d_act = nullptr;
+  std::vector<int> h_act;
+  int nact = -1;
+  std::vector<char> done_seen;  // a status read showed the slot's job done
+  SchedTable table() const { return SchedTable{d_slots, n, d_off, d_b2s, d_status, nullptr}; }
+  SchedTable table_act() const { return SchedTable{d_slots, n, d_off, d_b2s, d_status, d_act}; }
 };
 
 namespace {
@@ -1666,7 +1675,51 @@ SchedState idle_state() {
   return z;
 }
 
+// Rebuilds the list of launched blocks after a job starts, ends or is seen
+// finished; the upload is ordered on `s` after every tick issued before.
+int sched_update_active(gp_sched* Q, hipStream_t s) {
+  std::vector<int> a;
+  bool full = true;
+  for (int i = 0; i < Q->n; ++i) {
+    const bool run = Q->busy[i] && !Q->done_seen[i];
+    const int nb = run ? Q->h_slots[i].nb_cg : 0;
+    if (!run || nb != Q->off[i + 1] - Q->off[i]) full = false;
+    for (int k = 0; k < nb; ++k) a.push_back(Q->off[i] + k);
+  }
+  if (full) {
+    Q->nact = -1;
+    return GP_OK;
+  }
+  if (Q->nact == (int)a.size() && a == Q->h_act) return GP_OK;
+  Q->h_act.swap(a);
+  Q->nact = (int)Q->h_act.size();
+  if (Q->nact > 0)
+    HIPCHK(hipMemcpyAsync(Q->d_act, Q->h_act.data(), Q->nact * sizeof(int),
+                          hipMemcpyHostToDevice, s));
+  return GP_OK;
+}
+
+// Marks the jobs a status snapshot shows finished, then updates the list.
+int sched_note_status(gp_sched* Q, const SchedState* st, hipStream_t s) {
+  bool changed = false;
+  for (int i = 0; i < Q->n; ++i)
+    if (Q->busy[i] && !Q->done_seen[i] && st[i].phase == PH_DONE) {
+      Q->done_seen[i] = 1;
+      changed = true;
+    }
+  return changed ? sched_update_active(Q, s) : GP_OK;
+}
+
 int sched_issue(gp_sched* Q, long long n, hipStream_t s) {
+  if (Q->nact >= 0) {
+    // some blocks would only idle: launch just the listed ones (no graph,
+    // whose launches cover every block); no job running, no launch
+    const SchedTable tab = Q->table_act();
+    for (; n > 0; --n, ++Q->t)
+      if (Q->nact > 0) sched_tick(tab, Q->tj, Q->tk, Q->nact, (int)(Q->t & 1), s);
+    LAUNCHCHK();
+    return GP_OK;
+  }
   const SchedTable tab = Q->table();
   const int blocks = Q->off[Q->n];
   while (n > 0) {
@@ -1716,6 +1769,7 @@ int gp_sched_create(int32_t n_slots, const int32_t* slot_dims, int32_t tj, int32
   Q->dims.assign(slot_dims, slot_dims + 4 * n_slots);
   Q->h_slots.assign(n_slots, SchedSlot{});
   Q->busy.assign(n_slots, 0);
+  Q->done_seen.assign(n_slots, 0);
   Q->off.assign(n_slots + 1, 0);
   auto bail = [&](int code) {
     gp_sched_destroy(Q);
@@ -1744,6 +1798,7 @@ int gp_sched_create(int32_t n_slots, const int32_t* slot_dims, int32_t tj, int32
   int rc = alloc(reinterpret_cast<void**>(&Q->d_slots), n_slots * sizeof(SchedSlot));
   if (!rc) rc = alloc(reinterpret_cast<void**>(&Q->d_off), (n_slots + 1) * sizeof(int));
   if (!rc) rc = alloc(reinterpret_cast<void**>(&Q->d_b2s), total * sizeof(int));
+  if (!rc) rc = alloc(reinterpret_cast<void**>(&Q->d_act), total * sizeof(int));
   if (!rc) rc = alloc(reinterpret_cast<void**>(&Q->d_status), n_slots * sizeof(SchedState));
   if (rc) return bail(rc);
   if (hipHostMalloc(reinterpret_cast<void**>(&Q->h_status), 2 * n_slots * sizeof(SchedState)) !=
@@ -1775,6 +1830,7 @@ int gp_sched_destroy(gp_sched* Q) {
   if (Q->d_slots) hipFree(Q->d_slots);
   if (Q->d_off) hipFree(Q->d_off);
   if (Q->d_b2s) hipFree(Q->d_b2s);
+  if (Q->d_act) hipFree(Q->d_act);
   if (Q->d_status) hipFree(Q->d_status);
   if (Q->h_status) hipHostFree(Q->h_status);
   for (auto& e : Q->ev)
@@ -1836,7 +1892,8 @@ int gp_sched_admit(gp_sched* Q, int32_t slot, gp_problem* P, const gp_job* job, 
   HIPCHK(hipMemcpyAsync(Q->d_slots + slot, &q, sizeof(SchedSlot), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(Q->d_status + slot, &q.st[0], sizeof(SchedState), hipMemcpyHostToDevice, s));
   Q->busy[slot] = 1;
-  return GP_OK;
+  Q->done_seen[slot] = 0;
+  return sched_update_active(Q, s);
 }
 
 int gp_sched_set_rhs(gp_sched* Q, int32_t slot, const double* f, void* stream) {
@@ -1862,7 +1919,8 @@ int gp_sched_release(gp_sched* Q, int32_t slot, void* stream) {
                         hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(Q->d_status + slot, &q.st[0], sizeof(SchedState), hipMemcpyHostToDevice, s));
   Q->busy[slot] = 0;
-  return GP_OK;
+  Q->done_seen[slot] = 0;
+  return sched_update_active(Q, s);
 }
 
 int gp_sched_set_graph(gp_sched* Q, int32_t ticks) {
@@ -1898,7 +1956,7 @@ int gp_sched_status(gp_sched* Q, gp_slot_status* out, void* stream) {
   HIPCHK(hipMemcpyAsync(Q->h_status, Q->d_status, Q->n * sizeof(SchedState), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   for (int i = 0; i < Q->n; ++i) to_public(Q->h_status[i], &out[i]);
-  return GP_OK;
+  return sched_note_status(Q, Q->h_status, s);
 }
 
 int gp_sched_run(gp_sched* Q, int64_t max_ticks, int64_t* ticks_run, gp_slot_status* out,
@@ -1934,6 +1992,7 @@ int gp_sched_run(gp_sched* Q, int64_t max_ticks, int64_t* ticks_run, gp_slot_sta
     }
     HIPCHK(hipEventSynchronize(Q->ev[slot]));
     if (all_done(slot) || Q->t - t0 >= max_ticks) break;
+    rc = sched_note_status(Q, Q->h_status + slot * n, s);
     slot ^= 1;
   }
   if (rc) return rc;

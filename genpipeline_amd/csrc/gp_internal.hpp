@@ -182,6 +182,8 @@ struct SchedTable {
   const int* off;      // off[0..n]: slot s owns blocks [off[s], off[s+1])
   const int* b2s;      // slot of every block of the tick
   SchedState* status;  // state after the last tick, one per slot
+  const int* act;      // null: the launch covers every block; else the
+                       // blocks it covers (launch block i runs block act[i])
 };
 
 // Enqueues one tick with parity t & 1 (tj x tk must be 256 threads).

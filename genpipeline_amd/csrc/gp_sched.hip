@@ -160,7 +160,8 @@ template <int TJ, int TK>
 __global__ __launch_bounds__(kBlock, 4) void k_tick(SchedTable tab, int t) {
   static_assert(TJ * TK <= kBlock, "the scheduler runs every body with kBlock threads");
   __shared__ CornerLds<float, TJ * TK> cv;
-  const int gb = xcd_block(blockIdx.x, gridDim.x);
+  const int xb = xcd_block(blockIdx.x, gridDim.x);
+  const int gb = tab.act ? tab.act[xb] : xb;
   const int b = tab.b2s[gb];
   SchedSlot& S = tab.slots[b];
   const int lb = gb - tab.off[b];
