@@ -285,6 +285,8 @@ int ku_dispatch(gp_problem* P, const float* E, const T* u, T* y, bool mask, bool
 int validate_spec(const gp_spec* s) {
   if (!s) return fail(GP_ERR_ARG, "spec is NULL");
   if (s->nx < 1 || s->ny < 1 || s->nz < 1) return fail(GP_ERR_ARG, "grid must be >= 1 per axis");
+  if (s->nx > (1 << 20) || s->ny > (1 << 20) || s->nz > (1 << 20))
+    return fail(GP_ERR_ARG, "grid axis above 2^20 elements");
   const long long nn = (long long)(s->nx + 1) * (s->ny + 1) * (s->nz + 1);
   if (nn > (1ll << 29)) return fail(GP_ERR_ARG, "grid too large (%lld nodes)", nn);
   if (s->law != GP_LAW_CLAMP && s->law != GP_LAW_EMIN) return fail(GP_ERR_ARG, "bad law");

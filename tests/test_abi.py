@@ -52,3 +52,31 @@ def test_argument_errors_without_gpu():
     spec.rmin = 1.5
     assert lib.gp_footprint_bytes(ctypes.byref(spec), 0, ctypes.byref(out)) == _lib.GP_OK
     assert out.value > 0
+
+
+def test_size_limits_without_gpu():
+    """Maximum sizes are rejected before any allocation: a grid above 2^29
+    nodes, an axis past 2^20 (whose node count would overflow 32 bits
+    first), isosurface grids without a cube, and unsymmetric pre-processing
+    taps."""
+    lib = _lib.load()
+    spec = _lib.GpSpec()
+    spec.rmin = 1.5
+    spec.nu = 0.3
+    out = ctypes.c_size_t()
+    for dims in ((1024, 1024, 1024), (2 ** 31 - 1, 1, 1), (1, 2 ** 21, 1)):
+        spec.nx, spec.ny, spec.nz = dims
+        assert lib.gp_footprint_bytes(ctypes.byref(spec), 0, ctypes.byref(out)) == _lib.GP_ERR_ARG
+        h = ctypes.c_void_p()
+        assert lib.gp_problem_create(ctypes.byref(spec), ctypes.byref(h)) == _lib.GP_ERR_ARG
+        assert not h.value
+    spec.nx, spec.ny, spec.nz = 511, 511, 511  # 2^27 nodes: accepted
+    assert lib.gp_footprint_bytes(ctypes.byref(spec), 0, ctypes.byref(out)) == _lib.GP_OK
+    n = ctypes.c_int64()
+    assert lib.gp_mc_workspace(1, 5, 5, ctypes.byref(n)) == _lib.GP_ERR_ARG
+    assert lib.gp_mc_workspace(3, 3, 3, ctypes.byref(n)) == _lib.GP_OK and n.value == 2
+    w = (ctypes.c_double * 5)(0.1, 0.2, 0.4, 0.2, 0.3)  # not symmetric
+    buf = ctypes.c_void_p(1)
+    assert lib.gp_voxel_prep(buf, 1, 4, 4, 4, 2, 2, 2, w, 2, ctypes.c_double(15.0), buf, buf,
+                             None) == _lib.GP_ERR_ARG
+    assert b"symmetric" in lib.gp_last_error()
