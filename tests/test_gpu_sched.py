@@ -142,3 +142,50 @@ def test_admission_errors(gpu):
     with pytest.raises(_lib.GpError):
         Scheduler([(8, 8, 8, 2)], tile=(8, 32)).admit(5, P)
     S.run()
+
+
+def test_compacted_launches_each_equals_alone(gpu):
+    """Once the small jobs finish and a large one (>= 96 sweep blocks) runs
+    on, ticks launch only the running jobs' blocks instead of replaying the
+    graph of every block; a slot refilled meanwhile joins the list. Every
+    job still equals Problem.run bit for bit."""
+    from genpipeline_amd.sched import Scheduler
+    big = _problem(gpu, 48, 24, 24, 7, volfrac=0.3)
+    small = [_problem(gpu, *s, seed=10 + i) for i, s in enumerate([(8, 8, 8), (10, 6, 6), (12, 6, 5)])]
+    late = _problem(gpu, 16, 8, 8, 21, oc="gpu")
+    ref_big = _alone(big, (14, 18), 2, 4, tol=1e-6, max_iter=5000, warm_start=True)
+    ref_small = [_alone(P, (14, 18), 2, 1, tol=1e-6, max_iter=5000, warm_start=True) for P in small]
+    ref_late = _alone(late, (14, 18), 2, 2, tol=1e-6, max_iter=5000, warm_start=True)
+    S = Scheduler([(48, 24, 24, 2), (8, 8, 8, 2), (10, 6, 6, 2), (16, 8, 8, 2)], tile=(14, 18),
+                  graph_ticks=32)
+    try:
+        assert S.blocks(0) >= 96  # large enough for compacted launches
+        S.admit(0, big, n_iters=4, tol=1e-6, max_iter=5000, warm_start=True)
+        for k, P in enumerate(small[:2]):
+            S.admit(1 + k, P, n_iters=1, tol=1e-6, max_iter=5000, warm_start=True)
+        S.admit(3, small[2], n_iters=1, tol=1e-6, max_iter=5000, warm_start=True)
+        print("compacted launches: running until the small jobs finish", flush=True)
+        results, refilled, res_big, res_late = {}, False, None, None
+        for _ in range(20000):
+            S.tick(32)
+            st = S.status()
+            for slot in (1, 2):
+                if slot not in results and st[slot].done:
+                    results[slot] = S.result(slot, st[slot])
+            if not refilled and st[3].done:
+                _same(S.result(3, st[3]), ref_small[2])
+                S.release(3)
+                S.admit(3, late, n_iters=2, tol=1e-6, max_iter=5000, warm_start=True)
+                refilled = True
+                continue  # the next status read is the new job's
+            if refilled and st[3].done and st[0].done:
+                res_late, res_big = S.result(3, st[3]), S.result(0, st[0])
+                break
+        assert res_big is not None and res_late is not None
+        _same(res_big, ref_big)
+        _same(res_late, ref_late)
+        _same(results[1], ref_small[0])
+        _same(results[2], ref_small[1])
+    finally:
+        torch.cuda.synchronize()
+        S.close()
