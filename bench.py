@@ -574,8 +574,10 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
             "algorithmic_bytes_per_launch": it_bytes,
-            "ku_only_bytes_per_launch": ku_bytes,
-            "ku_only_gbs": ku_bytes / (ku_ms * 1e-3) / 1e9,
+            # K.z's own operand bytes (z read, w written, E read) inside the
+            # fused launch, over the whole launch; K.u alone: --workload ku
+            "ku_operand_bytes_per_launch": ku_bytes,
+            "ku_operand_gbs_within_fused_launch": ku_bytes / (ku_ms * 1e-3) / 1e9,
             "avg_launch_us": ku_ms * 1e3,
             "event_samples": ku_samples,
         },
