@@ -80,3 +80,16 @@ def test_size_limits_without_gpu():
     assert lib.gp_voxel_prep(buf, 1, 4, 4, 4, 2, 2, 2, w, 2, ctypes.c_double(15.0), buf, buf,
                              None) == _lib.GP_ERR_ARG
     assert b"symmetric" in lib.gp_last_error()
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: any other --gpus value exits with status 2
+    and says so, before touching a device."""
+    import subprocess
+    import sys
+    root = HEADER.parent.parent
+    p = subprocess.run([sys.executable, str(root / "bench.py"), "--gpus", "2"], cwd=root,
+                       capture_output=True, text=True, timeout=300)
+    assert p.returncode == 2
+    assert "one GPU only" in p.stdout + p.stderr
+    assert p.stdout.strip() == "" or not p.stdout.lstrip().startswith("{")
