@@ -523,14 +523,22 @@ int pcg_core(gp_problem* P, const float* E, const double* f, T* u, double tol, i
   T* wb[2] = {base + 2 * nd, base + 3 * nd};
   T* sb[2] = {base + 5 * nd, base + 6 * nd};
   T* pc = base + 7 * nd;
-  launch_diag_minv<T>(g, ke, E, P->fixmask, minv, true, s);
+  // the float sweep reads M^-1 per node; FP64 and the gather path per DOF
+  if constexpr (std::is_same<T, float>::value) {
+    if (modal)
+      launch_diag_node(g, P->ke32, E, minv, s);
+    else
+      launch_diag_minv<T>(g, ke, E, P->fixmask, minv, true, s);
+  } else {
+    launch_diag_minv<T>(g, ke, E, P->fixmask, minv, true, s);
+  }
   if (use_x0) {
     launch_mask_fixed<T>(g, u, P->fixmask, s);
     ku_dispatch<T>(P, E, u, modal ? wb[0] : Ap, true, false, s);
   }
   if (modal)
-    launch_cg_init<T>(g, f, wb[0], use_x0, u, rb[1], sb[0], wb[1], pc, minv, P->parts, sc, tol,
-                      max_iter, s);
+    launch_cg_init<T>(g, f, wb[0], use_x0, u, rb[1], sb[0], wb[1], pc, minv, P->fixmask,
+                      P->parts, sc, tol, max_iter, s);
   else
     launch_pcg_init<T>(g, f, Ap, use_x0, u, r, z, p, minv, P->parts, sc, tol, max_iter, s);
   LAUNCHCHK();
@@ -576,9 +584,9 @@ int pcg_core_lp(gp_problem* P, const float* E, const double* rhs, S* d, double t
   const ModalCoef<float> kc = P->mc32;
   const int prec = std::is_same<S, bf16>::value ? GP_BF16 : GP_F16;
   const KuTile tile = P->tile32.tj ? P->tile32 : KuTile{8, 32, 2};
-  launch_diag_minv<float>(g, P->ke32, E, P->fixmask, minv, true, s);
-  launch_cg_init<float, S>(g, rhs, nullptr, false, d, rb[1], sb[0], wb[1], pc, minv, P->parts, sc,
-                           tol, max_iter, s);
+  launch_diag_node(g, P->ke32, E, minv, s);
+  launch_cg_init<float, S>(g, rhs, nullptr, false, d, rb[1], sb[0], wb[1], pc, minv, P->fixmask,
+                           P->parts, sc, tol, max_iter, s);
   LAUNCHCHK();
   return pcg_drive(
       P, 2, max_iter, true,

@@ -25,13 +25,13 @@ __global__ __launch_bounds__(kBlock) void k_b_stiffness(BatchLaunch bl) {
 
 __global__ __launch_bounds__(kBlock) void k_b_diag(BatchLaunch bl) {
   GP_BATCH_BLOCK(off_node)
-  body_diag<float>(P.g, P.ke32, P.E, P.fixmask, P.minv, 1, lb, nb);
+  body_diag<float>(P.g, P.ke32, P.E, P.fixmask, P.minv, 2, lb, nb);  // per node
 }
 
 __global__ __launch_bounds__(kBlock) void k_b_cg_init(BatchLaunch bl, int use_x0) {
   GP_BATCH_BLOCK(off_vec)
   body_cg_init<float>(P.g.nn, P.f, P.wb[0], use_x0, P.u, P.rb[1], P.sb[0], P.wb[1], P.pc, P.minv,
-                      P.parts, lb, nb);
+                      P.fixmask, 1, P.parts, lb, nb);
 }
 
 __global__ __launch_bounds__(kBlock) void k_b_init_fin(BatchLaunch bl, double tol, int max_iter) {

@@ -124,8 +124,12 @@ __device__ inline void body_stiffness(int ne, const double* __restrict__ x, floa
     E[e] = (float)law_E(x[e], law, p, E0, Emin);
 }
 
-// diag(K) per DOF; with invert: Jacobi M^-1 (0 on fixed DOFs and on DOFs
-// with a non-positive diagonal).
+// diag(K) per DOF; with invert 1: Jacobi M^-1 (0 on fixed DOFs and on DOFs
+// with a non-positive diagonal). invert 2: one M^-1 per node, from the x
+// component, unmasked (out has n_nodes entries). For the isotropic element
+// the three diagonal entries of a node are equal; the float sweep reads this
+// form and the support bits stay with the recurrence (r, and so z, is 0 on
+// a fixed DOF from the set-up on).
 template <typename T>
 __device__ inline void body_diag(const Grid& g, const T* __restrict__ ke,
                                  const float* __restrict__ E, const uint8_t* __restrict__ fixmask,
@@ -149,6 +153,10 @@ __device__ inline void body_diag(const Grid& g, const T* __restrict__ ke,
           d1 += Ee * ke[(3 * L + 1) * 24 + 3 * L + 1];
           d2 += Ee * ke[(3 * L + 2) * 24 + 3 * L + 2];
         }
+    if (invert == 2) {
+      out[n] = d0 > 0 ? T(1) / d0 : T(0);
+      continue;
+    }
     if (invert) {
       const uint8_t fm = fixmask ? fixmask[n] : 0;
       d0 = ((fm & 1) || !(d0 > 0)) ? T(0) : T(1) / d0;
@@ -162,16 +170,26 @@ __device__ inline void body_diag(const Grid& g, const T* __restrict__ ke,
 // Single-kernel PCG set-up: r_{-1} = b - A x0 on free DOFs (or b), x masked,
 // s_{-2} = 0, w_{-1} = 0, p = 0; partials bb -> BB[lb], rr -> RR[lb].
 // S: storage of x and p (T for FP32/FP64; a 16-bit type with T = float).
+// A DOF is free when its M^-1 is nonzero: minv per DOF, or with node_m one
+// value per node (body_diag's invert 2) and the support bits.
 template <typename T, typename S = T>
 __device__ inline void body_cg_init(int nn, const double* __restrict__ f, const T* __restrict__ Ax,
                                     int use_x0, S* __restrict__ x, T* __restrict__ r,
                                     T* __restrict__ s0, T* __restrict__ w1, S* __restrict__ p,
-                                    const T* __restrict__ minv, const Parts& parts, int lb,
-                                    int nb) {
+                                    const T* __restrict__ minv, const uint8_t* __restrict__ fixmask,
+                                    int node_m, const Parts& parts, int lb, int nb) {
   double rr = 0.0, bb = 0.0;
   for (long long d = lb * (long long)kBlock + threadIdx.x; d < 3ll * nn;
        d += (long long)nb * kBlock) {
-    const bool freed = minv[d] != T(0);
+    bool freed;
+    if (node_m) {
+      const long long n = d / 3;
+      const int c = (int)(d - 3 * n);
+      const bool fixed = fixmask != nullptr && ((fixmask[n] >> c) & 1);
+      freed = !fixed && minv[n] != T(0);
+    } else {
+      freed = minv[d] != T(0);
+    }
     const double bv = freed ? f[d] : 0.0;
     const S xd = (use_x0 && freed) ? x[d] : st_s<S>(T(0));
     const T rd = freed ? (T)(bv - (use_x0 ? (double)Ax[d] : 0.0)) : T(0);

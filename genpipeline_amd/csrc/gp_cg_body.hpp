@@ -28,7 +28,9 @@ struct RawNode {
 // of elements outside the grid, whose modulus is exactly 0, and it is never
 // stored or summed, so its (finite) values do not reach any result. Leaving
 // the loaded values unselected lets all of a node's loads issue back to back.
-template <typename T>
+// NM: M^-1 is one value per node (float sweeps; body_diag's invert 2), else
+// one per DOF.
+template <typename T, bool NM>
 __device__ inline void load_node(const Grid& g, const T* __restrict__ rp, const T* __restrict__ wp,
                                  const T* __restrict__ sp, const T* __restrict__ m, int ip, int jj,
                                  int kk, RawNode<T>& o) {
@@ -40,7 +42,14 @@ __device__ inline void load_node(const Grid& g, const T* __restrict__ rp, const 
     o.r[q] = rp[3 * n + q];
     o.w[q] = wp[3 * n + q];
     o.s[q] = sp[3 * n + q];
-    o.m[q] = m[3 * n + q];
+  }
+  if constexpr (NM) {
+    const T mv = m[n];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) o.m[q] = mv;
+  } else {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) o.m[q] = m[3 * n + q];
   }
 }
 
@@ -139,6 +148,8 @@ __device__ inline bool body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
                                      CornerLds<T, TJ * TK>& cv) {
   constexpr int NT = TJ * TK;
   constexpr int NH = TJ + TK + 1;  // halo node columns (aa = 0 or bb = 0)
+  // float sweeps take M^-1 per node (minv has n_nodes entries), FP64 per DOF
+  constexpr bool NM = sizeof(T) == 4;
   static_assert(NB >= NT && (NB + 63) / 64 == (NT + 63) / 64, "padding within the last wave");
   static_assert(NB % 64 == 0, "whole waves: the block sums use the DPP wave sum");
   __shared__ T zs[2][3][TJ + 1][TK + 1];
@@ -174,8 +185,8 @@ __device__ inline bool body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
   auto load_stage = [&](int ip, bool own_plane, StageIn<T>& in) {
     const bool fin = owner && own_plane;
     const long long n = (long long)ip * g.sI + (long long)(je + 1) * g.sJ + (ke + 1);
-    load_node(g, rp, wp, sp, minv, ip, je + 1, ke + 1, in.ro);
-    if (has_h) load_node(g, rp, wp, sp, minv, ip, J0 - 1 + ha, K0 - 1 + hb, in.rh);
+    load_node<T, NM>(g, rp, wp, sp, minv, ip, je + 1, ke + 1, in.ro);
+    if (has_h) load_node<T, NM>(g, rp, wp, sp, minv, ip, J0 - 1 + ha, K0 - 1 + hb, in.rh);
 #pragma unroll
     for (int q = 0; q < 3; ++q) in.pv[q] = in.xv[q] = T(0);
     in.fm = 0;

@@ -112,24 +112,27 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(int nn, const double* __rest
                                                    S* __restrict__ x, T* __restrict__ r,
                                                    T* __restrict__ s0, T* __restrict__ w1,
                                                    S* __restrict__ p, const T* __restrict__ minv,
+                                                   const uint8_t* __restrict__ fixmask,
                                                    Parts parts) {
-  body_cg_init<T, S>(nn, f, Ax, use_x0, x, r, s0, w1, p, minv, parts, blockIdx.x, gridDim.x);
+  // float sweeps read M^-1 per node (launch_diag_node)
+  body_cg_init<T, S>(nn, f, Ax, use_x0, x, r, s0, w1, p, minv, fixmask, sizeof(T) == 4 ? 1 : 0,
+                     parts, blockIdx.x, gridDim.x);
 }
 
 template <typename T, typename S>
 void launch_cg_init(const Grid& g, const double* f, const T* Ax, bool use_x0, S* x, T* r, T* s0,
-                    T* w1, S* p, const T* minv, Parts parts, DevScalars* sc, double tol,
-                    int max_iter, hipStream_t st) {
+                    T* w1, S* p, const T* minv, const uint8_t* fixmask, Parts parts,
+                    DevScalars* sc, double tol, int max_iter, hipStream_t st) {
   const int nV = vec_blocks(g);
   hipLaunchKernelGGL((k_cg_init<T, S>), dim3(nV), dim3(kBlock), 0, st, g.nn, f, Ax,
-                     use_x0 ? 1 : 0, x, r, s0, w1, p, minv, parts);
+                     use_x0 ? 1 : 0, x, r, s0, w1, p, minv, fixmask, parts);
   launch_pcg_init_fin(parts, nV, tol, max_iter, sc, st);
 }
 
 #define GP_INST(T, S)                                                                         \
   template void launch_cg_init<T, S>(const Grid&, const double*, const T*, bool, S*, T*, T*, \
-                                     T*, S*, const T*, Parts, DevScalars*, double, int,      \
-                                     hipStream_t);
+                                     T*, S*, const T*, const uint8_t*, Parts, DevScalars*,   \
+                                     double, int, hipStream_t);
 GP_INST(float, float)
 GP_INST(double, double)
 GP_INST(float, _Float16)

@@ -285,6 +285,10 @@ int modal_blocks(const Grid& g, const KuTile& t);
 template <typename T>
 void launch_diag_minv(const Grid& g, const T* ke, const float* E, const uint8_t* fixmask,
                       T* diag_or_minv, bool invert, hipStream_t st);
+// Jacobi M^-1 with one value per node (n_nodes floats, unmasked), the form
+// the float single-kernel PCG sweep reads.
+void launch_diag_node(const Grid& g, const float* ke, const float* E, float* minv_node,
+                      hipStream_t st);
 template <typename T>
 void launch_mask_fixed(const Grid& g, T* x, const uint8_t* fixmask, hipStream_t st);
 int vec_blocks(const Grid& g);
@@ -297,8 +301,8 @@ void launch_pcg_init_fin(Parts parts, int nV, double tol, int max_iter, DevScala
 // S: storage of x and p (T, or _Float16 / bf16 with T = float)
 template <typename T, typename S = T>
 void launch_cg_init(const Grid& g, const double* f, const T* Ax, bool use_x0, S* x, T* r, T* s0,
-                    T* w1, S* p, const T* minv, Parts parts, DevScalars* sc, double tol,
-                    int max_iter, hipStream_t st);
+                    T* w1, S* p, const T* minv, const uint8_t* fixmask, Parts parts,
+                    DevScalars* sc, double tol, int max_iter, hipStream_t st);
 // One Jacobi-PCG iteration k (Chronopoulos-Gear form) on the modal sweep
 // (prec: GP_F32 / GP_F64, or GP_F16 / GP_BF16 16-bit storage of x and p):
 // reads r_{k-1} (rp), w_{k-1} (wp), s_{k-2} (sp); writes r_k, w_k, s_{k-1}

@@ -132,6 +132,12 @@ void launch_diag_minv(const Grid& g, const T* ke, const float* E, const uint8_t*
                      out, invert ? 1 : 0);
 }
 
+void launch_diag_node(const Grid& g, const float* ke, const float* E, float* out,
+                      hipStream_t st) {
+  hipLaunchKernelGGL((k_diag<float>), dim3(grid_blocks(g.nn)), dim3(kBlock), 0, st, g, ke, E,
+                     nullptr, out, 2);
+}
+
 template <typename T>
 __global__ void k_mask_fixed(Grid g, T* x, const uint8_t* __restrict__ fixmask) {
   const int n = blockIdx.x * kBlock + threadIdx.x;

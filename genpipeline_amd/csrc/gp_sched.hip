@@ -181,7 +181,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_tick(SchedTable tab, int t) {
       body_stiffness(P.g.ne, P.xPhys, P.E, P.law, P.penal, P.E0, P.Emin, lb, nb);
       break;
     case PH_DIAG:
-      body_diag<float>(P.g, P.ke32, P.E, P.fixmask, P.minv, 1, lb, nb);
+      body_diag<float>(P.g, P.ke32, P.E, P.fixmask, P.minv, 2, lb, nb);  // per node
       break;
     case PH_KX0:
       body_ku_sweep<float, TJ, TK, kBlock>(P.g, P.ni, P.ntj, P.ntk, P.mc, P.E, P.u, P.wb[0],
@@ -191,7 +191,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_tick(SchedTable tab, int t) {
       const int x0 = sched_use_x0(cur, S.cfg) ? 1 : 0;
       for (int v = lb; v < S.nb_vec; v += nb)
         body_cg_init<float>(P.g.nn, P.f, P.wb[0], x0, P.u, P.rb[1], P.sb[0], P.wb[1], P.pc,
-                            P.minv, P.parts, v, S.nb_vec);
+                            P.minv, P.fixmask, 1, P.parts, v, S.nb_vec);
       if (lb == 0 && threadIdx.x == 0) P.sc_oc->nan_flag = 0;
       break;
     }
