@@ -195,7 +195,8 @@ class VoxelFEMEvaluator:
             if not todo:
                 break
             corr = srv.run([Request(b, 0.0, shape, vox[b], "solve", 0.0, rhs=rs[b],
-                                    tol=self.batch_inner_tol) for b in todo], keep_u=True)
+                                    tol=self.batch_inner_tol) for b in todo], keep_u=True,
+                           outputs=False)
             for b, c in zip(todo, corr):
                 if c.status != 0:
                     failed.add(b)
@@ -219,17 +220,20 @@ class VoxelFEMEvaluator:
         tol0 = self.batch_inner_tol if self.batch_refine else self.batch_solve_tol
         reqs = [Request(b, 0.0, shape, vox[b], "solve", 0.0, tol=tol0) for b in range(len(zs))]
         try:
-            res = self._server(shape).run(reqs, keep_u=self.batch_refine)
+            # refined fields take their outputs after refinement
+            res = self._server(shape).run(reqs, keep_u=self.batch_refine,
+                                          outputs=not self.batch_refine)
             failed = {b for b, r in enumerate(res) if r.status != 0}
             fem = [(r.compliance, r.stress_max) for r in res]
             if self.batch_refine:
-                from ..serve import fem_outputs
+                from ..serve import fem_outputs_batch
                 us, bad, P = self._refine(shape, vox, [r.u for r in res])
                 failed |= bad
-                for b in range(len(zs)):
-                    if b not in failed:
-                        c, sm, _ = fem_outputs(P.compliance(us[b]), us[b], vox[b])
-                        fem[b] = (c, sm)
+                ok = [b for b in range(len(zs)) if b not in failed]
+                outs = fem_outputs_batch([P.compliance(us[b]) for b in ok], [us[b] for b in ok],
+                                         vox[ok])
+                for b, (c, sm, _) in zip(ok, outs):
+                    fem[b] = (c, sm)
         except Exception as e:
             logger.warning(f"[VoxelFEM] batched GPU FEM failed: {e}")
             srv = self._servers.pop(shape, None)

@@ -117,6 +117,15 @@ def make_problem(shape, kind, cfg: ServeConfig, device) -> Problem:
     return Problem(spec, fixed, f, device=device)
 
 
+def _fem_rule(compliance: float, n_solid: int, disp: float):
+    """GPUVoxelFEM's output proxies and sentinels (gpu_fem_solver.py:96-133)."""
+    if n_solid < 50 or compliance < 1.0:
+        return 1e6, 1e4, 1e3
+    return (float(np.clip(compliance, 0, 1e8)),
+            float(np.clip(compliance / max(n_solid, 1) * 10.0, 0, 1e4)),
+            float(np.clip(disp * 1e-3 * 1e3, 0, 1e4)))
+
+
 def fem_outputs(compliance: float, u: torch.Tensor, density: np.ndarray):
     """(compliance, stress_max, displacement_max) with GPUVoxelFEM's proxies and
     sentinels (gpu_fem_solver.py:96-133)."""
@@ -125,11 +134,17 @@ def fem_outputs(compliance: float, u: torch.Tensor, density: np.ndarray):
         n_solid = int((density > 0.5).sum().item())
     else:
         n_solid = int((np.asarray(density) > 0.5).sum())
-    if n_solid < 50 or compliance < 1.0:
-        return 1e6, 1e4, 1e3
-    return (float(np.clip(compliance, 0, 1e8)),
-            float(np.clip(compliance / max(n_solid, 1) * 10.0, 0, 1e4)),
-            float(np.clip(disp * 1e-3 * 1e3, 0, 1e4)))
+    return _fem_rule(compliance, n_solid, disp)
+
+
+def fem_outputs_batch(compliances, us, densities: torch.Tensor) -> list:
+    """fem_outputs of B fields of one grid (densities: a (B, ...) device
+    tensor) with two read-backs for the batch instead of two per field."""
+    if not len(us):
+        return []
+    disp = torch.stack([u.view(-1, 3) for u in us]).norm(dim=2).amax(dim=1).tolist()
+    n_solid = (densities.reshape(len(us), -1) > 0.5).sum(dim=1).tolist()
+    return [_fem_rule(c, int(n), d) for c, n, d in zip(compliances, n_solid, disp)]
 
 
 class Server:
@@ -186,10 +201,14 @@ class Server:
         return make_problem(shape, kind, self.cfg, self.device)
 
     def run(self, requests: Sequence[ServeRequest], time_scale: float = 1.0,
-            keep_xphys: bool = False, keep_u: bool = False) -> list[ServeResult]:
+            keep_xphys: bool = False, keep_u: bool = False,
+            outputs: bool = True) -> list[ServeResult]:
         """Serve requests arriving at t_arrive * time_scale seconds after the
         call; returns one result per request (request order). keep_xphys /
-        keep_u keep each job's density / displacement (device tensors)."""
+        keep_u keep each job's density / displacement (device tensors).
+        outputs=False skips the FEM output formulas of finished solves (each
+        reads two device reductions back): compliance is then the solver's,
+        stress_max and displacement_max None."""
         reqs = sorted(requests, key=lambda r: r.t_arrive)
         results: dict[int, ServeResult] = {}
         queue: deque = deque()
@@ -238,7 +257,7 @@ class Server:
                 r, P, t_admit = active.pop(slot)
                 job = self.S.jobs[slot]
                 s = st[slot]
-                if r.kind == "solve":
+                if r.kind == "solve" and outputs:
                     c, sm, dm = fem_outputs(s.compliance, job.u, r.density)
                 else:
                     c, sm, dm = s.compliance, None, None
