@@ -119,7 +119,10 @@ struct gp_problem {
   int device = 0;
   double ke64_h[576];
   Stencil st_h{};
-  // device constants
+  // device constants: the set-up buffers below (element tables, stencil,
+  // Hs, w, supports, load, scalars, partials) are 256-byte aligned pieces of
+  // one allocation, so a problem costs one hipMalloc and one hipFree for them
+  void* arena = nullptr;
   double* ke64 = nullptr;
   float* ke32 = nullptr;
   Stencil* st = nullptr;
@@ -751,18 +754,33 @@ int gp_problem_create(const gp_spec* spec, gp_problem** out) {
     int _r = alloc(reinterpret_cast<void**>(&(ptr)), (bytes));            \
     if (_r) return cleanup_fail(_r);                                      \
   } while (0)
-  CALLOC(P->ke64, 576 * sizeof(double));
-  CALLOC(P->ke32, 576 * sizeof(float));
-  CALLOC(P->st, sizeof(Stencil));
-  CALLOC(P->Hs, g.ne * sizeof(double));
-  CALLOC(P->w, g.ne * sizeof(double));
-  CALLOC(P->fixmask, g.nn);
-  CALLOC(P->f, g.ndof * sizeof(double));
-  CALLOC(P->sc, 3 * sizeof(DevScalars));
+  const size_t cap = partial_cap(g);
+  P->n_partials = kRegions * cap;
+  size_t arena_bytes = 0;
+  auto carve = [&](size_t bytes) {
+    const size_t at = arena_bytes;
+    arena_bytes += (bytes + 255) & ~size_t(255);
+    return at;
+  };
+  const size_t o_ke64 = carve(576 * sizeof(double)), o_ke32 = carve(576 * sizeof(float));
+  const size_t o_st = carve(sizeof(Stencil)), o_hs = carve(g.ne * sizeof(double));
+  const size_t o_w = carve(g.ne * sizeof(double)), o_fix = carve(g.nn);
+  const size_t o_f = carve(g.ndof * sizeof(double)), o_sc = carve(3 * sizeof(DevScalars));
+  const size_t o_part = carve(P->n_partials * sizeof(double));
+  CALLOC(P->arena, arena_bytes);
   {
-    const size_t cap = partial_cap(g);
-    P->n_partials = kRegions * cap;
-    CALLOC(P->partials, P->n_partials * sizeof(double));
+    char* a0 = static_cast<char*>(P->arena);
+    P->ke64 = reinterpret_cast<double*>(a0 + o_ke64);
+    P->ke32 = reinterpret_cast<float*>(a0 + o_ke32);
+    P->st = reinterpret_cast<Stencil*>(a0 + o_st);
+    P->Hs = reinterpret_cast<double*>(a0 + o_hs);
+    P->w = reinterpret_cast<double*>(a0 + o_w);
+    P->fixmask = reinterpret_cast<uint8_t*>(a0 + o_fix);
+    P->f = reinterpret_cast<double*>(a0 + o_f);
+    P->sc = reinterpret_cast<DevScalars*>(a0 + o_sc);
+    P->partials = reinterpret_cast<double*>(a0 + o_part);
+  }
+  {
     double* b = P->partials;
     P->parts.cap = (int)cap;
     P->parts.A = b;
@@ -799,13 +817,12 @@ int gp_problem_create(const gp_spec* spec, gp_problem** out) {
       hipEventCreateWithFlags(&P->ev[1], hipEventDisableTiming) != hipSuccess)
     return cleanup_fail(fail(GP_ERR_HIP, "hipEventCreate failed"));
   hipError_t e = hipSuccess;
+  // zero the whole set-up allocation (scalars, supports, load and partials
+  // start at 0), then the tables
+  e = e != hipSuccess ? e : hipMemset(P->arena, 0, arena_bytes);
   e = e != hipSuccess ? e : hipMemcpy(P->ke64, P->ke64_h, sizeof(double) * 576, hipMemcpyHostToDevice);
   e = e != hipSuccess ? e : hipMemcpy(P->ke32, ke32, sizeof(float) * 576, hipMemcpyHostToDevice);
   e = e != hipSuccess ? e : hipMemcpy(P->st, &P->st_h, sizeof(Stencil), hipMemcpyHostToDevice);
-  e = e != hipSuccess ? e : hipMemset(P->sc, 0, 3 * sizeof(DevScalars));
-  e = e != hipSuccess ? e : hipMemset(P->fixmask, 0, g.nn);
-  e = e != hipSuccess ? e : hipMemset(P->f, 0, g.ndof * sizeof(double));
-  e = e != hipSuccess ? e : hipMemset(P->partials, 0, P->n_partials * sizeof(double));
   if (e != hipSuccess) return cleanup_fail(fail(GP_ERR_HIP, "init copy: %s", hipGetErrorString(e)));
   launch_hs(g, P->st, P->Hs, nullptr);
   launch_oc_weights(g, P->st, P->Hs, nullptr, P->w, nullptr);
@@ -820,8 +837,7 @@ int gp_problem_destroy(gp_problem* P) {
   if (!P) return GP_OK;
   hipSetDevice(P->device);
   hipDeviceSynchronize();
-  void* dev[] = {P->ke64, P->ke32, P->st, P->Hs, P->w, P->fixmask, P->pres, P->f,
-                 P->sc, P->partials, P->ws32, P->ws64, P->ws16, P->r64, P->y64, P->d32, P->E,
+  void* dev[] = {P->arena, P->pres, P->ws32, P->ws64, P->ws16, P->r64, P->y64, P->d32, P->E,
                  P->dc, P->dcf, P->xnew, P->xphys_new, P->ce_sum};
   for (void* p : dev)
     if (p) hipFree(p);
