@@ -167,6 +167,35 @@ def test_pcg_max_iter_reports_not_converged(gpu, kernel):
     assert res.status == 5 and res.iters == 5
 
 
+@pytest.mark.parametrize("kernel", ["modal", "gather"])
+@pytest.mark.parametrize("refine,tol,bound", [(True, 1e-11, 1e-9), (False, 1e-6, 2e-4)])
+def test_fp32_pcg_with_partial_supports(gpu, kernel, refine, tol, bound):
+    """Supports that fix only one component of a node (a roller on face
+    y = 0 beside the clamp at x = 0). The float single-kernel sweep reads one
+    Jacobi value per node and keeps the support bits out of the sweep; its
+    solution matches the direct solve as the gather path's does, and every
+    fixed DOF stays exactly 0."""
+    grid = (9, 7, 5)
+    nx, ny, nz = grid
+    spec, fixed, f, x, rng = _problem(*grid, seed=21)
+    roller = [3 * O.node_id(i, 0, k, ny, nz) + 2 for i in range(1, nx + 1) for k in range(nz + 1)]
+    fixed = np.union1d(fixed, np.asarray(roller, dtype=np.int64))
+    KE = O.ke_hex8(0.3)
+    edof = O.edof_matrix(*grid)
+    E = O.stiffness(x, "emin", 3.0).astype(np.float32)
+    K = O.assemble_K(KE, E.astype(np.float64), edof, spec.n_dof)
+    u_ref, _ = O.solve(K, f, fixed, "direct")
+    with Problem(spec, fixed, f, device=gpu) as P:
+        if kernel == "gather":
+            P.set_ku_tile(0, precision="fp32")
+        u, res = P.pcg(torch.from_numpy(E), tol=tol, max_iter=20000, precision="fp32",
+                       refine=refine, inner_tol=1e-4)
+        u = u.cpu().numpy().astype(np.float64)
+    assert res.status == 0, res
+    assert np.all(u[fixed] == 0.0)
+    assert np.abs(u - u_ref).max() <= bound * np.abs(u_ref).max()
+
+
 def test_pcg_modal_and_gather_agree(gpu):
     """Same iteration count and solution (FP64) with either K.u kernel."""
     grid = (9, 7, 5)
