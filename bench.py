@@ -540,7 +540,8 @@ def main():
     if tf.exists():
         try:
             t = json.loads(tf.read_text())
-            if t.get("grid") == list(a.grid):
+            # the measured traffic holds for the grid and sweep tile it was taken on
+            if t.get("grid") == list(a.grid) and t.get("tile") == list(ku_tile)[:3]:
                 traffic = t.get("bytes_per_launch")
         except (ValueError, OSError):
             traffic = None
