@@ -20,6 +20,7 @@ GP_F32, GP_F64, GP_F16, GP_BF16 = 0, 1, 2, 3
 GP_LAW_CLAMP, GP_LAW_EMIN = 0, 1
 GP_OC_CPU, GP_OC_GPU = 0, 1
 GP_KE_HEX8, GP_KE_CUSTOM = 0, 1
+GP_FILTER_F64, GP_FILTER_F32 = 0, 1
 
 
 class GpSpec(C.Structure):
@@ -27,7 +28,7 @@ class GpSpec(C.Structure):
                 ("law", C.c_int32), ("oc_variant", C.c_int32), ("ke_kind", C.c_int32),
                 ("E0", C.c_double), ("Emin", C.c_double), ("nu", C.c_double),
                 ("penal", C.c_double), ("rmin", C.c_double), ("move", C.c_double),
-                ("ke", C.POINTER(C.c_double))]
+                ("ke", C.POINTER(C.c_double)), ("filter_weights", C.c_int32)]
 
 
 class GpPcgOpts(C.Structure):

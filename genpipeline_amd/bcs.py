@@ -77,10 +77,20 @@ def simp_bcs(bcs: dict, nx: int, ny: int, nz: int, force_mag: float):
     return f, fixed
 
 
+def _scalar_dof(ld):
+    """SIMPSolverGPU adds load_dof to a tensor of node ids; a list there
+    raises TypeError (simp_solver_gpu.py:246, 290, 309), so a list-valued
+    load_dof is rejected wherever the reference would apply it."""
+    if isinstance(ld, (list, tuple)):
+        raise TypeError("SIMPSolverGPU._get_bcs takes a single load_dof, got a list")
+    return int(ld)
+
+
 def simp_gpu_bcs(bcs: dict, nx: int, ny: int, nz: int, force_mag: float):
     """SIMPSolverGPU._get_bcs: one fixed face (unknown -> x_min), load spread
     evenly over the non-fixed nodes of the load face (unknown -> x_max), plus
-    the multi_load 'load_positions' and 'grid' patterns."""
+    the multi_load 'load_positions' and 'grid' patterns. A list load_dof
+    raises TypeError as in the reference."""
     i, j, k, nid = _nodes(nx, ny, nz)
     ff = bcs.get("fixed_face", "x_min")
     fixed_nodes = nid[_face(ff, i, j, k, nx, ny, nz, i == 0)]
@@ -101,7 +111,7 @@ def simp_gpu_bcs(bcs: dict, nx: int, ny: int, nz: int, force_mag: float):
             if ok.size:
                 share = float(force_mag) / len(positions)
                 for n in ok[:: max(1, ok.size // 3)]:
-                    f[3 * n + ld] -= share
+                    f[3 * n + _scalar_dof(ld)] -= share
         return f, fixed
     if multi and bcs.get("load_pattern", None) == "grid":
         step = max(1, nz // 4)
@@ -111,13 +121,13 @@ def simp_gpu_bcs(bcs: dict, nx: int, ny: int, nz: int, force_mag: float):
             if ok.size:
                 share = float(force_mag) / ((nz // step) + 1)
                 for n in ok[:: max(1, ok.size // 2)]:
-                    f[3 * n + ld] -= share
+                    f[3 * n + _scalar_dof(ld)] -= share
         return f, fixed
     if load_nodes.size:
         ok = load_nodes[~np.isin(load_nodes, fixed_nodes)]
         if ok.size == 0:
             ok = load_nodes
-        f[3 * ok + ld] = -float(force_mag) / ok.size
+        f[3 * ok + _scalar_dof(ld)] = -float(force_mag) / ok.size
     return f, fixed
 
 

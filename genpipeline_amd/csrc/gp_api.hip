@@ -297,6 +297,8 @@ int validate_spec(const gp_spec* s) {
   if (s->ke_kind == GP_KE_CUSTOM && s->ke == nullptr) return fail(GP_ERR_ARG, "custom KE is NULL");
   if (s->ke_kind != GP_KE_HEX8 && s->ke_kind != GP_KE_CUSTOM) return fail(GP_ERR_ARG, "bad ke_kind");
   if (!(s->rmin > 0) || s->rmin > 3.0) return fail(GP_ERR_ARG, "rmin must be in (0, 3]");
+  if (s->filter_weights != GP_FILTER_F64 && s->filter_weights != GP_FILTER_F32)
+    return fail(GP_ERR_ARG, "bad filter_weights");
   if (s->ke_kind == GP_KE_HEX8 && !(s->nu > -1.0 && s->nu < 0.5)) return fail(GP_ERR_ARG, "nu out of range");
   return GP_OK;
 }
@@ -316,7 +318,10 @@ Grid make_grid(const gp_spec* s) {
   return g;
 }
 
-void make_stencil(double rmin, Stencil* st) {
+// f32: each weight rounded once to float32 (SIMPSolverGPU._build_filter
+// fills its value arrays with np.float32, simp_solver_gpu.py:915). Zero
+// weights (|d| = rmin) are left out: they add nothing to a product or a sum.
+void make_stencil(double rmin, bool f32, Stencil* st) {
   const int r = (int)std::ceil(rmin);
   st->n = 0;
   for (int di = -r; di <= r; ++di)
@@ -324,7 +329,7 @@ void make_stencil(double rmin, Stencil* st) {
       for (int dk = -r; dk <= r; ++dk) {
         const double dist = std::sqrt((double)(di * di + dj * dj + dk * dk));
         if (dist > rmin) continue;
-        const double w = rmin - dist;
+        const double w = f32 ? (double)(float)(rmin - dist) : rmin - dist;
         if (!(w > 0.0)) continue;
         st->di[st->n] = di;
         st->dj[st->n] = dj;
@@ -743,7 +748,7 @@ int gp_problem_create(const gp_spec* spec, gp_problem** out) {
     std::memcpy(P->ke64_h, spec->ke, sizeof(double) * 576);
   float ke32[576];
   for (int i = 0; i < 576; ++i) ke32[i] = (float)P->ke64_h[i];
-  make_stencil(spec->rmin, &P->st_h);
+  make_stencil(spec->rmin, spec->filter_weights == GP_FILTER_F32, &P->st_h);
   const Grid& g = P->g;
   auto cleanup_fail = [&](int code) {
     gp_problem_destroy(P);
@@ -801,6 +806,17 @@ int gp_problem_create(const gp_spec* spec, gp_problem** out) {
   {
     ModalCoef<double> mc;
     P->modal_ok = modal_coefs(P->ke64_h, &mc);
+    // The float sweeps keep one Jacobi value per node, built from each
+    // corner's x-component diagonal (body_diag invert 2). That equals the
+    // per-DOF M^-1 only when the three float diagonal entries of every corner
+    // are equal, as they are for the isotropic hex8 element; a custom table
+    // that matches the modal form only to rounding may not satisfy it, and
+    // then keeps the gather kernels with M^-1 per DOF.
+    for (int c = 0; c < 8 && P->modal_ok; ++c) {
+      const float d0 = ke32[(3 * c) * 24 + 3 * c];
+      if (ke32[(3 * c + 1) * 24 + 3 * c + 1] != d0 || ke32[(3 * c + 2) * 24 + 3 * c + 2] != d0)
+        P->modal_ok = false;
+    }
     P->mc64 = mc;
     P->mc32 = {(float)mc.An, (float)mc.Ln, (float)mc.S, (float)mc.B1,
                (float)mc.B2, (float)mc.T1, (float)mc.T2, (float)mc.X};

@@ -18,6 +18,7 @@ from . import _lib as L
 
 LAWS = {"clamp": L.GP_LAW_CLAMP, "emin": L.GP_LAW_EMIN}
 OCS = {"cpu": L.GP_OC_CPU, "gpu": L.GP_OC_GPU}
+FILTER_WEIGHTS = {"f64": L.GP_FILTER_F64, "f32": L.GP_FILTER_F32}
 # "fp16" / "bf16": 16-bit PCG vector storage with float arithmetic, only as the
 # inner solve of FP64 iterative refinement (refine=True)
 PRECISIONS = {"fp32": L.GP_F32, "fp64": L.GP_F64, "fp16": L.GP_F16, "bf16": L.GP_BF16}
@@ -38,6 +39,9 @@ class ProblemSpec:
     nu: float = 0.3
     law: str = "emin"          # "emin": Emin + x^p (E0-Emin); "clamp": max(x,1e-3)^p
     oc: str = "cpu"            # OC constants of SIMPSolver ("cpu") or SIMPSolverGPU ("gpu")
+    # filter weights rmin - |d| in float64 (SIMPSolver) or rounded to float32
+    # (SIMPSolverGPU, simp_solver_gpu.py:915)
+    filter_weights: str = "f64"
 
     @property
     def n_elem(self) -> int:
@@ -58,6 +62,7 @@ class ProblemSpec:
         s.oc_variant = OCS[self.oc]
         s.E0, s.Emin, s.nu, s.penal, s.rmin, s.move = (self.E0, self.Emin, self.nu, self.penal,
                                                        self.rmin, self.move)
+        s.filter_weights = FILTER_WEIGHTS[self.filter_weights]
         keep = None
         if ke is None:
             s.ke_kind = L.GP_KE_HEX8

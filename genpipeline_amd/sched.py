@@ -135,6 +135,16 @@ class Scheduler:
              else torch.as_tensor(u0, dtype=torch.float64, device=dev).reshape(-1).clone())
         if x.numel() != sp.n_elem or xp.numel() != sp.n_elem or u.numel() != sp.n_dof:
             raise ValueError("state shapes do not match the problem")
+        # every argument check comes before the device slot is taken, so a
+        # rejected call leaves the slot free
+        f = None
+        if rhs is not None:
+            if n_iters != 0 or warm_start:
+                raise ValueError("rhs applies to single solves from zero (n_iters=0, "
+                                 "warm_start=False)")
+            f = torch.as_tensor(rhs, dtype=torch.float64, device=dev).reshape(-1).contiguous()
+            if f.numel() != sp.n_dof:
+                raise ValueError("rhs length does not match the problem")
         j = L.GpJob()
         j.n_iters = int(n_iters)
         j.volfrac = vf
@@ -148,17 +158,17 @@ class Scheduler:
             L.call("gp_sched_admit", self._h, int(slot), problem.handle, C.byref(j),
                    C.c_void_p(x.data_ptr()), C.c_void_p(xp.data_ptr()), C.c_void_p(u.data_ptr()),
                    _stream(dev))
-        f = None
-        if rhs is not None:
-            if n_iters != 0 or warm_start:
-                raise ValueError("rhs applies to single solves from zero (n_iters=0, "
-                                 "warm_start=False)")
-            f = torch.as_tensor(rhs, dtype=torch.float64, device=dev).reshape(-1).contiguous()
-            if f.numel() != sp.n_dof:
-                raise ValueError("rhs length does not match the problem")
-            with torch.cuda.device(dev):
-                L.call("gp_sched_set_rhs", self._h, int(slot), C.c_void_p(f.data_ptr()),
-                       _stream(dev))
+        if f is not None:
+            try:
+                with torch.cuda.device(dev):
+                    L.call("gp_sched_set_rhs", self._h, int(slot), C.c_void_p(f.data_ptr()),
+                           _stream(dev))
+            except Exception:
+                # the slot holds pointers to this call's tensors: free it
+                # before they go
+                with torch.cuda.device(dev):
+                    L.call("gp_sched_release", self._h, int(slot), _stream(dev))
+                raise
         job = Job(problem, x, xp, u, int(n_iters), vf, f)
         self.jobs[slot] = job
         return job

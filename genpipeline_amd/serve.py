@@ -81,8 +81,8 @@ class ServeConfig:
 def _spec(shape, kind) -> ProblemSpec:
     nx, ny, nz = shape
     if kind == "solve":
-        return ProblemSpec(nx, ny, nz, penal=1.0, law="clamp", oc="gpu")
-    return ProblemSpec(nx, ny, nz, penal=3.0, law="clamp", oc="gpu")
+        return ProblemSpec(nx, ny, nz, penal=1.0, law="clamp", oc="gpu", filter_weights="f32")
+    return ProblemSpec(nx, ny, nz, penal=3.0, law="clamp", oc="gpu", filter_weights="f32")
 
 
 def _flat64(density):

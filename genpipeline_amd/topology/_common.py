@@ -39,11 +39,12 @@ def _digest(*arrays) -> str:
     return h.hexdigest()
 
 
-def cached_problem(owner, law, oc, f, fixed, preserved, volfrac, penal=None, E0=1.0, Emin=1e-9):
+def cached_problem(owner, law, oc, f, fixed, preserved, volfrac, penal=None, E0=1.0, Emin=1e-9,
+                   filter_weights="f64"):
     """The owner's device problem for these BCs, created on first use and
     reused while grid, material, loads and supports stay the same."""
     penal = owner.penal if penal is None else penal
-    key = (law, oc, float(penal), float(owner.rmin), float(E0), float(Emin),
+    key = (law, oc, filter_weights, float(penal), float(owner.rmin), float(E0), float(Emin),
            _digest(np.asarray(f, np.float64), np.sort(np.asarray(fixed, np.int64)),
                    None if preserved is None else np.asarray(preserved, bool)))
     P = getattr(owner, "_problem", None)
@@ -52,7 +53,8 @@ def cached_problem(owner, law, oc, f, fixed, preserved, volfrac, penal=None, E0=
     if P is not None:
         P.close()
     spec = ProblemSpec(owner.nx, owner.ny, owner.nz, penal=float(penal), rmin=float(owner.rmin),
-                       volfrac=float(volfrac), E0=float(E0), Emin=float(Emin), law=law, oc=oc)
+                       volfrac=float(volfrac), E0=float(E0), Emin=float(Emin), law=law, oc=oc,
+                       filter_weights=filter_weights)
     P = Problem(spec, fixed, f, preserved=preserved, device=getattr(owner, "_torch_device", None))
     owner._problem = P
     owner._problem_key = key

@@ -9,6 +9,8 @@ Semantics kept from the reference:
     clamp (:769-791), filter adjoint (:796-799);
   * OC with l2 = 2 max(max(-dc), 1), <= 30 steps, relative gap 1e-5,
     floor 1e-15 (:801-841);
+  * filter weights rmin - |d| rounded to float32 as _build_filter stores
+    them whatever the dtype (:915), Hs their row sums;
   * loads of SIMPSolverGPU._get_bcs: spread evenly over the non-fixed
     load-face nodes, with the ``load_positions`` / ``grid`` patterns
     (:138-311);
@@ -70,7 +72,8 @@ class SIMPSolverGPU:
     def __init__(self, nx: int = 32, ny: int = 8, nz: int = 8, penal: float = 3.0,
                  rmin: float = 1.5, boundary_conditions: dict | None = None,
                  device: str = "cuda", preserved_mask: np.ndarray | None = None,
-                 dtype: torch.dtype = torch.float32, *, max_iter: int = 20000):
+                 dtype: torch.dtype = torch.float32, *, max_iter: int = 20000,
+                 pcg_tol: float | None = None):
         if not torch.cuda.is_available():
             raise RuntimeError("SIMPSolverGPU needs a ROCm GPU (torch.cuda.is_available() is False)")
         self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
@@ -81,6 +84,10 @@ class SIMPSolverGPU:
         self.dtype = dtype
         self.precision = "fp64" if dtype == torch.float64 else "fp32"
         self.max_iter = int(max_iter)
+        # None: run() follows the reference's tolerance schedule; a number:
+        # every solve of run() to that relative residual (no counterpart in
+        # the reference)
+        self.pcg_tol = None if pcg_tol is None else float(pcg_tol)
         self.Ke = torch.from_numpy(_lib.hex8_ke(0.3)).to(
             torch.float64 if dtype == torch.float64 else torch.float32)
         self.last_compliance = 0.0
@@ -118,7 +125,8 @@ class SIMPSolverGPU:
         f = f.detach().double().cpu().numpy() if torch.is_tensor(f) else np.asarray(f, np.float64)
         fixed = (fixed.detach().cpu().numpy() if torch.is_tensor(fixed)
                  else np.asarray(fixed)).astype(np.int64)
-        return cached_problem(self, "clamp", "gpu", f, fixed, self.preserved_mask, volfrac)
+        return cached_problem(self, "clamp", "gpu", f, fixed, self.preserved_mask, volfrac,
+                              filter_weights="f32")
 
     def _assemble_K(self, xPhys) -> StiffnessOperator:
         """Element moduli max(xPhys, 1e-3)^penal on the device, as an operator
@@ -157,9 +165,11 @@ class SIMPSolverGPU:
             x = np.full(n, float(volfrac))
         if self.preserved_mask is not None:
             x[self.preserved_mask] = 1.0
-        res = P.run(int(n_iters), x0=x, xphys0=x, volfrac=float(volfrac), tol=1e-6,
-                    max_iter=self.max_iter, precision=self.precision, warm_start=True,
-                    tol_schedule=True, early_stop=True, compliance_each_iter=True)
+        fixed_tol = self.pcg_tol is not None
+        res = P.run(int(n_iters), x0=x, xphys0=x, volfrac=float(volfrac),
+                    tol=self.pcg_tol if fixed_tol else 1e-6, max_iter=self.max_iter,
+                    precision=self.precision, warm_start=True, tol_schedule=not fixed_tol,
+                    early_stop=True, compliance_each_iter=True)
         self.last_result = res
         self._u_prev = res.u
         self.last_compliance = float(res.compliance)

@@ -54,6 +54,13 @@ extern "C" {
 #define GP_KE_HEX8 0          /* exact 2x2x2-Gauss trilinear hex (default)   */
 #define GP_KE_CUSTOM 1        /* caller-supplied 24x24 table (gp_spec.ke)    */
 
+#define GP_FILTER_F64 0       /* filter weights rmin - |d| in float64        */
+                              /* (SIMPSolver._build_filter, simp_solver.py:  */
+                              /* 1545-1567)                                  */
+#define GP_FILTER_F32 1       /* the same weights rounded once to float32,   */
+                              /* as SIMPSolverGPU._build_filter stores them  */
+                              /* whatever its dtype (simp_solver_gpu.py:915) */
+
 /* ---- problem specification ------------------------------------------ */
 typedef struct gp_spec {
   int32_t nx, ny, nz;         /* elements per axis                           */
@@ -62,6 +69,7 @@ typedef struct gp_spec {
   int32_t ke_kind;            /* GP_KE_*                                     */
   double E0, Emin, nu, penal, rmin, move;
   const double* ke;           /* host 24x24 row-major if GP_KE_CUSTOM        */
+  int32_t filter_weights;     /* GP_FILTER_*                                 */
 } gp_spec;
 
 typedef struct gp_problem gp_problem;   /* opaque */

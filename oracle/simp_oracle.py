@@ -28,14 +28,21 @@ How each function is pinned (tests/test_oracle.py):
                   invariant; also via the golden run trajectories.
   build_filter    closed-form 19-point weights, Hs brute force, H.1/Hs = 1.
   get_bcs_cpu     golden ``_get_bcs`` vectors for 14 BC dicts x 2 grids.
-  get_bcs_gpu     closed-form invariants (sum f = -force, node counts);
-                  parity unpinned beyond those (the reference GPU module is not
-                  runnable without CUDA, see simp_solver_gpu.py:677).
+  get_bcs_gpu     golden ``SIMPSolverGPU._get_bcs`` vectors (20 BC dicts x 4
+                  grids, float64 bit for bit, float32 to its rounding, and the
+                  dicts the reference rejects) + closed-form invariants.
   solve           direct vs PCG agreement; axial KAT.
   sensitivity     u=0 => dc=0 (tests/test_aggressive_kernel.py:106-125);
                   sum_e E_e ce_e = f.u compliance identity.
-  oc_update_*     golden run trajectories (cpu); invariants (gpu).
+  oc_update_*     golden run trajectories (cpu); golden ``SIMPSolverGPU.
+                  _oc_update`` vectors on seeded (x, dc) (gpu); invariants.
+  filter_dc       golden ``SIMPSolverGPU._build_filter`` H, Hs and
+                  ``_filter_dc`` vectors.
   run             golden run trajectories (ke="ref", solver="direct").
+  simp_gpu_run    golden ``SIMPSolverGPU.run`` runs with the reference's
+                  ``_solve`` replaced by a direct solve (the reference method
+                  is not runnable without CUDA, simp_solver_gpu.py:677), with
+                  the reference table and with the exact hex8 table.
 """
 from __future__ import annotations
 
@@ -119,9 +126,11 @@ def edof_matrix(nx: int, ny: int, nz: int) -> np.ndarray:
 
 
 # ------------------------------------------------------------------- filter
-def build_filter(nx: int, ny: int, nz: int, rmin: float):
+def build_filter(nx: int, ny: int, nz: int, rmin: float, weights: str = "f64"):
     """H[e1,e2] = rmin - dist for dist <= rmin inside the box r=ceil(rmin)
-    (simp_solver.py:1545-1567); Hs = row sums. Returns (H csr, Hs)."""
+    (simp_solver.py:1545-1567); Hs = row sums. Returns (H csr, Hs).
+    weights "f32": each weight rounded once to float32, as SIMPSolverGPU's
+    _build_filter fills its value arrays (simp_solver_gpu.py:915)."""
     r = int(np.ceil(rmin))
     i, j, k = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
     i, j, k = i.ravel(), j.ravel(), k.ravel()
@@ -134,6 +143,8 @@ def build_filter(nx: int, ny: int, nz: int, rmin: float):
                 if dist > rmin:
                     continue
                 w = rmin - dist
+                if weights == "f32":
+                    w = float(np.float32(w))
                 if w == 0.0:
                     continue  # lil_matrix does not store explicit zeros
                 i2, j2, k2 = i + di, j + dj, k + dk
@@ -201,6 +212,14 @@ def get_bcs_cpu(bcs: dict, nx: int, ny: int, nz: int, force_mag: float):
     return f, fixed
 
 
+def _gpu_dof(ld):
+    """The reference adds load_dof to a torch tensor of node ids
+    (simp_solver_gpu.py:246,290,309): a list there is a TypeError."""
+    if isinstance(ld, (list, tuple)):
+        raise TypeError("tensor + list")
+    return ld
+
+
 def get_bcs_gpu(bcs: dict, nx: int, ny: int, nz: int, force_mag: float):
     """Restates SIMPSolverGPU._get_bcs (simp_solver_gpu.py:138-311): the plain
     branch spreads the load evenly over all non-fixed load-face nodes."""
@@ -230,7 +249,7 @@ def get_bcs_gpu(bcs: dict, nx: int, ny: int, nz: int, force_mag: float):
             if len(v) > 0:
                 fpn = float(force_mag) / len(positions)
                 for n in v[:: max(1, len(v) // 3)]:
-                    f[3 * n + ld] -= fpn
+                    f[3 * n + _gpu_dof(ld)] -= fpn
         return f, fixed
     if multi and pattern == "grid":
         sp_ = max(1, nz // 4)
@@ -240,13 +259,13 @@ def get_bcs_gpu(bcs: dict, nx: int, ny: int, nz: int, force_mag: float):
             if len(v) > 0:
                 fpn = float(force_mag) / ((nz // sp_) + 1)
                 for n in v[:: max(1, len(v) // 2)]:
-                    f[3 * n + ld] -= fpn
+                    f[3 * n + _gpu_dof(ld)] -= fpn
         return f, fixed
     if len(load_nodes) > 0:
         v = load_nodes[~np.isin(load_nodes, fixed_nodes)]
         if len(v) == 0:
             v = load_nodes
-        f[3 * v + ld] = -float(force_mag) / len(v)
+        f[3 * v + _gpu_dof(ld)] = -float(force_mag) / len(v)
     return f, fixed
 
 
@@ -440,6 +459,12 @@ class OracleProblem:
     ke: str = "hex8"
     oc: str = "cpu"
     preserved: np.ndarray | None = None
+    # "cpu": element blocks are KE^T (the kron index vectors of
+    # simp_solver.py:280-281); "gpu": KE itself (repeat_interleave / repeat
+    # index vectors, simp_solver_gpu.py:66-67). Only an asymmetric table
+    # tells them apart.
+    assembly: str = "cpu"
+    filter_weights: str = "f64"   # "f32": SIMPSolverGPU's filter (build_filter)
     _cache: dict = field(default_factory=dict, repr=False)
 
     @property
@@ -454,14 +479,15 @@ class OracleProblem:
         if not self._cache:
             self._cache["KE"] = get_ke(self.ke)
             self._cache["edof"] = edof_matrix(self.nx, self.ny, self.nz)
-            self._cache["H"], self._cache["Hs"] = build_filter(self.nx, self.ny, self.nz, self.rmin)
+            self._cache["H"], self._cache["Hs"] = build_filter(self.nx, self.ny, self.nz, self.rmin,
+                                                               self.filter_weights)
         c = self._cache
         return c["KE"], c["edof"], c["H"], c["Hs"]
 
     def solve_u(self, xPhys, method="pcg", tol=1e-8, maxit=20000, x0=None):
         KE, edof, _, _ = self.setup()
         Ee = stiffness(xPhys, self.law, self.penal, self.E0, self.Emin)
-        K = assemble_K(KE, Ee, edof, self.n_dof)
+        K = assemble_K(KE if self.assembly == "cpu" else KE.T, Ee, edof, self.n_dof)
         return solve(K, self.f, self.fixed, method, tol, maxit, x0)
 
     def step(self, x, xPhys, method="pcg", tol=1e-8, x0=None):
@@ -497,11 +523,62 @@ class OracleProblem:
 
 def reference_problem(nx, ny, nz, bcs=None, force_mag=1.0, penal=3.0, rmin=1.5, volfrac=0.4,
                       preserved=None, ke="ref", law="clamp", oc="cpu", gpu_bcs=False):
-    """Problem with the reference SIMPSolver defaults (simp_solver.py:24-54)."""
+    """Problem with the reference SIMPSolver defaults (simp_solver.py:24-54);
+    gpu_bcs: SIMPSolverGPU's loads and assembly orientation."""
     b = {"fixed_face": "x_min", "load_face": "x_max", "load_dof": 2}
     if bcs:
         b.update(bcs)
     f, fixed = (get_bcs_gpu if gpu_bcs else get_bcs_cpu)(b, nx, ny, nz, force_mag)
     pm = None if preserved is None else np.asarray(preserved).ravel().astype(bool)
     return OracleProblem(nx, ny, nz, f, fixed, penal=penal, rmin=rmin, volfrac=volfrac,
-                         law=law, ke=ke, oc=oc, preserved=pm)
+                         law=law, ke=ke, oc=oc, preserved=pm,
+                         assembly="gpu" if gpu_bcs else "cpu",
+                         filter_weights="f32" if gpu_bcs else "f64")
+
+
+def gpu_tol_schedule(i):
+    """SIMPSolverGPU.run's per-iteration PCG tolerance (simp_solver_gpu.py:97-103)."""
+    return 1e-4 if i < 3 else (1e-5 if i < 8 else 1e-6)
+
+
+def simp_gpu_run(P, n_iters, volfrac, x_init=None, method="pcg", tol=None):
+    """SIMPSolverGPU.run (simp_solver_gpu.py:71-136) on the oracle.
+
+    x_init is rescaled to volfrac and clipped to [1e-3, 1] (:79-84);
+    preserved elements start at 1 and xPhys0 = x0 (:86-88). Each iteration:
+    a solve warm-started from the previous compliance solve (``_u_prev``,
+    :112 and :747-751 with :585-705), sensitivity, NaN abort (:106-108),
+    filter adjoint, GPU-constant OC, a compliance solve from zero, and the
+    early stop after three consecutive relative changes below 1% (:114-125).
+    ``tol`` None follows the tolerance schedule (``method`` "pcg"); a number
+    fixes every solve's tolerance. Returns (xPhys, last_compliance,
+    iterations done, compliance after each iteration)."""
+    KE, edof, H, Hs = P.setup()
+    if x_init is not None:
+        x = np.asarray(x_init, np.float64).ravel()
+        x = np.clip(x * (volfrac / max(x.mean(), 1e-6)), 1e-3, 1.0)
+    else:
+        x = np.full(P.n_elem, float(volfrac))
+    if P.preserved is not None:
+        x[P.preserved] = 1.0
+    xPhys = x.copy()
+    u_prev, prev, changes, comps, curr, done = None, np.inf, [], [], None, 0
+    for i in range(n_iters):
+        t = gpu_tol_schedule(i) if tol is None else tol
+        u, _ = P.solve_u(xPhys, method, t, x0=u_prev)
+        dc = sensitivity(KE, u, edof, xPhys, P.law, P.penal, P.E0, P.Emin)
+        if np.isnan(dc).any():
+            break
+        x, xPhys, _, _ = oc_update_gpu(x, filter_dc(H, Hs, dc), H, Hs, volfrac, P.move,
+                                       P.preserved)
+        done = i + 1
+        uc, _ = P.solve_u(xPhys, method, 1e-6 if tol is None else tol)
+        u_prev = uc
+        curr = float(P.f @ uc)
+        comps.append(curr)
+        if np.isfinite(prev):
+            changes.append(abs(prev - curr) / (prev + 1e-10))
+        if len(changes) >= 3 and all(c < 0.01 for c in changes[-3:]):
+            break
+        prev = curr
+    return xPhys, curr, done, comps

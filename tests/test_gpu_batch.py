@@ -64,3 +64,29 @@ def test_batch_matches_oracle(gpu):
     assert np.abs(res[0].xPhys.cpu().numpy() - xp).max() <= 1e-4
     P.close()
     P2.close()
+
+
+def test_batch_with_roller_supports_equals_alone(gpu):
+    """Supports that fix one component of a node (a roller on face y = 0
+    beside the clamp): the lockstep batch reads one Jacobi value per node as
+    the single-problem sweep does, gives bitwise Problem.run's results, and
+    every fixed DOF of u stays exactly 0."""
+    probs, vfs, fixeds = [], [], []
+    for i, (nx, ny, nz) in enumerate([(9, 7, 5), (12, 6, 6)]):
+        rng = np.random.default_rng(300 + i)
+        vf = float(rng.uniform(0.25, 0.4))
+        f = point_load_vector(nx, ny, nz, rng, 2)
+        roller = [3 * O.node_id(a, 0, k, ny, nz) + 2 for a in range(1, nx + 1) for k in range(nz + 1)]
+        fixed = np.union1d(clamp_x0(nx, ny, nz), np.asarray(roller, np.int64))
+        f[fixed] = 0.0
+        probs.append(Problem(ProblemSpec(nx, ny, nz, volfrac=vf), fixed, f, device=gpu))
+        vfs.append(vf)
+        fixeds.append(fixed)
+    with Batch(probs) as B:
+        res_b = B.run(4, volfracs=vfs, tol=1e-6)
+    for P, vf, fixed, rb in zip(probs, vfs, fixeds, res_b):
+        r = P.run(4, volfrac=vf, tol=1e-6, precision="fp32", warm_start=True)
+        assert r.compliance == rb.compliance and r.pcg_iters == rb.pcg_iters
+        assert torch.equal(r.xPhys, rb.xPhys) and torch.equal(r.u, rb.u)
+        assert torch.all(rb.u[torch.from_numpy(fixed).to(gpu)] == 0)
+        P.close()

@@ -86,6 +86,32 @@ def test_custom_table_uses_gather(gpu):
         assert P.ku_tile()[3] == 0 and P.ku_tile()[0] == 0
 
 
+def test_custom_table_with_unequal_float_diagonals_uses_gather(gpu):
+    """The float sweeps read one Jacobi value per node (the x diagonal). A
+    custom table that matches the modal form to 1e-10 but whose float32
+    diagonal entries of one corner differ keeps the per-DOF gather path, and
+    still solves to the direct answer."""
+    spec, fixed, f, x, rng = _problem(6, 3, 4, seed=4)
+    KE = O.ke_hex8(0.3)
+    # scale so the diagonal sits just below a float32 rounding midpoint, then
+    # push one entry across it by 3e-11 relative
+    s = (1.0 + 2.0 ** -24 - 1e-11) / KE[0, 0]
+    base = KE * s
+    bent = base.copy()
+    bent[4, 4] *= 1.0 + 3e-11
+    assert np.float32(base[4, 4]) != np.float32(bent[4, 4])
+    with Problem(spec, fixed, f, ke=base, device=gpu) as P:
+        assert P.ku_tile()[3] == 1
+    E = O.stiffness(x, "emin", 3.0).astype(np.float32)
+    K = O.assemble_K(bent.T.copy(), E.astype(np.float64), O.edof_matrix(6, 3, 4), spec.n_dof)
+    ud, _ = O.solve(K, f, fixed, "direct")
+    with Problem(spec, fixed, f, ke=bent, device=gpu) as P:
+        assert P.ku_tile()[3] == 0 and P.ku_tile()[0] == 0
+        u, st = P.pcg(torch.from_numpy(E), tol=1e-10, refine=True)
+    assert st.status == 0
+    assert np.abs(u.cpu().numpy() - ud).max() <= 1e-8 * np.abs(ud).max()
+
+
 def test_ku_custom_table(gpu):
     """A caller-supplied (asymmetric) table is applied as y_e = KE u_e."""
     grid = (4, 3, 2)

@@ -132,11 +132,21 @@ def test_sched_rhs_replaces_the_load(gpu):
     assert out[0][1].status == 0 and out[0][1].pcg_iters == out[1][1].pcg_iters
     assert out[0][1].compliance == out[1][1].compliance
     assert torch.equal(out[0][0], out[1][0])
-    with pytest.raises(ValueError):
-        S = Scheduler([(*shape, 8)], tile=(14, 18), device=gpu)
-        try:
+    # a rejected admission leaves the slot free: the same slot then takes a
+    # job and runs it to the same result
+    S = Scheduler([(*shape, 8)], tile=(14, 18), device=gpu)
+    try:
+        with pytest.raises(ValueError):
             S.admit(0, P, n_iters=5, rhs=torch.as_tensor(g, device=gpu), warm_start=False)
-        finally:
-            S.close()
+        with pytest.raises(ValueError):
+            S.admit(0, P, n_iters=0, rhs=torch.as_tensor(g[:-3], device=gpu), warm_start=False)
+        assert S.jobs[0] is None
+        S.admit(0, Pg, n_iters=0, x0=dens, xphys0=dens, tol=1e-6, warm_start=False)
+        _, st = S.run()
+        assert st[0].status == 0 and st[0].compliance == out[1][1].compliance
+        assert torch.equal(S.jobs[0].u, out[1][0])
+        torch.cuda.synchronize()
+    finally:
+        S.close()
     for p in (P, Q, Pg):
         p.close()

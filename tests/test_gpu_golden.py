@@ -1,0 +1,97 @@
+"""The HIP path against golden vectors of the reference GPU solver
+(tests/golden/make_golden_gpu.py imported SIMPSolverGPU on CPU torch):
+
+* the density filter and its adjoint with SIMPSolverGPU's float32-rounded
+  weights (simp_solver_gpu.py:869-930, 796-799);
+* the OC update with the GPU constants (:801-841), preserved elements kept;
+* ``SIMPSolverGPU.run`` (:71-136) through the drop-in facade in FP64 with
+  every solve tight (the golden runs solve directly), for the runs that use
+  the exact hex8 table: the iteration count (early stop included), the
+  compliance after every iteration and the final density.
+"""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+import torch
+
+from genpipeline_amd import Problem, ProblemSpec
+from genpipeline_amd.bcs import clamp_x0
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = Path(__file__).resolve().parent / "golden" / "ref_simp_gpu_golden.npz"
+
+
+@pytest.fixture(scope="module")
+def gold():
+    meta = json.loads(GOLDEN.with_suffix(".json").read_text())
+    with np.load(GOLDEN, allow_pickle=False) as d:
+        arrays = {k: np.array(d[k]) for k in d.files}
+    return arrays, meta
+
+
+def _pres(c):
+    if not c.get("preserved"):
+        return None
+    m = np.zeros((c["nx"], c["ny"], c["nz"]), bool)
+    m[c["nx"] - 2:, :, :2] = True
+    return m.ravel()
+
+
+def _problem(c, gpu, pres=None):
+    spec = ProblemSpec(c["nx"], c["ny"], c["nz"], rmin=c["rmin"], law="clamp", oc="gpu",
+                       filter_weights="f32", volfrac=c.get("volfrac", 0.3))
+    f = np.zeros(spec.n_dof)
+    f[-1] = -1.0
+    return Problem(spec, clamp_x0(c["nx"], c["ny"], c["nz"]), f, preserved=pres, device=gpu)
+
+
+def test_filter_adjoint_matches_golden(gpu, gold):
+    arrays, meta = gold
+    for tag, c in meta["filter"].items():
+        print("[golden filter]", tag, flush=True)
+        with _problem(c, gpu) as P:
+            out = P.filter(torch.from_numpy(arrays[tag + "/dc_in"]), mode=1).cpu().numpy()
+            ones = P.filter(torch.ones(P.spec.n_elem, dtype=torch.float64), mode=0).cpu().numpy()
+        assert np.allclose(out, arrays[tag + "/dc_out"], rtol=1e-14, atol=1e-15), tag
+        assert np.abs(ones - 1.0).max() <= 4e-16, tag
+
+
+def test_oc_gpu_constants_match_golden(gpu, gold):
+    arrays, meta = gold
+    for tag, c in meta["oc"].items():
+        print("[golden oc]", tag, flush=True)
+        pres = _pres(c)
+        with _problem(c, gpu, pres) as P:
+            xn, xp, _, _ = P.oc(torch.from_numpy(arrays[tag + "/x"]),
+                                torch.from_numpy(arrays[tag + "/dc"]), volfrac=c["volfrac"])
+            xn, xp = xn.cpu().numpy(), xp.cpu().numpy()
+        assert np.abs(xn - arrays[tag + "/xnew"]).max() <= 1e-12, tag
+        assert np.abs(xp - arrays[tag + "/xphys"]).max() <= 1e-12, tag
+        if pres is not None:
+            assert np.all(xn[pres] == 1.0) and np.all(xp[pres] == 1.0)
+
+
+def test_simp_solver_gpu_driver_matches_golden_runs(gpu, gold):
+    from genpipeline_amd.topology import SIMPSolverGPU
+    arrays, meta = gold
+    n = 0
+    for name, r in meta["runs"].items():
+        if r["table"] != "hex8":
+            continue
+        print("[golden run]", name, flush=True)
+        pres = _pres(r)
+        s = SIMPSolverGPU(r["nx"], r["ny"], r["nz"], boundary_conditions=r["bcs"],
+                          preserved_mask=None if pres is None else pres.reshape(r["nx"], r["ny"], r["nz"]),
+                          dtype=torch.float64, pcg_tol=1e-12)
+        xi = arrays.get(f"run/{name}/x_init")
+        xp = s.run(volfrac=r["volfrac"], n_iters=r["n_iters"], force_mag=r["force"], x_init=xi)
+        assert s.last_result.iters_done == r["iters_done"], name
+        gc = float(arrays[f"run/{name}/compliance"])
+        assert abs(s.last_compliance - gc) <= 1e-8 * abs(gc), (name, s.last_compliance, gc)
+        err = np.abs(xp.ravel() - arrays[f"run/{name}/xPhys"].ravel()).max()
+        assert err <= 1e-7, (name, err)
+        n += 1
+    assert n == 6

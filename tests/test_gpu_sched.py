@@ -189,3 +189,28 @@ def test_compacted_launches_each_equals_alone(gpu):
     finally:
         torch.cuda.synchronize()
         S.close()
+
+
+def test_roller_supports_job_equals_alone(gpu):
+    """A roller (one fixed component per node on face y = 0) beside the
+    clamp: the scheduled job equals Problem.run bit for bit, and every fixed
+    DOF of u stays exactly 0."""
+    from genpipeline_amd.sched import Scheduler
+    nx, ny, nz = 9, 7, 5
+    rng = np.random.default_rng(41)
+    f = point_load_vector(nx, ny, nz, rng, 2)
+    roller = [3 * O.node_id(a, 0, k, ny, nz) + 2 for a in range(1, nx + 1) for k in range(nz + 1)]
+    fixed = np.union1d(clamp_x0(nx, ny, nz), np.asarray(roller, np.int64))
+    f[fixed] = 0.0
+    P = Problem(ProblemSpec(nx, ny, nz, volfrac=0.3), fixed, f, device=gpu)
+    for tile in ((14, 18), (8, 32)):
+        ref = _alone(P, tile, 2, 4, tol=1e-6, max_iter=5000, warm_start=True)
+        S = Scheduler([(nx, ny, nz, 2)], tile=tile)
+        S.admit(0, P, n_iters=4, tol=1e-6, max_iter=5000, warm_start=True)
+        _, st = S.run()
+        res = S.result(0, st[0])
+        _same(res, ref)
+        assert torch.all(res.u[torch.from_numpy(fixed).to(gpu)] == 0)
+        torch.cuda.synchronize()
+        S.close()
+    P.close()
