@@ -618,33 +618,33 @@ int pcg_refine(gp_problem* P, const float* E, const double* f, double* u, const 
   rc = lp ? ensure_lp_ws(P) : ensure_pcg_ws(P, GP_F32);
   if (rc) return rc;
   DevScalars* sc2 = &P->sc[2];
-  // ||f_free||^2
-  HIPCHK(hipMemsetAsync(P->y64, 0, g.ndof * sizeof(double), s));
-  launch_residual64(g, f, P->y64, P->fixmask, P->r64, P->parts, sc2, s);
-  LAUNCHCHK();
-  rc = sync_read(P, 2, s);
-  if (rc) return rc;
-  const double bb = P->h_sc[2].sum;
   if (!o->use_x0) HIPCHK(hipMemsetAsync(u, 0, g.ndof * sizeof(double), s));
   launch_mask_fixed<double>(g, u, P->fixmask, s);
   st->status = GP_OK;
-  if (bb == 0.0) {
-    HIPCHK(hipMemsetAsync(u, 0, g.ndof * sizeof(double), s));
-    st->relres = 0.0;
-    return sync_read(P, 2, s);
-  }
-  const double tol2 = o->tol * o->tol * bb;
   const int max_outer = o->max_outer > 0 ? o->max_outer : 50;
-  double rr = 0.0;
+  double rr = 0.0, bb = 0.0;
+  bool brk = false;
+  // The same sequence as a scheduler job with refinement (PH_RES, PH_RESFIN,
+  // inner solve, PH_CORR): every sweep forms the FP64 residual with both
+  // sums, then stops on convergence, a broken-down inner solve or the cap.
   for (int k = 0;; ++k) {
-    ku_dispatch<double>(P, E, u, P->y64, true, false, s);
-    launch_residual64(g, f, P->y64, P->fixmask, P->r64, P->parts, sc2, s);
+    launch_res64(g, P->ke64, E, u, f, P->fixmask, P->r64, P->parts, sc2, s);
     LAUNCHCHK();
     rc = sync_read(P, 2, s);
     if (rc) return rc;
-    rr = P->h_sc[2].sum;
+    rr = P->h_sc[2].ir_rr;
+    bb = P->h_sc[2].ir_bb;
     st->outer = k;
-    if (rr <= tol2 || !std::isfinite(rr)) break;
+    if (bb == 0.0) {
+      HIPCHK(hipMemsetAsync(u, 0, g.ndof * sizeof(double), s));
+      st->relres = 0.0;
+      return sync_read(P, 2, s);
+    }
+    if (rr <= o->tol * o->tol * bb || !std::isfinite(rr)) break;
+    if (brk) {
+      st->status = GP_BREAKDOWN;
+      break;
+    }
     if (k >= max_outer) {
       st->status = GP_NOT_CONVERGED;
       break;
@@ -670,16 +670,9 @@ int pcg_refine(gp_problem* P, const float* E, const double* f, double* u, const 
     }
     if (rc) return rc;
     st->iters += in.iters;
-    if (in.status == GP_BREAKDOWN) {
-      st->status = GP_BREAKDOWN;
-      // one more residual for the report
-      ku_dispatch<double>(P, E, u, P->y64, true, false, s);
-      launch_residual64(g, f, P->y64, P->fixmask, P->r64, P->parts, sc2, s);
-      rc = sync_read(P, 2, s);
-      if (rc) return rc;
-      rr = P->h_sc[2].sum;
-      break;
-    }
+    // a broken-down inner solve still adds its correction; one more
+    // residual is formed for the report
+    if (in.status == GP_BREAKDOWN) brk = true;
   }
   st->relres = std::sqrt(rr / bb);
   if (!std::isfinite(rr)) st->status = GP_ERR_NAN;
@@ -1455,6 +1448,7 @@ int fill_batch_prob(gp_problem* P, int tj, int tk, int ni, BatchProb& q) {
   q.dcf = P->dcf;
   q.E = P->E;
   q.u = P->d32;
+  q.r64 = P->r64;
   q.rb[0] = base;
   q.rb[1] = base + nd;
   q.wb[0] = base + 2 * nd;
@@ -1910,6 +1904,9 @@ int gp_sched_admit(gp_sched* Q, int32_t slot, gp_problem* P, const gp_job* job, 
   if (P->device != Q->device) return fail(GP_ERR_ARG, "problem lives on another device");
   if (job->n_iters < 0 || job->max_iter < 1 || !(job->tol >= 0))
     return fail(GP_ERR_ARG, "bad job parameters");
+  if (job->refine != 0 && job->refine != 1) return fail(GP_ERR_ARG, "refine must be 0 or 1");
+  if (job->refine && (!(job->inner_tol >= 0) || job->max_outer < 0))
+    return fail(GP_ERR_ARG, "bad refinement parameters");
   HIPCHK(hipSetDevice(Q->device));
   SchedSlot& q = Q->h_slots[slot];
   const int ni = Q->dims[4 * slot + 3];
@@ -1938,8 +1935,13 @@ int gp_sched_admit(gp_sched* Q, int32_t slot, gp_problem* P, const gp_job* job, 
   q.cfg.early_stop = job->early_stop;
   q.cfg.warm_start = job->warm_start;
   q.cfg.tol = job->tol;
+  // the defaults of pcg_refine for a zero inner tolerance or sweep cap
+  q.cfg.refine = job->refine;
+  q.cfg.inner_tol = job->inner_tol > 0 ? job->inner_tol : 1e-3;
+  q.cfg.max_outer = job->max_outer > 0 ? job->max_outer : 50;
   SchedState s0 = idle_state();
-  s0.phase = job->warm_start ? PH_CAST_IN : PH_STIFF;
+  // an FP64-refined job works on the caller's u directly: no float copy in
+  s0.phase = job->warm_start && !job->refine ? PH_CAST_IN : PH_STIFF;
   s0.solve = job->n_iters > 0 ? SK_OC : SK_FINAL;
   q.st[0] = s0;
   q.st[1] = s0;

@@ -167,6 +167,77 @@ __device__ inline void body_diag(const Grid& g, const T* __restrict__ ke,
   }
 }
 
+// FP64 refinement residual r = f - K(E) u on the free DOFs (0 on a fixed
+// one). One thread per node sums, over the eight elements around it, E_e
+// times rows 3L..3L+2 of the FP64 table (L: the node's corner in that
+// element) applied to the element's corner values, in a fixed order. The
+// table addresses are uniform across a wave, so they are scalar loads; an
+// element outside the grid reads corner node 0 with modulus 0. Block sums of
+// r.r and of the free part of f.f go to partR[lb], partB[lb]. Runs once per
+// refinement sweep, not per PCG iteration.
+__device__ inline void body_res64(const Grid& g, const double* __restrict__ ke,
+                                  const float* __restrict__ E, const double* __restrict__ u,
+                                  const double* __restrict__ f,
+                                  const uint8_t* __restrict__ fixmask, double* __restrict__ r,
+                                  double* partR, double* partB, int lb, int nb) {
+  double rr = 0.0, bb = 0.0;
+  for (int n = lb * kBlock + threadIdx.x; n < g.nn; n += nb * kBlock) {
+    const int i = n / g.sI, rem = n - i * g.sI, j = rem / g.sJ, k = rem - j * g.sJ;
+    double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+#pragma unroll 1
+    for (int abc = 0; abc < 8; ++abc) {  // the eight elements around the node
+          const int a = abc >> 2, b = (abc >> 1) & 1, c = abc & 1;
+          const int ei = i - 1 + a, ej = j - 1 + b, ek = k - 1 + c;
+          const bool ok = (unsigned)ei < (unsigned)g.nx && (unsigned)ej < (unsigned)g.ny &&
+                          (unsigned)ek < (unsigned)g.nz;
+          const int e = ok ? ei * g.eI + ej * g.eJ + ek : 0;
+          const int n0 = ok ? ei * g.sI + ej * g.sJ + ek : 0;
+          const double Ee = ok ? (double)E[e] : 0.0;
+          const int L = LOC[(1 - a) * 4 + (1 - b) * 2 + (1 - c)];
+          double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+#pragma unroll
+          for (int m = 0; m < 8; ++m) {
+            const V3<double> v = ld3(u, n0 + CX[m] * g.sI + CY[m] * g.sJ + CZ[m]);
+            const double* r0 = ke + (3 * L + 0) * 24 + 3 * m;
+            const double* r1 = ke + (3 * L + 1) * 24 + 3 * m;
+            const double* r2 = ke + (3 * L + 2) * 24 + 3 * m;
+            s0 += r0[0] * v.x + r0[1] * v.y + r0[2] * v.z;
+            s1 += r1[0] * v.x + r1[1] * v.y + r1[2] * v.z;
+            s2 += r2[0] * v.x + r2[1] * v.y + r2[2] * v.z;
+          }
+          y0 += Ee * s0;
+          y1 += Ee * s1;
+          y2 += Ee * s2;
+        }
+    const uint8_t fm = fixmask[n];
+    const double f0 = (fm & 1) ? 0.0 : f[3ll * n];
+    const double f1 = (fm & 2) ? 0.0 : f[3ll * n + 1];
+    const double f2 = (fm & 4) ? 0.0 : f[3ll * n + 2];
+    const double r0 = (fm & 1) ? 0.0 : f0 - y0;
+    const double r1 = (fm & 2) ? 0.0 : f1 - y1;
+    const double r2 = (fm & 4) ? 0.0 : f2 - y2;
+    st3(r, n, r0, r1, r2);
+    rr += r0 * r0 + r1 * r1 + r2 * r2;
+    bb += f0 * f0 + f1 * f1 + f2 * f2;
+  }
+  rr = block_sum0(rr);
+  bb = block_sum0(bb);
+  if (threadIdx.x == 0) {
+    partR[lb] = rr;
+    partB[lb] = bb;
+  }
+}
+
+// One block: the refinement residual's sums (partials of body_res64).
+__device__ inline void body_res_fin(const Parts& parts, int nV, DevScalars* sc) {
+  const double rr = reduce_all(parts.A, nV);
+  const double bb = reduce_all(parts.RZ[1], nV);
+  if (threadIdx.x == 0) {
+    sc->ir_rr = rr;
+    sc->ir_bb = bb;
+  }
+}
+
 // Single-kernel PCG set-up: r_{-1} = b - A x0 on free DOFs (or b), x masked,
 // s_{-2} = 0, w_{-1} = 0, p = 0; partials bb -> BB[lb], rr -> RR[lb].
 // S: storage of x and p (T for FP32/FP64; a 16-bit type with T = float).

@@ -70,6 +70,7 @@ struct DevScalars {
   double volfrac, move, oc_eps, oc_tol, oc_floor;
   double sum;                // one-shot reduction result
   double cgg[2], cga[2];     // single-kernel PCG: gamma and alpha by parity
+  double ir_rr, ir_bb;       // FP64 refinement: |r|^2 and |f_free|^2 of the last residual
   int iter, max_iter, done, status;
   int oc_steps, oc_max, oc_done, nan_flag;
 };
@@ -92,8 +93,9 @@ struct BatchProb {
   const double* f;
   double *x, *xPhys, *xnew, *xphys_new, *dc, *dcf;
   float* E;
-  float* u;                     // float32 PCG iterate
+  float* u;                     // float32 PCG iterate (FP64-refined jobs: the correction)
   double* u64;                  // caller's float64 state
+  double* r64;                  // FP64 refinement residual
   float *rb[2], *wb[2], *sb[2], *pc, *minv;
   DevScalars* sc;               // PCG scalars
   DevScalars* sc_oc;            // OC / one-shot scalars
@@ -149,19 +151,26 @@ enum SchedPhase : int {
   PH_DOT,        // f.u partials
   PH_SUMFIN,     // compliance; early-stop test
   PH_CAST_OUT,   // u64 <- (double) u32
-  PH_DONE
+  PH_DONE,       // (18)
+  // FP64 iterative refinement around FP32 inner solves (jobs with refine)
+  PH_Z64,        // u64 <- 0
+  PH_RES,        // r64 = f - K u64 on free DOFs (FP64), |r|^2 and |f|^2 partials
+  PH_RESFIN,     // residual sums; converged, capped, broken down, or an inner solve
+  PH_CORR        // u64 += correction
 };
 enum SolveKind : int { SK_OC = 0, SK_COMPL = 1, SK_FINAL = 2 };
 
 struct SchedCfg {
   int n_iters, max_iter, tol_schedule, compl_each, early_stop, warm_start;
   double tol;
+  int refine, max_outer;  // FP64 refinement (outer tolerance: tol, as scheduled)
+  double inner_tol;
 };
 
 struct SchedState {
   int phase, k, oc_it, sub;
   int solve, small, status, pcg_total;
-  int bis_total, nan, pad0, pad1;
+  int bis_total, nan, outer, ir_brk;  // refinement sweep; inner breakdown seen
   double prev_c, curr_c;
 };
 
@@ -321,6 +330,11 @@ void launch_pcg_c(const Grid& g, int it, T* p, const T* z, Parts parts, DevScala
 // FP64 refinement helpers
 void launch_residual64(const Grid& g, const double* f, const double* Ku, const uint8_t* fixmask,
                        double* r, Parts parts, DevScalars* sc, hipStream_t st);
+// r = f - K(E) u on the free DOFs in FP64 (24x24 table ke), sc->ir_rr = |r|^2,
+// sc->ir_bb = |f_free|^2; vec_blocks(g) partial sums, fixed order.
+void launch_res64(const Grid& g, const double* ke, const float* E, const double* u,
+                  const double* f, const uint8_t* fixmask, double* r, Parts parts, DevScalars* sc,
+                  hipStream_t st);
 template <typename T>
 void launch_add_correction(const Grid& g, double* u, const T* d, hipStream_t st);
 // u += scale * d for 16-bit stored corrections (S = _Float16 or bf16)

@@ -327,6 +327,25 @@ void launch_residual64(const Grid& g, const double* f, const double* Ku, const u
   hipLaunchKernelGGL(k_sum_fin, dim3(1), dim3(kBlock), 0, st, parts.S, nb, &sc->sum);
 }
 
+__global__ __launch_bounds__(kBlock) void k_res64(Grid g, const double* __restrict__ ke,
+                                                 const float* __restrict__ E,
+                                                 const double* __restrict__ u,
+                                                 const double* __restrict__ f,
+                                                 const uint8_t* __restrict__ fixmask,
+                                                 double* __restrict__ r, Parts parts) {
+  body_res64(g, ke, E, u, f, fixmask, r, parts.A, parts.RZ[1], blockIdx.x, gridDim.x);
+}
+__global__ __launch_bounds__(kBlock) void k_res_fin(Parts parts, int nV, DevScalars* sc) {
+  body_res_fin(parts, nV, sc);
+}
+void launch_res64(const Grid& g, const double* ke, const float* E, const double* u,
+                  const double* f, const uint8_t* fixmask, double* r, Parts parts, DevScalars* sc,
+                  hipStream_t st) {
+  const int nb = vec_blocks(g);
+  hipLaunchKernelGGL(k_res64, dim3(nb), dim3(kBlock), 0, st, g, ke, E, u, f, fixmask, r, parts);
+  hipLaunchKernelGGL(k_res_fin, dim3(1), dim3(kBlock), 0, st, parts, nb, sc);
+}
+
 template <typename T>
 __global__ void k_add_correction(long long n, double* __restrict__ u, const T* __restrict__ d) {
   const long long i = blockIdx.x * (long long)kBlock + threadIdx.x;

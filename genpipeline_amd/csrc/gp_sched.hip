@@ -33,6 +33,17 @@ __device__ inline bool sched_use_x0(const SchedState& s, const SchedCfg& c) {
   return c.warm_start != 0 && s.solve != SK_COMPL;
 }
 
+// Last phase of a job: FP32 jobs copy their float iterate out, FP64-refined
+// jobs already hold the caller's u.
+__device__ inline int sched_end_phase(const SchedCfg& c) {
+  return c.refine ? PH_DONE : PH_CAST_OUT;
+}
+
+// First phase after a finished solve.
+__device__ inline int sched_post_solve(const SchedState& s) {
+  return s.solve == SK_OC ? PH_SENS : PH_DOT;
+}
+
 // State after a finished OC update (iteration count oc_it already advanced).
 __device__ inline void sched_after_update(SchedState& nx, const SchedCfg& c) {
   nx.phase = PH_STIFF;
@@ -55,7 +66,50 @@ __device__ inline SchedState sched_next(const SchedState& cur, const SchedCfg& C
       nx.phase = PH_DIAG;
       break;
     case PH_DIAG:
-      nx.phase = sched_use_x0(cur, C) ? PH_KX0 : PH_CGINIT;
+      if (C.refine) {
+        // pcg_refine: a warm start keeps u (its fixed DOFs cleared in this
+        // phase), otherwise u = 0; then the first residual
+        nx.phase = sched_use_x0(cur, C) ? PH_RES : PH_Z64;
+        nx.outer = 0;
+        nx.ir_brk = 0;
+      } else {
+        nx.phase = sched_use_x0(cur, C) ? PH_KX0 : PH_CGINIT;
+      }
+      break;
+    case PH_Z64:
+      // outer < 0: f has no free part, the solve ends with u = 0
+      nx.phase = cur.outer < 0 ? sched_post_solve(cur) : PH_RES;
+      break;
+    case PH_RES:
+      nx.phase = PH_RESFIN;
+      break;
+    case PH_RESFIN: {
+      // pcg_refine's decisions, in its order
+      const double rr = P.sc->ir_rr, bb = P.sc->ir_bb;
+      const double tol = sched_tol(cur, C);
+      int st = -1;
+      if (bb == 0.0) {
+        nx.phase = PH_Z64;
+        nx.outer = -1;
+        break;
+      }
+      if (rr <= tol * tol * bb || !isfinite(rr))
+        st = isfinite(rr) ? GP_OK : GP_ERR_NAN;
+      else if (cur.ir_brk)
+        st = GP_BREAKDOWN;
+      else if (cur.outer >= C.max_outer)
+        st = GP_NOT_CONVERGED;
+      if (st >= 0) {
+        if (st > nx.status) nx.status = st;
+        nx.phase = sched_post_solve(cur);
+      } else {
+        nx.phase = PH_CGINIT;  // inner solve K d = r64 from d = 0
+        nx.outer = cur.outer + 1;
+      }
+      break;
+    }
+    case PH_CORR:
+      nx.phase = PH_RES;
       break;
     case PH_KX0:
       nx.phase = PH_CGINIT;
@@ -70,8 +124,15 @@ __device__ inline SchedState sched_next(const SchedState& cur, const SchedCfg& C
     case PH_CG:
       if (fin) {
         nx.pcg_total += P.sc->iter;
-        if (P.sc->status > nx.status) nx.status = P.sc->status;
-        nx.phase = cur.solve == SK_OC ? PH_SENS : PH_DOT;
+        if (C.refine) {
+          // an inner solve: its correction is added whatever its status; a
+          // breakdown ends the refinement after one more residual
+          if (P.sc->status == GP_BREAKDOWN) nx.ir_brk = 1;
+          nx.phase = PH_CORR;
+        } else {
+          if (P.sc->status > nx.status) nx.status = P.sc->status;
+          nx.phase = sched_post_solve(cur);
+        }
       } else {
         nx.k = cur.k + 1;
       }
@@ -85,7 +146,7 @@ __device__ inline SchedState sched_next(const SchedState& cur, const SchedCfg& C
         nx.status = GP_ERR_NAN;
         nx.nan = 1;
         if (C.compl_each && cur.oc_it > 0) {
-          nx.phase = PH_CAST_OUT;
+          nx.phase = sched_end_phase(C);
         } else {
           nx.phase = PH_STIFF;
           nx.solve = SK_FINAL;
@@ -134,13 +195,13 @@ __device__ inline SchedState sched_next(const SchedState& cur, const SchedCfg& C
           if (!stop) nx.prev_c = c;
         }
         if (stop || cur.oc_it >= C.n_iters) {
-          nx.phase = PH_CAST_OUT;
+          nx.phase = sched_end_phase(C);
         } else {
           nx.phase = PH_STIFF;
           nx.solve = SK_OC;
         }
       } else {
-        nx.phase = PH_CAST_OUT;
+        nx.phase = sched_end_phase(C);
       }
       break;
     }
@@ -182,22 +243,51 @@ __global__ __launch_bounds__(kBlock, 4) void k_tick(SchedTable tab, int t) {
       break;
     case PH_DIAG:
       body_diag<float>(P.g, P.ke32, P.E, P.fixmask, P.minv, 2, lb, nb);  // per node
+      if (S.cfg.refine && sched_use_x0(cur, S.cfg))
+        for (int n = lb * kBlock + threadIdx.x; n < P.g.nn; n += nb * kBlock) {
+          const uint8_t fm = P.fixmask[n];
+          if (fm & 1) P.u64[3ll * n] = 0.0;
+          if (fm & 2) P.u64[3ll * n + 1] = 0.0;
+          if (fm & 4) P.u64[3ll * n + 2] = 0.0;
+        }
+      break;
+    case PH_Z64:
+      for (long long d = lb * (long long)kBlock + threadIdx.x; d < P.g.ndof;
+           d += (long long)nb * kBlock)
+        P.u64[d] = 0.0;
+      break;
+    case PH_RES:
+      for (int v = lb; v < S.nb_vec; v += nb)
+        body_res64(P.g, P.ke64, P.E, P.u64, P.f, P.fixmask, P.r64, P.parts.A, P.parts.RZ[1], v,
+                   S.nb_vec);
+      break;
+    case PH_RESFIN:
+      if (lb == 0) body_res_fin(P.parts, S.nb_vec, P.sc);
+      break;
+    case PH_CORR:
+      for (long long d = lb * (long long)kBlock + threadIdx.x; d < P.g.ndof;
+           d += (long long)nb * kBlock)
+        P.u64[d] += (double)P.u[d];
       break;
     case PH_KX0:
       body_ku_sweep<float, TJ, TK, kBlock>(P.g, P.ni, P.ntj, P.ntk, P.mc, P.E, P.u, P.wb[0],
                                            P.fixmask, lb, cv);
       break;
     case PH_CGINIT: {
-      const int x0 = sched_use_x0(cur, S.cfg) ? 1 : 0;
+      // refined jobs: the inner solve of K d = r64 from d = 0
+      const bool ref = S.cfg.refine != 0;
+      const int x0 = !ref && sched_use_x0(cur, S.cfg) ? 1 : 0;
       for (int v = lb; v < S.nb_vec; v += nb)
-        body_cg_init<float>(P.g.nn, P.f, P.wb[0], x0, P.u, P.rb[1], P.sb[0], P.wb[1], P.pc,
-                            P.minv, P.fixmask, 1, P.parts, v, S.nb_vec);
+        body_cg_init<float>(P.g.nn, ref ? P.r64 : P.f, P.wb[0], x0, P.u, P.rb[1], P.sb[0],
+                            P.wb[1], P.pc, P.minv, P.fixmask, 1, P.parts, v, S.nb_vec);
       if (lb == 0 && threadIdx.x == 0) P.sc_oc->nan_flag = 0;
       break;
     }
     case PH_INITFIN:
       if (lb == 0)
-        body_pcg_init_fin(P.parts, S.nb_vec, sched_tol(cur, S.cfg), S.cfg.max_iter, P.sc);
+        body_pcg_init_fin(P.parts, S.nb_vec,
+                          S.cfg.refine ? S.cfg.inner_tol : sched_tol(cur, S.cfg), S.cfg.max_iter,
+                          P.sc);
       break;
     case PH_CG: {
       const int k = cur.k, c = k & 1, o = c ^ 1;
@@ -207,8 +297,12 @@ __global__ __launch_bounds__(kBlock, 4) void k_tick(SchedTable tab, int t) {
       break;
     }
     case PH_SENS:
-      body_sens<float, true>(P.g, P.ke64, P.mc64, P.xPhys, P.u, P.law, P.penal, P.E0, P.Emin,
-                             P.dc, 0, nullptr, P.sc_oc, lb, nb);
+      if (S.cfg.refine)
+        body_sens<double, true>(P.g, P.ke64, P.mc64, P.xPhys, P.u64, P.law, P.penal, P.E0,
+                                P.Emin, P.dc, 0, nullptr, P.sc_oc, lb, nb);
+      else
+        body_sens<float, true>(P.g, P.ke64, P.mc64, P.xPhys, P.u, P.law, P.penal, P.E0, P.Emin,
+                               P.dc, 0, nullptr, P.sc_oc, lb, nb);
       break;
     case PH_FILT:
       body_filter(P.g, P.st, P.Hs, P.dc, P.dcf, 1, lb, nb);
@@ -243,7 +337,10 @@ __global__ __launch_bounds__(kBlock, 4) void k_tick(SchedTable tab, int t) {
       break;
     case PH_DOT:
       for (int v = lb; v < S.nb_vec; v += nb)
-        body_dot<double, float>(P.g.ndof, P.f, P.u, P.parts.S, v, S.nb_vec);
+        if (S.cfg.refine)
+          body_dot<double, double>(P.g.ndof, P.f, P.u64, P.parts.S, v, S.nb_vec);
+        else
+          body_dot<double, float>(P.g.ndof, P.f, P.u, P.parts.S, v, S.nb_vec);
       break;
     case PH_SUMFIN:
       if (lb == 0) body_sum_fin(P.parts.S, S.nb_vec, &P.sc_oc->sum);

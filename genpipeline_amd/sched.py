@@ -113,13 +113,17 @@ class Scheduler:
     def admit(self, slot: int, problem: Problem, n_iters: int = 0, volfrac: float | None = None,
               tol: float = 1e-6, max_iter: int = 20000, tol_schedule: bool = False,
               early_stop: bool = False, compliance_each_iter: bool = False,
-              warm_start: bool = True, x0=None, xphys0=None, u0=None, rhs=None) -> Job:
+              warm_start: bool = True, x0=None, xphys0=None, u0=None, rhs=None,
+              refine: bool = False, inner_tol: float = 1e-3, max_outer: int = 50) -> Job:
         """Start a job in a free slot (takes effect at the next tick).
 
         n_iters = 0 runs one solve at the given density (``xphys0``, else
         ``x0``, else volfrac) and reports its compliance. ``rhs`` (n_dof,
         single solves only) replaces the problem's load: the job solves
-        K d = rhs from d = 0 (``warm_start`` must be False)."""
+        K d = rhs from d = 0 (``warm_start`` must be False). ``refine``: every
+        solve is FP64 iterative refinement around FP32 inner solves at
+        ``inner_tol`` (``tol`` the FP64 residual tolerance), bitwise
+        ``Problem.run(refine=True)``."""
         sp = problem.spec
         vf = sp.volfrac if volfrac is None else float(volfrac)
         dev = self.device
@@ -154,6 +158,9 @@ class Scheduler:
         j.early_stop = 1 if early_stop else 0
         j.compliance_each_iter = 1 if compliance_each_iter else 0
         j.warm_start = 1 if warm_start else 0
+        j.refine = 1 if refine else 0
+        j.inner_tol = float(inner_tol)
+        j.max_outer = int(max_outer)
         with torch.cuda.device(dev):
             L.call("gp_sched_admit", self._h, int(slot), problem.handle, C.byref(j),
                    C.c_void_p(x.data_ptr()), C.c_void_p(xp.data_ptr()), C.c_void_p(u.data_ptr()),

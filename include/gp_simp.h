@@ -255,10 +255,17 @@ typedef struct gp_job {
   int32_t compliance_each_iter;
   int32_t warm_start;         /* first solve starts from u; later solves    */
                               /* from the previous solution                */
+  int32_t refine;             /* 1: FP64 iterative refinement around FP32   */
+                              /*    inner solves, as gp_pcg with refine = 1 */
+                              /*    (tol is then the outer FP64 residual    */
+                              /*    tolerance); u is the FP64 iterate       */
+  double inner_tol;           /* inner PCG tolerance (0: 1e-3)              */
+  int32_t max_outer;          /* refinement sweeps cap (0: 50)              */
 } gp_job;
 
 typedef struct gp_slot_status {
-  int32_t phase;              /* 0 idle, 18 done, else running              */
+  int32_t phase;              /* 0 idle, 18 done, else running (1-17, and   */
+                              /* 19-22 in the refinement sweeps)            */
   int32_t done;
   int32_t iters_done;         /* OC iterations completed                    */
   int32_t pcg_iters_total;

@@ -214,3 +214,72 @@ def test_roller_supports_job_equals_alone(gpu):
         torch.cuda.synchronize()
         S.close()
     P.close()
+
+
+@pytest.mark.parametrize("semantics", ["cpu_solver", "gpu_solver", "single_solve"])
+def test_refined_job_equals_problem_run(gpu, semantics):
+    """FP64 iterative refinement around FP32 inner solves as a scheduler job
+    (residual, sums, inner solve, correction as phases) gives bitwise
+    Problem.run(refine=True)'s results, for SIMPSolver semantics (warm
+    starts), SIMPSolverGPU semantics (scheduled outer tolerance, compliance
+    solve after every update, early stop) and a single solve; the refined
+    answer is the FP64 oracle's to the outer tolerance."""
+    from genpipeline_amd.sched import Scheduler
+    if semantics == "cpu_solver":
+        P = _problem(gpu, 12, 6, 5, 31, pres=True)
+        kw = dict(tol=1e-10, inner_tol=1e-3, max_iter=5000, warm_start=True)
+        n = 4
+    elif semantics == "gpu_solver":
+        P = _problem(gpu, 12, 6, 6, 32, law="clamp", oc="gpu", volfrac=0.35)
+        kw = dict(tol=1e-6, inner_tol=1e-2, max_iter=5000, tol_schedule=True, early_stop=True,
+                  compliance_each_iter=True, warm_start=True)
+        n = 25
+    else:
+        P = _problem(gpu, 10, 6, 6, 33, law="clamp", penal=1.0)
+        kw = dict(tol=1e-10, inner_tol=1e-3, max_iter=5000, warm_start=False)
+        n = 0
+    dens = np.random.default_rng(3).uniform(0.05, 1.0, P.spec.n_elem) if n == 0 else None
+    x0 = dict(x0=dens, xphys0=dens) if dens is not None else {}
+    P.set_ku_tile(14, 18, 2, precision="fp32")
+    ref = P.run(n, precision="fp32", refine=True, **x0, **kw)
+    if semantics == "gpu_solver":
+        assert ref.iters_done < n  # the early stop fired
+    assert ref.status == 0
+    S = Scheduler([(P.spec.nx, P.spec.ny, P.spec.nz, 2)], tile=(14, 18))
+    S.admit(0, P, n_iters=n, refine=True, **x0, **kw)
+    _, st = S.run()
+    _same(S.result(0, st[0]), ref)
+    torch.cuda.synchronize()
+    S.close()
+    if n == 0:
+        Po = O.OracleProblem(10, 6, 6, P.f_host, P.fixed, penal=1.0, law="clamp")
+        u, _ = Po.solve_u(dens, "direct")
+        assert np.abs(ref.u.cpu().numpy() - u).max() <= 1e-8 * np.abs(u).max()
+
+
+def test_refined_and_fp32_jobs_share_ticks(gpu):
+    """Refined and plain FP32 jobs of different sizes in one scheduler, one
+    admitted while the others run: each equals its job alone."""
+    from genpipeline_amd.sched import Scheduler
+    A = _problem(gpu, 14, 6, 6, 41)
+    B = _problem(gpu, 10, 8, 6, 42, volfrac=0.35, oc="gpu")
+    Cp = _problem(gpu, 12, 6, 6, 43)
+    tile = (14, 18)
+    for P in (A, B, Cp):
+        P.set_ku_tile(*tile, 2, precision="fp32")
+    ka = dict(tol=1e-9, inner_tol=1e-3, max_iter=5000, refine=True)
+    kb = dict(tol=1e-6, max_iter=5000)
+    ra = A.run(3, precision="fp32", **ka)
+    rb = B.run(3, precision="fp32", **kb)
+    rc = Cp.run(2, precision="fp32", **ka)
+    S = Scheduler([(16, 8, 8, 2)] * 3, tile=tile, graph_ticks=16)
+    S.admit(0, A, n_iters=3, **ka)
+    S.admit(1, B, n_iters=3, **kb)
+    S.tick(50)
+    S.admit(2, Cp, n_iters=2, **ka)
+    _, st = S.run()
+    _same(S.result(0, st[0]), ra)
+    _same(S.result(1, st[1]), rb)
+    _same(S.result(2, st[2]), rc)
+    torch.cuda.synchronize()
+    S.close()
