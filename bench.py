@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--oc-iters", type=int, default=100)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="default workload: skip the 48-problem batch and serving summaries")
     ap.add_argument("--cpu-oc-iters", type=int, default=2)
     ap.add_argument("--traffic-file",
                     default=str(ROOT / "profiles" / "k_cg_modal_64x64x64_traffic.json"))
@@ -90,12 +92,16 @@ def _line(metric, value, unit, steps, warmup, ms, dtype, config, **extra):
     return out
 
 
-def bench_batch48(a):
+def batch48_legs(a, refine_leg=True, fp64_sample=4):
     """C4: 48 SIMP problems, grids drawn by seed from {32^3, 48x24x24, 64x32x32, 64^3},
     volfrac ~ U[0.2, 0.4], 1-4 seeded point loads on face x = max, clamp at x = 0,
-    60 OC iterations, FP32 PCG at 1e-6, all in one device scheduler (independent
-    progress, ticks replayed from a HIP graph); timed beside the same problems
-    run one after another through the single-problem path."""
+    60 OC iterations, all in one device scheduler (independent progress, ticks
+    replayed from a HIP graph), timed beside the same problems run one after
+    another through the single-problem FP32 path (PCG at 1e-6). Legs: the
+    FP32 batch at 1e-6, and the mixed-precision batch (FP64 refinement to
+    1e-7 around FP32 inner solves at 1e-2). A sample of one problem per size
+    class is also solved in FP64 at 1e-10: each leg's compliance and density
+    error against it."""
     import torch
     from genpipeline_amd import Problem
     from genpipeline_amd.problems import batch_specs
@@ -104,36 +110,80 @@ def bench_batch48(a):
     ws = batch_specs(seed=0, n=a.batch_n, n_iters=a.batch_oc_iters)
     dev = torch.device("cuda", 0)
     Ps = [Problem(w.spec, w.fixed, w.f, device=dev) for w in ws]
-    run_batch(Ps[:4], 2, tile=(14, 18), ni=8, graph_ticks=32)  # warm-up
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    res, ticks = run_batch(Ps, a.batch_oc_iters, tile=(14, 18), ni=8, graph_ticks=32, tol=1e-6,
-                           max_iter=20000)
-    torch.cuda.synchronize()
-    t_b = time.perf_counter() - t0
-    print(f"[bench] batch48 scheduler: {t_b:.2f} s, {ticks} ticks", file=sys.stderr, flush=True)
+    sched = dict(tile=(14, 18), ni=8, graph_ticks=32, max_iter=20000)
+    legs = {"fp32": dict(tol=1e-6)}
+    if refine_leg:
+        legs["fp32+ir"] = dict(tol=1e-7, refine=True, inner_tol=1e-2)
+    run_batch(Ps[:4], 2, **sched)  # warm-up
+    out = {}
+    results = {}
+    for name, kw in legs.items():
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res, ticks = run_batch(Ps, a.batch_oc_iters, **sched, **kw)
+        torch.cuda.synchronize()
+        t_b = time.perf_counter() - t0
+        results[name] = res
+        pcg_bytes = sum(r.pcg_iters * (120 * P.spec.n_nodes + 4 * P.spec.n_elem)
+                        for r, P in zip(res, Ps))
+        out[name] = {"seconds": t_b, "solves_per_s": len(Ps) / t_b, "ticks": ticks,
+                     "us_per_tick": t_b / ticks * 1e6,
+                     "pcg_iters_total": sum(r.pcg_iters for r in res),
+                     "pcg_algorithmic_gbs": pcg_bytes / t_b / 1e9,
+                     "worst_status": max(r.status for r in res)}
+        print(f"[bench] batch48 scheduler {name}: {t_b:.2f} s, {ticks} ticks", file=sys.stderr,
+              flush=True)
     t0 = time.perf_counter()
     seq = [P.run(a.batch_oc_iters, tol=1e-6, max_iter=20000, precision="fp32", warm_start=True)
            for P in Ps]
     torch.cuda.synchronize()
     t_s = time.perf_counter() - t0
-    print(f"[bench] batch48 one by one: {t_s:.2f} s", file=sys.stderr, flush=True)
-    rel = max(abs(x.compliance - y.compliance) / abs(y.compliance) for x, y in zip(res, seq))
-    its = sum(r.pcg_iters for r in res)
-    # algorithmic bytes of every PCG iteration the batch ran (120 B/node + 4 B/elem each)
-    pcg_bytes = sum(r.pcg_iters * (120 * P.spec.n_nodes + 4 * P.spec.n_elem)
-                    for r, P in zip(res, Ps))
-    n = len(Ps)
-    return _line("SIMP solves/sec (48-problem batch, device scheduler)", n / t_b, "solves/s", 1,
-                 1, t_b * 1e3, "f32",
-                 {"workload": "simp_batch48_c4_seed0", "problems": n, "oc_iters": a.batch_oc_iters,
+    print(f"[bench] batch48 one by one (fp32, 1e-6): {t_s:.2f} s", file=sys.stderr, flush=True)
+    one = {"seconds": t_s, "solves_per_s": len(Ps) / t_s}
+    rel = max(abs(x.compliance - y.compliance) / abs(y.compliance)
+              for x, y in zip(results["fp32"], seq))
+    for name in out:
+        out[name]["speedup_vs_fp32_one_by_one"] = t_s / out[name]["seconds"]
+    # FP64 references for one problem of each size class
+    sample, seen = [], set()
+    for i, w in enumerate(ws):
+        g = (w.spec.nx, w.spec.ny, w.spec.nz)
+        if g not in seen and len(sample) < fp64_sample:
+            seen.add(g)
+            sample.append(i)
+    err = {name: {"compliance_rel_max": 0.0, "density_max_abs": 0.0} for name in out}
+    err["fp32_one_by_one"] = {"compliance_rel_max": 0.0, "density_max_abs": 0.0}
+    for i in sample:
+        ref = Ps[i].run(a.batch_oc_iters, tol=1e-10, max_iter=40000, precision="fp64",
+                        warm_start=True)
+        for name, rs in list(results.items()) + [("fp32_one_by_one", seq)]:
+            e = err[name]
+            e["compliance_rel_max"] = max(e["compliance_rel_max"],
+                                          abs(rs[i].compliance - ref.compliance)
+                                          / abs(ref.compliance))
+            e["density_max_abs"] = max(e["density_max_abs"],
+                                       float((rs[i].xPhys - ref.xPhys).abs().max()))
+    for P in Ps:
+        P.close()
+    return {"problems": len(Ps), "oc_iters": a.batch_oc_iters, "legs": out,
+            "fp32_one_by_one": one, "max_rel_compliance_diff_fp32_batch_vs_one_by_one": rel,
+            "error_vs_fp64_1e-10": err,
+            "fp64_sample": [f"{ws[i].spec.nx}x{ws[i].spec.ny}x{ws[i].spec.nz}#{i}" for i in sample]}
+
+
+def bench_batch48(a):
+    """C4 as its own bench line (value: the FP32 batch's solves/s)."""
+    d = batch48_legs(a)
+    leg = d["legs"]["fp32"]
+    return _line("SIMP solves/sec (48-problem batch, device scheduler)", leg["solves_per_s"],
+                 "solves/s", 1, 1, leg["seconds"] * 1e3, "f32",
+                 {"workload": "simp_batch48_c4_seed0", "problems": d["problems"],
+                  "oc_iters": a.batch_oc_iters,
                   "grids": "32^3 / 48x24x24 / 64x32x32 / 64^3 by seed",
-                  "pcg": "jacobi, fp32 vectors, fp64 dots, rel tol 1e-6, warm start",
+                  "pcg": "jacobi, fp32 vectors, fp64 dots, rel tol 1e-6, warm start; "
+                         "fp32+ir leg: FP64 refinement to 1e-7 around FP32 inner solves at 1e-2",
                   "scheduler": "14x18 tile in 256-lane blocks, ni 8, 32-tick HIP graph"},
-                 extra={"one_by_one_solves_per_s": n / t_s, "speedup_vs_one_by_one": t_s / t_b,
-                        "ticks": ticks, "us_per_tick": t_b / ticks * 1e6, "pcg_iters_total": its,
-                        "pcg_algorithmic_gbs": pcg_bytes / t_b / 1e9,
-                        "max_rel_compliance_diff_vs_one_by_one": rel})
+                 extra=d)
 
 
 def bench_serve(a):
@@ -459,6 +509,36 @@ def cpu_baseline(w, gpu_pcg_iters, gpu_oc_iters, n_sample):
     }
 
 
+def accuracy_vs_fp64(P, w, r_head):
+    """The headline solve's error against the same SIMP run in FP64 at PCG
+    tolerance 1e-10 (compliance relative, density max-abs), and the error and
+    rate of FP64 refinement around FP32 inner solves at an outer tolerance of
+    1e-9, the mode that meets 1e-6 / 1e-5."""
+    import torch
+    out = {}
+    ref = None
+    for name, kw in (("fp64_tol1e-10", dict(precision="fp64", tol=1e-10)),
+                     ("fp32+ir_tol1e-9", dict(precision="fp32", refine=True, tol=1e-9,
+                                              inner_tol=1e-3))):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rr = P.run(w.n_iters, max_iter=40000, warm_start=True, **kw)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        if ref is None:
+            ref = rr
+        out[name] = {"seconds": dt, "solves_per_s": 1.0 / dt, "pcg_iters": rr.pcg_iters,
+                     "compliance": rr.compliance,
+                     "compliance_rel_err_vs_fp64": abs(rr.compliance - ref.compliance)
+                     / abs(ref.compliance),
+                     "density_max_abs_err_vs_fp64": float((rr.xPhys - ref.xPhys).abs().max())}
+        print(f"[bench] {name}: {dt:.2f} s, C={rr.compliance:.12g}", file=sys.stderr, flush=True)
+    out["headline_compliance_rel_err_vs_fp64"] = (abs(r_head.compliance - ref.compliance)
+                                                  / abs(ref.compliance))
+    out["headline_density_max_abs_err_vs_fp64"] = float((r_head.xPhys - ref.xPhys).abs().max())
+    return out
+
+
 def _cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -591,12 +671,20 @@ def main():
             "ku_tile_fp32": list(ku_tile),
         },
     }
+    print("[bench] accuracy against FP64", file=sys.stderr, flush=True)
+    out["extra"]["accuracy"] = accuracy_vs_fp64(P, w, r)
+    P.close()
+    if not a.no_extra:
+        for key, fn in (("batch48", lambda: batch48_legs(a)), ("serve", lambda: serve_legs(a))):
+            try:
+                out["extra"][key] = fn()
+            except Exception as e:  # the headline line is printed whatever a side leg does
+                out["extra"][key] = {"error": f"{type(e).__name__}: {e}"}
     if not a.no_cpu:
         print("[bench] cpu baseline sample", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(w, r.pcg_iters, w.n_iters, a.cpu_oc_iters)
     else:
         out["cpu_baseline"] = None
-    P.close()
     print(json.dumps(out), flush=True)
 
 

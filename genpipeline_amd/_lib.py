@@ -49,7 +49,8 @@ class GpOcStats(C.Structure):
 class GpSimpOpts(C.Structure):
     _fields_ = [("n_iters", C.c_int32), ("volfrac", C.c_double), ("pcg", GpPcgOpts),
                 ("tol_schedule", C.c_int32), ("early_stop", C.c_int32),
-                ("compliance_each_iter", C.c_int32), ("warm_start", C.c_int32)]
+                ("compliance_each_iter", C.c_int32), ("warm_start", C.c_int32),
+                ("warm_compliance", C.c_int32)]
 
 
 class GpSimpStats(C.Structure):
@@ -62,7 +63,8 @@ class GpJob(C.Structure):
     _fields_ = [("n_iters", C.c_int32), ("volfrac", C.c_double), ("tol", C.c_double),
                 ("max_iter", C.c_int32), ("tol_schedule", C.c_int32), ("early_stop", C.c_int32),
                 ("compliance_each_iter", C.c_int32), ("warm_start", C.c_int32),
-                ("refine", C.c_int32), ("inner_tol", C.c_double), ("max_outer", C.c_int32)]
+                ("refine", C.c_int32), ("inner_tol", C.c_double), ("max_outer", C.c_int32),
+                ("warm_compliance", C.c_int32)]
 
 
 class GpSlotStatus(C.Structure):

@@ -1180,7 +1180,8 @@ int gp_simp_run(gp_problem* P, const gp_simp_opts* o, double* x, double* xPhys, 
       // the caller's iteration cap as above); its u becomes the next warm
       // start (simp_solver_gpu.py:593,704)
       launch_stiffness(g, xPhys, P->E, sp.law, sp.penal, sp.E0, sp.Emin, s);
-      rc = solve(1e-6 > o->pcg.tol ? o->pcg.tol : 1e-6, max_iter, false, &ps);
+      rc = solve(1e-6 > o->pcg.tol ? o->pcg.tol : 1e-6, max_iter,
+                 o->warm_compliance != 0 && o->warm_start != 0, &ps);
       if (rc) return rc;
       rc = compliance(&curr_c);
       if (rc) return rc;
@@ -1939,6 +1940,7 @@ int gp_sched_admit(gp_sched* Q, int32_t slot, gp_problem* P, const gp_job* job, 
   q.cfg.refine = job->refine;
   q.cfg.inner_tol = job->inner_tol > 0 ? job->inner_tol : 1e-3;
   q.cfg.max_outer = job->max_outer > 0 ? job->max_outer : 50;
+  q.cfg.warm_compl = job->warm_compliance != 0 ? 1 : 0;
   SchedState s0 = idle_state();
   // an FP64-refined job works on the caller's u directly: no float copy in
   s0.phase = job->warm_start && !job->refine ? PH_CAST_IN : PH_STIFF;

@@ -165,6 +165,7 @@ struct SchedCfg {
   double tol;
   int refine, max_outer;  // FP64 refinement (outer tolerance: tol, as scheduled)
   double inner_tol;
+  int warm_compl, pad;    // compliance solves warm-started too
 };
 
 struct SchedState {

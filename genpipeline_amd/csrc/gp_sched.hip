@@ -30,7 +30,7 @@ __device__ inline double sched_tol(const SchedState& s, const SchedCfg& c) {
 }
 
 __device__ inline bool sched_use_x0(const SchedState& s, const SchedCfg& c) {
-  return c.warm_start != 0 && s.solve != SK_COMPL;
+  return c.warm_start != 0 && (s.solve != SK_COMPL || c.warm_compl != 0);
 }
 
 // Last phase of a job: FP32 jobs copy their float iterate out, FP64-refined

@@ -18,11 +18,11 @@ the decoded field, instead of on the host.
 reference's four worker threads (:559-611): one VAE decode of all latent
 vectors, one pre-processing launch chain, and every solve admitted into the
 continuous-batching scheduler together, with GPUVoxelFEM semantics per
-request and GPUVoxelFEM's own refinement, batched: FP32 Jacobi-PCG (FP64 dot
-products) from zero to ``batch_inner_tol``, then FP64 residuals r = f - K u
-and further batched FP32 solves K d = r until every field's residual is below
-``batch_tol``. With ``batch_refine = False`` each field gets one FP32 solve to
-``batch_solve_tol`` instead: about twice as fast, compliance within about
+request and GPUVoxelFEM's own refinement as scheduler jobs: FP64 residuals
+r = f - K u and FP32 Jacobi-PCG (FP64 dot products) corrections K d = r at
+``batch_inner_tol`` until each field's residual is below ``batch_tol``, all
+on the device. With ``batch_refine = False`` each field gets one FP32 solve
+to ``batch_solve_tol`` instead: about twice as fast, compliance within about
 1e-5 of the refined value on solid/void fields. Results and history entries
 are in parameter order.
 
@@ -85,12 +85,6 @@ def decode_to_voxels(zs, vae_model):
     return prep_voxels(decode_batch(zs, vae_model))
 
 
-def torch_stack_norms(vs):
-    """Euclidean norms of a list of device vectors, one host transfer."""
-    import torch
-    return torch.stack([v.norm() for v in vs]).cpu()
-
-
 class VoxelFEMEvaluator:
     """Evaluator with the reference constructor (voxel_fem.py:406-438).
     ``ccx_cmd`` and ``output_dir`` are accepted and kept for callers that
@@ -100,7 +94,7 @@ class VoxelFEMEvaluator:
     batch_refine: bool = True       # FP64 iterative refinement (GPUVoxelFEM's accuracy)
     batch_tol: float = 1e-8         # refined: FP64 relative residual reached
     batch_inner_tol: float = 1e-3   # refined: FP32 tolerance of every batched solve
-    batch_max_outer: int = 6        # refined: batched solves per field at most
+    batch_max_outer: int = 60       # refined: correction solves per field at most
     batch_solve_tol: float = 1e-6   # unrefined: tolerance of the one FP32 solve
 
     def __init__(self, ccx_cmd: str = None, output_dir: str = None, fixed_face: str = "x_min",
@@ -116,15 +110,11 @@ class VoxelFEMEvaluator:
         self.evaluation_history = []
         self._counter = 0
         self._servers = {}
-        self._problems = {}
 
     def close(self):
         for srv in self._servers.values():
             srv.close()
-        for p in self._problems.values():
-            p.close()
         self._servers = {}
-        self._problems = {}
 
     def __del__(self):
         try:
@@ -164,45 +154,16 @@ class VoxelFEMEvaluator:
         from ..serve import Server, ServeConfig
         srv = self._servers.get(shape)
         if srv is None:
-            cfg = ServeConfig(force_n=float(self.force_n), solve_tol=float(self.batch_solve_tol),
+            refine = bool(self.batch_refine)
+            cfg = ServeConfig(force_n=float(self.force_n),
+                              solve_tol=float(self.batch_tol if refine else self.batch_solve_tol),
+                              solve_refine=refine, solve_inner_tol=float(self.batch_inner_tol),
+                              solve_max_outer=int(self.batch_max_outer),
                               bcs={"fixed_face": self.fixed_face, "load_face": self.load_face,
                                    "load_dof": 2})
             srv = Server({shape: int(self.batch_slots)}, cfg=cfg, tile=(14, 18), ni=8)
             self._servers[shape] = srv
         return srv
-
-    def _refine(self, shape, vox, us):
-        """FP64 iterative refinement of a batch of FP32 solutions (the
-        scheme of GPUVoxelFEM.solve): each pass forms r = f - K u in FP64 for
-        every field, and the fields with ||r|| > batch_tol ||f|| solve
-        K d = r from zero in FP32 (tolerance batch_inner_tol) as one
-        scheduler batch; u += d. Returns the corrected displacements, the
-        fields whose correction failed, and the problem used for K u."""
-        from ..problems import Request
-        from ..serve import make_problem
-        srv = self._server(shape)
-        P = self._problems.get(shape)
-        if P is None:
-            P = make_problem(shape, "solve", srv.cfg, srv.device)
-            self._problems[shape] = P
-        Es = [P.stiffness(vox[b].reshape(-1).double()) for b in range(len(us))]
-        fn = float(P.f.norm().item())
-        failed = set()
-        for _ in range(int(self.batch_max_outer) - 1):
-            rs = [P.f - P.ku(Es[b], us[b]) for b in range(len(us))]
-            rel = (torch_stack_norms(rs) / fn).tolist()
-            todo = [b for b in range(len(us)) if b not in failed and rel[b] > self.batch_tol]
-            if not todo:
-                break
-            corr = srv.run([Request(b, 0.0, shape, vox[b], "solve", 0.0, rhs=rs[b],
-                                    tol=self.batch_inner_tol) for b in todo], keep_u=True,
-                           outputs=False)
-            for b, c in zip(todo, corr):
-                if c.status != 0:
-                    failed.add(b)
-                else:
-                    us[b] = us[b] + c.u
-        return us, failed, P
 
     def evaluate_latents(self, zs, bbox: dict = None) -> list:
         """Result dicts of ``evaluate`` for B latent vectors, evaluated as one
@@ -217,23 +178,23 @@ class VoxelFEMEvaluator:
         vox = decode_to_voxels(zs, self.vae_model)
         shape = tuple(int(n) for n in vox.shape[1:])
         solid = (vox > 0.5).reshape(len(zs), -1).float().mean(dim=1).double().cpu().numpy()
-        tol0 = self.batch_inner_tol if self.batch_refine else self.batch_solve_tol
-        reqs = [Request(b, 0.0, shape, vox[b], "solve", 0.0, tol=tol0) for b in range(len(zs))]
+        reqs = [Request(b, 0.0, shape, vox[b], "solve", 0.0) for b in range(len(zs))]
         try:
-            # refined fields take their outputs after refinement
-            res = self._server(shape).run(reqs, keep_u=self.batch_refine,
-                                          outputs=not self.batch_refine)
+            from ..serve import fem_outputs_batch
+            srv = self._server(shape)
+            if srv.cfg.solve_refine != bool(self.batch_refine):
+                srv.close()
+                self._servers.pop(shape)
+                srv = self._server(shape)
+            # the output formulas run once for the whole batch (two read-backs)
+            res = srv.run(reqs, keep_u=True, outputs=False)
             failed = {b for b, r in enumerate(res) if r.status != 0}
             fem = [(r.compliance, r.stress_max) for r in res]
-            if self.batch_refine:
-                from ..serve import fem_outputs_batch
-                us, bad, P = self._refine(shape, vox, [r.u for r in res])
-                failed |= bad
-                ok = [b for b in range(len(zs)) if b not in failed]
-                outs = fem_outputs_batch([P.compliance(us[b]) for b in ok], [us[b] for b in ok],
-                                         vox[ok])
-                for b, (c, sm, _) in zip(ok, outs):
-                    fem[b] = (c, sm)
+            ok = [b for b in range(len(zs)) if b not in failed]
+            outs = fem_outputs_batch([res[b].compliance for b in ok], [res[b].u for b in ok],
+                                     vox[ok])
+            for b, (c, sm, _) in zip(ok, outs):
+                fem[b] = (c, sm)
         except Exception as e:
             logger.warning(f"[VoxelFEM] batched GPU FEM failed: {e}")
             srv = self._servers.pop(shape, None)

@@ -321,8 +321,11 @@ class Problem:
 
     def run(self, n_iters, x0=None, volfrac=None, tol=1e-6, max_iter=10000, precision="fp32",
             refine=False, inner_tol=1e-3, max_outer=50, warm_start=True, tol_schedule=False,
-            early_stop=False, compliance_each_iter=False, u0=None, xphys0=None) -> SimpResult:
-        """Whole SIMP loop on the device (gp_simp_run)."""
+            early_stop=False, compliance_each_iter=False, u0=None, xphys0=None,
+            warm_compliance=False) -> SimpResult:
+        """Whole SIMP loop on the device (gp_simp_run). warm_compliance: the
+        compliance solve after each update starts from the previous u
+        instead of zero."""
         sp = self.spec
         vf = sp.volfrac if volfrac is None else volfrac
         if x0 is None:
@@ -348,6 +351,7 @@ class Problem:
         o.early_stop = 1 if early_stop else 0
         o.compliance_each_iter = 1 if compliance_each_iter else 0
         o.warm_start = 1 if warm_start else 0
+        o.warm_compliance = 1 if warm_compliance else 0
         st = L.GpSimpStats()
         L.call("gp_simp_run", self._h, C.byref(o), _ptr(x), _ptr(xPhys), _ptr(u), C.byref(st),
                _stream(self.device))

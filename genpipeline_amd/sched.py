@@ -114,7 +114,8 @@ class Scheduler:
               tol: float = 1e-6, max_iter: int = 20000, tol_schedule: bool = False,
               early_stop: bool = False, compliance_each_iter: bool = False,
               warm_start: bool = True, x0=None, xphys0=None, u0=None, rhs=None,
-              refine: bool = False, inner_tol: float = 1e-3, max_outer: int = 50) -> Job:
+              refine: bool = False, inner_tol: float = 1e-3, max_outer: int = 50,
+              warm_compliance: bool = False) -> Job:
         """Start a job in a free slot (takes effect at the next tick).
 
         n_iters = 0 runs one solve at the given density (``xphys0``, else
@@ -123,7 +124,8 @@ class Scheduler:
         K d = rhs from d = 0 (``warm_start`` must be False). ``refine``: every
         solve is FP64 iterative refinement around FP32 inner solves at
         ``inner_tol`` (``tol`` the FP64 residual tolerance), bitwise
-        ``Problem.run(refine=True)``."""
+        ``Problem.run(refine=True)``. ``warm_compliance``: compliance solves
+        start from the previous u (as ``Problem.run``'s option)."""
         sp = problem.spec
         vf = sp.volfrac if volfrac is None else float(volfrac)
         dev = self.device
@@ -161,6 +163,7 @@ class Scheduler:
         j.refine = 1 if refine else 0
         j.inner_tol = float(inner_tol)
         j.max_outer = int(max_outer)
+        j.warm_compliance = 1 if warm_compliance else 0
         with torch.cuda.device(dev):
             L.call("gp_sched_admit", self._h, int(slot), problem.handle, C.byref(j),
                    C.c_void_p(x.data_ptr()), C.c_void_p(xp.data_ptr()), C.c_void_p(u.data_ptr()),

@@ -13,7 +13,8 @@ Request kinds mirror the reference entry points the BO loop issues:
   (genpipeline/fem/gpu_fem_solver.py:76-133): E = max(voxel, 1e-3), supports
   and load of SIMPSolverGPU._get_bcs with force_n, outputs compliance, the
   stress proxy 10 C / n_solid, the displacement proxy, and the sentinels
-  {1e4, 1e3, 1e6} when fewer than 50 voxels are solid or C < 1.
+  {1e4, 1e3, 1e6} when fewer than 50 voxels are solid or C < 1. Solved by
+  FP64 iterative refinement (to 1e-8 by default) around FP32 inner solves.
 * ``"simp"`` -- SIMPSolverGPU.run(volfrac, n_iters, x_init=density)
   (genpipeline/topology/simp_solver_gpu.py:71-136): x_init rescaled to the
   volume fraction and clipped to [1e-3, 1], tolerance schedule, compliance
@@ -68,7 +69,17 @@ class ServeConfig:
     force_n: float = 1000.0       # "solve" total load (GPUVoxelFEM default)
     simp_force: float = 1.0       # "simp" load magnitude (SIMPSolverGPU.run default)
     simp_iters: int = 30
-    solve_tol: float = 1e-6
+    # "solve" requests: FP64 iterative refinement to solve_tol around FP32
+    # inner solves at solve_inner_tol (refine False: one FP32 solve to
+    # solve_tol, whose accuracy stops near 1e-5 on solid/void fields)
+    solve_refine: bool = True
+    solve_tol: float = 1e-8
+    solve_inner_tol: float = 1e-2
+    solve_max_outer: int = 50
+    # "simp" requests: the compliance solve after each update starts from the
+    # previous displacement instead of zero (SIMPSolverGPU solves it from zero,
+    # simp_solver_gpu.py:747-751; same answers to the tolerance, fewer iterations)
+    simp_warm_compliance: bool = False
     max_iter: int = 20000
     # admission order among queued requests that fit: "fifo" (arrival order)
     # or "sjf" (single solves before SIMP runs, each kind in arrival order;
@@ -98,14 +109,18 @@ def job_args(req: ServeRequest, cfg: ServeConfig) -> dict:
     dens = _flat64(req.density)
     if req.kind == "solve":
         tol = cfg.solve_tol if getattr(req, "tol", None) is None else float(req.tol)
+        refine = cfg.solve_refine and getattr(req, "refine", True)
+        extra = (dict(refine=True, inner_tol=cfg.solve_inner_tol, max_outer=cfg.solve_max_outer)
+                 if refine else {})
         return dict(n_iters=0, x0=dens, xphys0=dens, tol=tol, max_iter=cfg.max_iter,
-                    warm_start=False, rhs=getattr(req, "rhs", None))
+                    warm_start=False, rhs=getattr(req, "rhs", None), **extra)
     if isinstance(dens, torch.Tensor):
         dens = dens.cpu().numpy()
     x = np.clip(dens * (req.volfrac / max(dens.mean(), 1e-6)), 1e-3, 1.0)
     return dict(n_iters=cfg.simp_iters, volfrac=req.volfrac, x0=x, xphys0=x, tol=1e-6,
                 max_iter=cfg.max_iter, tol_schedule=True, early_stop=True,
-                compliance_each_iter=True, warm_start=True)
+                compliance_each_iter=True, warm_start=True,
+                warm_compliance=cfg.simp_warm_compliance)
 
 
 def make_problem(shape, kind, cfg: ServeConfig, device) -> Problem:

@@ -109,6 +109,10 @@ typedef struct gp_simp_opts {
                               /*    (simp_solver_gpu.py:118-125)             */
   int32_t compliance_each_iter; /* 1: compliance solve every OC iteration     */
   int32_t warm_start;         /* 1: PCG starts from the previous u           */
+  int32_t warm_compliance;    /* 1: with compliance_each_iter and warm_start, */
+                              /*    the compliance solve also starts from the */
+                              /*    previous u (the reference solves it from  */
+                              /*    zero, simp_solver_gpu.py:747-751)         */
 } gp_simp_opts;
 
 typedef struct gp_simp_stats {
@@ -261,6 +265,7 @@ typedef struct gp_job {
                               /*    tolerance); u is the FP64 iterate       */
   double inner_tol;           /* inner PCG tolerance (0: 1e-3)              */
   int32_t max_outer;          /* refinement sweeps cap (0: 50)              */
+  int32_t warm_compliance;    /* as gp_simp_opts.warm_compliance            */
 } gp_job;
 
 typedef struct gp_slot_status {

@@ -60,3 +60,123 @@ def test_c2_64cube_point_loads_matches_oracle(gpu):
     err_x = float(np.abs(r.xPhys.cpu().numpy() - xp).max())
     assert err_x <= 1e-5, err_x
     assert abs(r.compliance - c) <= 1e-6 * abs(c), (r.compliance, c)
+
+
+# ------------------------------------------------- the other configurations
+def test_c3_mixed_precision_128x64x64_against_fp64(gpu):
+    """C3 at its own size: 128x64x64, traction of total 1 on face x = max,
+    seeded smoothed-noise density floored at 0.01 and rescaled to 0.25; two
+    OC iterations with FP64 refinement (outer 1e-9, inner 1e-3) around inner
+    PCG solves storing x and p in fp16, in bf16 and in fp32, against the
+    FP64 run at tolerance 1e-10 on the same state (itself pinned to the C
+    oracle above): compliance within 1e-6 relative, density within 1e-5."""
+    import torch
+    from genpipeline_amd import Problem
+    from genpipeline_amd.problems import mixed_128x64x64
+    w = mixed_128x64x64(seed=0, n_iters=2)
+    assert w.x0.min() >= 0.01 - 1e-15 and abs(w.x0.mean() - 0.25) < 1e-12
+    base = dict(x0=w.x0, xphys0=w.x0, max_iter=20000, max_outer=60, warm_start=True)
+    with Problem(w.spec, w.fixed, w.f, device=gpu) as P:
+        print("[c3] fp64 reference", flush=True)
+        ref = P.run(2, precision="fp64", tol=1e-10, **base)
+        assert ref.status == 0
+        for prec in ("fp16", "bf16", "fp32"):
+            print(f"[c3] {prec}+ir", flush=True)
+            r = P.run(2, precision=prec, refine=True, tol=1e-9, inner_tol=1e-3, **base)
+            assert r.status == 0, prec
+            crel = abs(r.compliance - ref.compliance) / abs(ref.compliance)
+            dmax = float((r.xPhys - ref.xPhys).abs().max())
+            print(f"[c3] {prec}+ir: compliance {crel:.2e} density {dmax:.2e} "
+                  f"({r.pcg_iters} PCG its)", flush=True)
+            assert crel <= 1e-6 and dmax <= 1e-5, (prec, crel, dmax)
+        torch.cuda.synchronize()
+
+
+def test_c4_batch48_each_equals_alone_and_oracle(gpu):
+    """C4 at its own size: the seed-0 48-problem mix (32^3, 48x24x24,
+    64x32x32, 64^3) for two OC iterations in one device scheduler, FP32 at
+    1e-6: every problem bitwise its Problem.run alone. Then eight of them as
+    FP64-refined jobs (outer 1e-10) beside the rest: bitwise alone again, and
+    a 32^3 one against the oracle's FP64 C restatement (1e-6 / 1e-5)."""
+    import torch
+    from genpipeline_amd import Problem
+    from genpipeline_amd.problems import batch_specs
+    from genpipeline_amd.sched import Scheduler, run_batch
+    from oracle import cpu_ref
+    ws = batch_specs(seed=0, n=48, n_iters=2)
+    assert {(w.spec.nx, w.spec.ny, w.spec.nz) for w in ws} == {
+        (32, 32, 32), (48, 24, 24), (64, 32, 32), (64, 64, 64)}
+    Ps = [Problem(w.spec, w.fixed, w.f, device=gpu) for w in ws]
+    print("[c4] 48 FP32 jobs", flush=True)
+    res, ticks = run_batch(Ps, 2, tile=(14, 18), ni=8, graph_ticks=32, tol=1e-6, max_iter=20000)
+    for i, (P, rb) in enumerate(zip(Ps, res)):
+        ra = P.run(2, tol=1e-6, max_iter=20000, precision="fp32", warm_start=True)
+        assert ra.compliance == rb.compliance and ra.pcg_iters == rb.pcg_iters, i
+        assert torch.equal(ra.xPhys, rb.xPhys) and torch.equal(ra.u, rb.u), i
+    refined = [i for i in range(48) if i % 6 == 0]
+    kr = dict(tol=1e-10, inner_tol=1e-3, max_iter=20000, refine=True)
+    print("[c4] 8 refined jobs among 40 FP32 jobs", flush=True)
+    S = Scheduler([(P.spec.nx, P.spec.ny, P.spec.nz, 8) for P in Ps], tile=(14, 18),
+                  graph_ticks=32)
+    try:
+        for i, P in enumerate(Ps):
+            S.admit(i, P, n_iters=2, **(kr if i in refined else dict(tol=1e-6, max_iter=20000)))
+        _, st = S.run()
+        for i in refined:
+            ra = Ps[i].run(2, precision="fp32", warm_start=True, **{k: v for k, v in kr.items()})
+            rb = S.result(i, st[i])
+            assert ra.compliance == rb.compliance and torch.equal(ra.xPhys, rb.xPhys), i
+            assert torch.equal(ra.u, rb.u), i
+        i32 = next(i for i in refined if ws[i].spec.nx == 32 and ws[i].spec.ny == 32)
+        rb = S.result(i32, st[i32])
+        threads = max(1, min(16, cpu_ref.max_threads()))
+        w = ws[i32]
+        _, xp, _, c, _ = cpu_ref.simp_run((32, 32, 32), O.ke_hex8(0.3), w.f, w.fixed, 2,
+                                          w.spec.volfrac, 1e-10, law=w.spec.law, threads=threads)
+        assert float((rb.xPhys.cpu() - torch.from_numpy(xp)).abs().max()) <= 1e-5
+        assert abs(rb.compliance - c) <= 1e-6 * abs(c)
+        torch.cuda.synchronize()
+    finally:
+        S.close()
+        for P in Ps:
+            P.close()
+
+
+def test_c5_serving_32cube_64cube_alone_vs_mix(gpu):
+    """C5 at its own sizes: Poisson arrivals of 32^3 (60%) and 64^3 (40%)
+    smoothed-noise density fields, half single solves (GPUVoxelFEM
+    semantics, FP64-refined to 1e-8) and half 30-iteration SIMP refinements
+    (SIMPSolverGPU.run semantics), served with continuous batching: every
+    result is bitwise the same alone and in the mix, and the requests
+    overlapped on the device. Single solves finish with status 0; a SIMP
+    request's FP32 solves on fields of modulus contrast 1e9 (x clipped at
+    1e-3, cubed) may stop at a breakdown at FP32's floor, which its status
+    reports the same alone and in the mix."""
+    import torch
+    from genpipeline_amd.problems import serving_requests
+    from genpipeline_amd.serve import Server, ServeConfig, run_alone
+    reqs = serving_requests(200, 8, seed=7)
+    assert {r.shape for r in reqs} == {(32, 32, 32), (64, 64, 64)}
+    assert {r.kind for r in reqs} == {"solve", "simp"}
+    cfg = ServeConfig(policy="sjf")
+    srv = Server({(32, 32, 32): 4, (64, 64, 64): 2}, cfg=cfg, tile=(14, 18), ni=8,
+                 chunk_ticks=64)
+    try:
+        print("[c5] serving 8 requests", flush=True)
+        res = srv.run(reqs, keep_xphys=True)
+        torch.cuda.synchronize()
+    finally:
+        srv.close()
+    overlapping = sum(1 for a in res for b in res
+                      if a.rid < b.rid and a.t_admit < b.t_done and b.t_admit < a.t_done)
+    assert overlapping > 0
+    for r, got in zip(reqs, res):
+        print(f"[c5] alone {r.rid} {r.kind} {r.shape}", flush=True)
+        alone = run_alone(r, cfg, tile=(14, 18), ni=8)
+        assert got.status == alone.status, r.rid
+        if r.kind == "solve":
+            assert got.status == 0, r.rid
+        assert got.compliance == alone.compliance and got.pcg_iters == alone.pcg_iters, r.rid
+        assert got.stress_max == alone.stress_max
+        if r.kind == "simp":
+            assert torch.equal(got.xPhys, alone.xPhys)

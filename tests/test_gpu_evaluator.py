@@ -54,10 +54,10 @@ def _evaluator(gpu, seed=0):
 
 
 def test_evaluate_batch_matches_one_by_one(gpu):
-    """Batched (FP32 Jacobi-PCG at 1e-6 in the scheduler, then FP64
-    refinement passes to 1e-8) against evaluate() (GPUVoxelFEM: FP64-refined
-    to 1e-8) on the same latent vectors: compliance and stress within 1e-6
-    relative; and the reference chain on the host for the voxel fields."""
+    """Batched (FP64 refinement to 1e-8 around FP32 Jacobi-PCG inner solves,
+    as scheduler jobs) against evaluate() (GPUVoxelFEM: FP64-refined to 1e-8)
+    on the same latent vectors: compliance and stress within 1e-6 relative;
+    and the reference chain on the host for the voxel fields."""
     from genpipeline_amd.problems import bo_latents
     from genpipeline_amd.schema import DesignParameters
     from genpipeline_amd.fem.voxel_fem import decode_batch

@@ -283,3 +283,24 @@ def test_refined_and_fp32_jobs_share_ticks(gpu):
     _same(S.result(2, st[2]), rc)
     torch.cuda.synchronize()
     S.close()
+
+
+def test_warm_compliance_job_equals_alone(gpu):
+    """Compliance solves warm-started from the previous displacement (an
+    option beside SIMPSolverGPU's zero start): the scheduled job equals
+    Problem.run with the option bit for bit, runs fewer PCG iterations than
+    the zero-start run, and ends within the tolerance of it."""
+    from genpipeline_amd.sched import Scheduler
+    P = _problem(gpu, 12, 6, 6, 7, law="clamp", oc="gpu", volfrac=0.35)
+    kw = dict(tol=1e-6, max_iter=5000, tol_schedule=True, early_stop=True,
+              compliance_each_iter=True, warm_start=True)
+    cold = _alone(P, (8, 32), 2, 25, **kw)
+    warm = _alone(P, (8, 32), 2, 25, warm_compliance=True, **kw)
+    assert warm.pcg_iters < cold.pcg_iters
+    assert abs(warm.compliance - cold.compliance) <= 1e-4 * abs(cold.compliance)
+    S = Scheduler([(12, 6, 6, 2)])
+    S.admit(0, P, n_iters=25, warm_compliance=True, **kw)
+    _, st = S.run()
+    _same(S.result(0, st[0]), warm)
+    torch.cuda.synchronize()
+    S.close()
