@@ -74,7 +74,7 @@ class ServeConfig:
     # solve_tol, whose accuracy stops near 1e-5 on solid/void fields)
     solve_refine: bool = True
     solve_tol: float = 1e-8
-    solve_inner_tol: float = 1e-2
+    solve_inner_tol: float = 1e-3
     solve_max_outer: int = 50
     # "simp" requests: the compliance solve after each update starts from the
     # previous displacement instead of zero (SIMPSolverGPU solves it from zero,
