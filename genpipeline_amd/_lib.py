@@ -121,6 +121,9 @@ SIGNATURES = {
     "gp_mc_count": [P, I32, I32, I32, F64, P, C.POINTER(I64), P],
     "gp_mc_emit": [P, I32, I32, I32, F64, F64, F64, F64, P, P, P],
     "gp_mc_tables": [P, P],
+    "gp_hex_mesh_workspace": [I32, I32, I32, C.POINTER(I64)],
+    "gp_hex_mesh_count": [P, I32, I32, I32, P, C.POINTER(I64), C.POINTER(I64), P],
+    "gp_hex_mesh_emit": [P, I32, I32, I32, P, P, P, P, P],
 }
 EXTRA = {"gp_last_error": ([], C.c_char_p), "gp_version": ([], C.c_int)}
 

@@ -1288,6 +1288,51 @@ int gp_mc_tables(int8_t* ntri, int8_t* tri) {
   return GP_OK;
 }
 
+// C3D8 mesh of the solid voxels (voxel_fem.py:120-146, mesher_simd.cpp:11-52).
+static int hex_check(int32_t D, int32_t H, int32_t W) {
+  if (D < 1 || H < 1 || W < 1) return fail(GP_ERR_ARG, "voxel grid must be >= 1 per axis");
+  if ((long long)D * H * W > (1LL << 28) || (long long)(D + 1) * (H + 1) * (W + 1) >= (1LL << 30))
+    return fail(GP_ERR_ARG, "voxel grid %dx%dx%d too large for 32-bit node numbers", D, H, W);
+  return GP_OK;
+}
+
+int gp_hex_mesh_workspace(int32_t D, int32_t H, int32_t W, int64_t* n_ints) {
+  if (!n_ints) return fail(GP_ERR_ARG, "bad argument");
+  int rc = hex_check(D, H, W);
+  if (rc) return rc;
+  *n_ints = 2ll * (hex_blocks(D, H, W) + 1) + (int64_t)(D + 1) * (H + 1) * (W + 1);
+  return GP_OK;
+}
+
+int gp_hex_mesh_count(const float* voxels, int32_t D, int32_t H, int32_t W, int32_t* workspace,
+                      int64_t* n_elem, int64_t* n_node, void* stream) {
+  if (!voxels || !workspace || !n_elem || !n_node) return fail(GP_ERR_ARG, "bad argument");
+  int rc = hex_check(D, H, W);
+  if (rc) return rc;
+  hipStream_t s = S(stream);
+  launch_hex_count(voxels, D, H, W, workspace, s);
+  LAUNCHCHK();
+  const int nb = hex_blocks(D, H, W);
+  int32_t tot[2] = {0, 0};
+  HIPCHK(hipMemcpyAsync(&tot[0], workspace + nb, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(&tot[1], workspace + 2 * nb + 1, sizeof(int32_t), hipMemcpyDeviceToHost,
+                        s));
+  HIPCHK(hipStreamSynchronize(s));
+  *n_elem = tot[0];
+  *n_node = tot[1];
+  return GP_OK;
+}
+
+int gp_hex_mesh_emit(const float* voxels, int32_t D, int32_t H, int32_t W, int32_t* workspace,
+                     int32_t* elems, int32_t* node_ijk, int32_t* elem_voxel, void* stream) {
+  if (!voxels || !workspace || !elems || !node_ijk) return fail(GP_ERR_ARG, "bad argument");
+  int rc = hex_check(D, H, W);
+  if (rc) return rc;
+  launch_hex_emit(voxels, D, H, W, workspace, elems, node_ijk, elem_voxel, S(stream));
+  LAUNCHCHK();
+  return GP_OK;
+}
+
 // Evaluator pre-processing (voxel_fem.py:452-462, pipeline_utils.py:55-76).
 int gp_voxel_prep(const float* in, int32_t batch, int32_t d, int32_t h, int32_t w, int32_t od,
                   int32_t oh, int32_t ow, const double* weights, int32_t radius,

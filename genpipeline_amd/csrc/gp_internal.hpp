@@ -391,6 +391,17 @@ void launch_mc_count(const double* d, int nx, int ny, int nz, double level, int*
 void launch_mc_emit(const double* d, int nx, int ny, int nz, double level, double sx, double sy,
                     double sz, const int* off, float* out, hipStream_t st);
 
+// C3D8 hex mesh of the solid voxels (> 0.5) of a D x H x W float32 field:
+// launch_hex_count fills the workspace (2 (nb + 1) offsets, then the
+// (D+1)(H+1)(W+1) node grid) and leaves the element total at ws[nb] and the
+// node total at ws[2 nb + 1]; launch_hex_emit writes 8 node numbers
+// (1-based) per element, 3 grid ints per node and, when elem_voxel is not
+// null, each element's voxel index.
+int hex_blocks(int D, int H, int W);
+void launch_hex_count(const float* v, int D, int H, int W, int* ws, hipStream_t st);
+void launch_hex_emit(const float* v, int D, int H, int W, int* ws, int* elems, int* node_ijk,
+                     int* elem_voxel, hipStream_t st);
+
 // Evaluator pre-processing of decoded fields: order-1 zoom to od x oh x ow,
 // a separable symmetric (2 radius + 1)-tap filter with reflect boundaries
 // along each axis, logistic sharpening. work and out: batch*od*oh*ow floats.

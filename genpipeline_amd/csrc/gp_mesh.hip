@@ -279,10 +279,190 @@ __global__ __launch_bounds__(kBlock) void k_mc_emit(const double* __restrict__ d
   }
 }
 
+// ------------------------------------------------- voxel hex mesh (C3D8)
+// The CalculiX path of the evaluator meshes every solid voxel (value > 0.5)
+// as one C3D8 element, in C order of the voxels, and numbers the grid
+// corners in the order the elements first touch them, corners in C3D8 order
+// (genpipeline/fem/voxel_fem.py:120-146). A corner's number is therefore
+// taken by the solid voxel of smallest C index among the up to eight around
+// it, and within that voxel by its corner position. Each pass is one thread
+// per voxel; counts become offsets by the same one-block scan as the
+// triangles.
+
+__device__ inline bool solid_at(const float* __restrict__ v, int D, int H, int W, int i, int j,
+                                int k) {
+  if ((unsigned)i >= (unsigned)D || (unsigned)j >= (unsigned)H || (unsigned)k >= (unsigned)W)
+    return false;
+  return v[((long long)i * H + j) * W + k] > 0.5f;
+}
+
+// Corner c of solid voxel (i, j, k) is numbered by this voxel when no solid
+// voxel of smaller C index shares that corner.
+__device__ inline bool owns_corner(const float* __restrict__ v, int D, int H, int W, int i, int j,
+                                   int k, int c) {
+  const int X = i + kCx[c], Y = j + kCy[c], Z = k + kCz[c];
+  const long long self = ((long long)i * H + j) * W + k;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        const int ui = X - 1 + a, uj = Y - 1 + b, uk = Z - 1 + e;
+        if (!solid_at(v, D, H, W, ui, uj, uk)) continue;
+        if (((long long)ui * H + uj) * W + uk < self) return false;
+      }
+  return true;
+}
+
+__device__ inline void voxel_of(long long q, int H, int W, int& i, int& j, int& k) {
+  k = (int)(q % W);
+  const long long r = q / W;
+  j = (int)(r % H);
+  i = (int)(r / H);
+}
+
+// Per 256-voxel block: solid voxels (cnt_e) and corners they number (cnt_n).
+__global__ __launch_bounds__(kBlock) void k_hex_count(const float* __restrict__ v, int D, int H,
+                                                      int W, int* __restrict__ cnt_e,
+                                                      int* __restrict__ cnt_n) {
+  const long long q = (long long)blockIdx.x * kBlock + threadIdx.x;
+  int ne = 0, nn = 0;
+  if (q < (long long)D * H * W) {
+    int i, j, k;
+    voxel_of(q, H, W, i, j, k);
+    if (v[q] > 0.5f) {
+      ne = 1;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) nn += owns_corner(v, D, H, W, i, j, k, c) ? 1 : 0;
+    }
+  }
+  __shared__ int sh[2][kBlock];
+  sh[0][threadIdx.x] = ne;
+  sh[1][threadIdx.x] = nn;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int se = 0, sn = 0;
+    for (int t = 0; t < kBlock; ++t) {
+      se += sh[0][t];
+      sn += sh[1][t];
+    }
+    cnt_e[blockIdx.x] = se;
+    cnt_n[blockIdx.x] = sn;
+  }
+}
+
+// Exclusive in-block ranks of a and b; returns them through ra, rb.
+__device__ inline void block_ranks2(int a, int b, int& ra, int& rb) {
+  __shared__ int sh[2][kBlock];
+  sh[0][threadIdx.x] = a;
+  sh[1][threadIdx.x] = b;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int sa = 0, sb = 0;
+    for (int t = 0; t < kBlock; ++t) {
+      const int xa = sh[0][t], xb = sh[1][t];
+      sh[0][t] = sa;
+      sh[1][t] = sb;
+      sa += xa;
+      sb += xb;
+    }
+  }
+  __syncthreads();
+  ra = sh[0][threadIdx.x];
+  rb = sh[1][threadIdx.x];
+}
+
+// Numbers the corners (1-based, in first-touch order) into the node grid
+// (D+1)(H+1)(W+1) and writes each new node's grid corner (3 ints); records
+// every solid voxel's C index in element order.
+__global__ __launch_bounds__(kBlock) void k_hex_nodes(const float* __restrict__ v, int D, int H,
+                                                      int W, const int* __restrict__ off_e,
+                                                      const int* __restrict__ off_n,
+                                                      int* __restrict__ node_grid,
+                                                      int* __restrict__ node_ijk,
+                                                      int* __restrict__ elem_voxel) {
+  const long long q = (long long)blockIdx.x * kBlock + threadIdx.x;
+  int i = 0, j = 0, k = 0, ne = 0, nn = 0, own = 0;
+  if (q < (long long)D * H * W) {
+    voxel_of(q, H, W, i, j, k);
+    if (v[q] > 0.5f) {
+      ne = 1;
+#pragma unroll
+      for (int c = 0; c < 8; ++c)
+        if (owns_corner(v, D, H, W, i, j, k, c)) {
+          own |= 1 << c;
+          ++nn;
+        }
+    }
+  }
+  int re, rn;
+  block_ranks2(ne, nn, re, rn);
+  if (!ne) return;
+  if (elem_voxel) elem_voxel[off_e[blockIdx.x] + re] = (int)q;
+  int id = off_n[blockIdx.x] + rn;  // 0-based
+#pragma unroll
+  for (int c = 0; c < 8; ++c)
+    if (own & (1 << c)) {
+      const int X = i + kCx[c], Y = j + kCy[c], Z = k + kCz[c];
+      node_grid[((long long)X * (H + 1) + Y) * (W + 1) + Z] = id + 1;
+      node_ijk[3ll * id] = X;
+      node_ijk[3ll * id + 1] = Y;
+      node_ijk[3ll * id + 2] = Z;
+      ++id;
+    }
+}
+
+// Eight node numbers per solid voxel, C3D8 corner order.
+__global__ __launch_bounds__(kBlock) void k_hex_elems(const float* __restrict__ v, int D, int H,
+                                                      int W, const int* __restrict__ off_e,
+                                                      const int* __restrict__ node_grid,
+                                                      int* __restrict__ elems) {
+  const long long q = (long long)blockIdx.x * kBlock + threadIdx.x;
+  int i = 0, j = 0, k = 0, ne = 0;
+  if (q < (long long)D * H * W) {
+    voxel_of(q, H, W, i, j, k);
+    ne = v[q] > 0.5f ? 1 : 0;
+  }
+  int re, unused;
+  block_ranks2(ne, 0, re, unused);
+  if (!ne) return;
+  const long long e = off_e[blockIdx.x] + re;
+#pragma unroll
+  for (int c = 0; c < 8; ++c)
+    elems[8 * e + c] =
+        node_grid[((long long)(i + kCx[c]) * (H + 1) + (j + kCy[c])) * (W + 1) + (k + kCz[c])];
+}
+
 std::mutex g_mc_mu;
 std::vector<int> g_mc_ready;  // per device: tables copied
 
 }  // namespace
+
+int hex_blocks(int D, int H, int W) {
+  return (int)(((long long)D * H * W + kBlock - 1) / kBlock);
+}
+
+// Workspace: off_e[nb + 1], off_n[nb + 1], node grid (D+1)(H+1)(W+1).
+void launch_hex_count(const float* v, int D, int H, int W, int* ws, hipStream_t st) {
+  const int nb = hex_blocks(D, H, W);
+  int* off_e = ws;
+  int* off_n = ws + nb + 1;
+  hipLaunchKernelGGL(k_hex_count, dim3(nb), dim3(kBlock), 0, st, v, D, H, W, off_e, off_n);
+  hipLaunchKernelGGL(k_mc_scan, dim3(1), dim3(kBlock), 0, st, off_e, nb);
+  hipLaunchKernelGGL(k_mc_scan, dim3(1), dim3(kBlock), 0, st, off_n, nb);
+}
+
+void launch_hex_emit(const float* v, int D, int H, int W, int* ws, int* elems, int* node_ijk,
+                     int* elem_voxel, hipStream_t st) {
+  const int nb = hex_blocks(D, H, W);
+  const int* off_e = ws;
+  const int* off_n = ws + nb + 1;
+  int* grid = ws + 2 * (nb + 1);
+  hipLaunchKernelGGL(k_hex_nodes, dim3(nb), dim3(kBlock), 0, st, v, D, H, W, off_e, off_n, grid,
+                     node_ijk, elem_voxel);
+  hipLaunchKernelGGL(k_hex_elems, dim3(nb), dim3(kBlock), 0, st, v, D, H, W, off_e, grid, elems);
+}
 
 int mc_blocks(int nx, int ny, int nz) {
   const long long ncube = (long long)(nx - 1) * (ny - 1) * (nz - 1);

@@ -26,10 +26,12 @@ to ``batch_solve_tol`` instead: about twice as fast, compliance within about
 1e-5 of the refined value on solid/void fields. Results and history entries
 are in parameter order.
 
-The CalculiX path (.inp writer, ccx run, .frd parser; :84-399) is not part
-of this library. With ``use_gpu=False``, or when the GPU solve fails, the
-result is the reference's no-solver penalty: stress and compliance
-FEM_SENTINEL, mass 1.0, ``failure_reason`` set (:530-536).
+With ``use_gpu=False``, or when the GPU solve fails, the evaluation takes
+the reference's CalculiX path (:502-557): the C3D8 deck of the solid voxels
+(``VoxelHexMesher``, meshed on the device), a ccx run and the .frd parse.
+Without a CalculiX executable the result is the reference's no-solver
+penalty: stress and compliance FEM_SENTINEL, mass 1.0, ``failure_reason``
+"no_ccx" (the deck, which nothing could read, is then not written).
 """
 from __future__ import annotations
 
@@ -86,9 +88,9 @@ def decode_to_voxels(zs, vae_model):
 
 
 class VoxelFEMEvaluator:
-    """Evaluator with the reference constructor (voxel_fem.py:406-438).
-    ``ccx_cmd`` and ``output_dir`` are accepted and kept for callers that
-    read them; nothing is written to ``output_dir``."""
+    """Evaluator with the reference constructor (voxel_fem.py:406-438):
+    ``ccx_cmd`` defaults to the CalculiX found on the system, decks of the
+    CalculiX path go to ``output_dir``."""
 
     batch_slots: int = 16           # concurrent solves of one grid size in a batch
     batch_refine: bool = True       # FP64 iterative refinement (GPUVoxelFEM's accuracy)
@@ -100,8 +102,10 @@ class VoxelFEMEvaluator:
     def __init__(self, ccx_cmd: str = None, output_dir: str = None, fixed_face: str = "x_min",
                  load_face: str = "x_max", force_n: float = 1000.0, vae_model=None,
                  use_gpu: bool = False):
-        self.ccx_cmd = ccx_cmd
+        from .voxel_hex import find_ccx
+        self.ccx_cmd = ccx_cmd or find_ccx()
         self.output_dir = Path(output_dir or "./optimisation_results/voxel_fem")
+        self.output_dir.mkdir(parents=True, exist_ok=True)
         self.fixed_face = fixed_face
         self.load_face = load_face
         self.force_n = force_n
@@ -122,18 +126,43 @@ class VoxelFEMEvaluator:
         except Exception:
             pass
 
+    def _ccx_path(self, voxels, bbox) -> Dict[str, float]:
+        """The reference's CalculiX branch (voxel_fem.py:502-557)."""
+        from ..pipeline_utils import is_valid_fem_result
+        from .voxel_hex import VoxelHexMesher
+        self._counter += 1
+        if self.ccx_cmd is None:
+            return dict(_NO_CCX)
+        inp = str(self.output_dir / f"voxel_fem_{self._counter:04d}.inp")
+        try:
+            VoxelHexMesher.voxels_to_inp(voxels, bbox=bbox, fixed_face=self.fixed_face,
+                                         load_face=self.load_face, force_n=self.force_n,
+                                         output_path=inp)
+        except ValueError as e:
+            logger.warning(f"[VoxelFEM] Mesh failed: {e}")
+            return {"stress": FEM_SENTINEL, "compliance": FEM_SENTINEL, "mass": 1.0,
+                    "failure_reason": "mesh_error"}
+        fem = VoxelHexMesher.run_ccx(inp, ccx_cmd=self.ccx_cmd)
+        if not is_valid_fem_result({"stress": fem["stress_max"]}):
+            return {"stress": fem["stress_max"], "compliance": fem["compliance"], "mass": 1.0,
+                    "failure_reason": fem.get("failure_reason", "fem_failed")}
+        v = np.asarray(voxels)
+        return {"stress": fem["stress_max"], "compliance": fem["compliance"],
+                "mass": _mass(float((v > 0.5).mean()), v.shape, bbox)}
+
     def evaluate_voxels(self, voxels: np.ndarray, bbox: dict = None) -> Dict[str, float]:
         """FEM metrics of an already decoded and smoothed voxel field."""
         if not self.use_gpu:
-            self._counter += 1
-            return dict(_NO_CCX)
+            return self._ccx_path(voxels, bbox)
         from .gpu_fem_solver import GPUVoxelFEM
         voxels = np.asarray(voxels)
         try:
             fem = GPUVoxelFEM(voxels=voxels, fixed_face=self.fixed_face, load_face=self.load_face,
                               force_n=self.force_n).solve()
-        except Exception as e:  # the reference falls back to ccx here; there is none
-            logger.warning(f"[VoxelFEM] GPU FEM failed: {e}")
+        except Exception as e:  # the reference falls back to ccx here
+            logger.warning(f"[VoxelFEM] GPU FEM failed: {e}, falling back to ccx")
+            if self.ccx_cmd is not None:
+                return self._ccx_path(voxels, bbox)
             self._counter += 1
             return dict(_GPU_ERR)
         result = {"stress": fem["stress_max"], "compliance": fem["compliance"],
@@ -144,7 +173,7 @@ class VoxelFEMEvaluator:
 
     def evaluate(self, z: np.ndarray, vae_model, bbox: dict = None) -> Dict[str, float]:
         """Decode z -> voxels -> FEM -> {stress, compliance, mass}."""
-        if not self.use_gpu:  # the penalty does not depend on the field
+        if not self.use_gpu and self.ccx_cmd is None:  # the penalty needs no field
             self._counter += 1
             return dict(_NO_CCX)
         vox = decode_to_voxels([z], vae_model)[0]
@@ -172,8 +201,11 @@ class VoxelFEMEvaluator:
         if not zs:
             return []
         if not self.use_gpu:
-            self._counter += len(zs)
-            return [dict(_NO_CCX) for _ in zs]
+            if self.ccx_cmd is None:
+                self._counter += len(zs)
+                return [dict(_NO_CCX) for _ in zs]
+            vox = decode_to_voxels(zs, self.vae_model).cpu().numpy()
+            return [self._ccx_path(v, bbox) for v in vox]
         from ..problems import Request
         vox = decode_to_voxels(zs, self.vae_model)
         shape = tuple(int(n) for n in vox.shape[1:])

@@ -360,6 +360,27 @@ int gp_mc_emit(const double* density, int32_t nx, int32_t ny, int32_t nz, double
                void* stream);
 int gp_mc_tables(int8_t* ntri, int8_t* tri);
 
+/* ---- C3D8 hex mesh of a voxel field ------------------------------------ *
+ * The mesh the CalculiX path writes to its .inp file
+ * (VoxelHexMesher.voxels_to_inp, genpipeline/fem/voxel_fem.py:120-146, with
+ * the solid-voxel scan get_solid_voxels_simd, cuda_kernels/__init__.py:
+ * 371-388 and mesher_simd.cpp:11-52): one element per voxel > 0.5 of a
+ * D x H x W float32 field (C order), elements in C order of their voxels;
+ * grid corners numbered from 1 in the order the elements first touch them,
+ * each element's corners in C3D8 order (0,0,0) (1,0,0) (1,1,0) (0,1,0)
+ * (0,0,1) (1,0,1) (1,1,1) (0,1,1).
+ * gp_hex_mesh_workspace: int32 entries of device workspace;
+ * gp_hex_mesh_count: fills it, returns the element and node counts
+ *   (synchronises the stream);
+ * gp_hex_mesh_emit: elems (n_elem x 8 node numbers), node_ijk (n_node x 3
+ *   grid corner indices, in node-number order), elem_voxel (n_elem voxel
+ *   C indices, or NULL).                                                    */
+int gp_hex_mesh_workspace(int32_t D, int32_t H, int32_t W, int64_t* n_ints);
+int gp_hex_mesh_count(const float* voxels, int32_t D, int32_t H, int32_t W, int32_t* workspace,
+                      int64_t* n_elem, int64_t* n_node, void* stream);
+int gp_hex_mesh_emit(const float* voxels, int32_t D, int32_t H, int32_t W, int32_t* workspace,
+                     int32_t* elems, int32_t* node_ijk, int32_t* elem_voxel, void* stream);
+
 /* ---- evaluator pre-processing ------------------------------------------ *
  * Decoded fields -> FEM voxel densities, on the device. Replaces the CPU
  * chain of VoxelFEMEvaluator.evaluate (genpipeline/fem/voxel_fem.py:452-462):
