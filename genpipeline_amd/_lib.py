@@ -116,6 +116,7 @@ SIGNATURES = {
     "gp_generic_sens": [P, P, P, P, I64, F64, P, P],
     "gp_generic_spmv": [P, P, P, P, I64, I64, F64, P, P],
     "gp_csr_pcg": [P, P, P, P, P, P, P, I64, I64, I32, F64, C.POINTER(GpPcgStats), P],
+    "gp_csr_pcg_noalloc": [P, P, P, P, P, P, P, I64, I64, I32, F64, C.POINTER(GpPcgStats), P],
     "gp_voxel_prep": [P, I32, I32, I32, I32, I32, I32, I32, P, I32, F64, P, P, P],
     "gp_mc_workspace": [I32, I32, I32, C.POINTER(I64)],
     "gp_mc_count": [P, I32, I32, I32, F64, P, C.POINTER(I64), P],

@@ -14,6 +14,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <utility>
@@ -1214,6 +1215,112 @@ int gp_simp_run(gp_problem* P, const gp_simp_opts* o, double* x, double* xPhys, 
 }
 
 // ------------------------------------------------------------ generic API
+}  // extern "C"
+
+namespace {
+
+// Workspace of the CSR Jacobi-PCG: four vectors of n floats, the scalars
+// and kVecBlocks partial sums.
+struct CsrPcgWork {
+  float *r, *z, *p, *Ap;
+  DevScalars* sc;
+  double* part;
+};
+
+// Jacobi-PCG on a CSR matrix with the reference's pcg_solve_cuda semantics
+// (simp_pcg_cuda.cu:66-124: r starts at b, fixed DOFs zeroed in r and A p,
+// ||r|| < tol ||b||). h: host copy of the scalars. Allocates nothing.
+int csr_pcg_run(const int64_t* rp, const int64_t* ci, const float* v, const float* diagonal,
+                const float* b, float* x, const int64_t* fixed, int64_t n_fixed, int64_t n,
+                int32_t max_iter, double tol, const CsrPcgWork& w, DevScalars* h,
+                gp_pcg_stats* st, hipStream_t s) {
+  float *r = w.r, *z = w.z, *p = w.p, *Ap = w.Ap;
+  auto dot = [&](const float* a, const float* c, double* out) -> int {
+    Parts pp{};
+    pp.S = w.part;
+    launch_dot<float, float>(a, c, n, pp, w.sc, s);
+    HIPCHK(hipMemcpyAsync(h, w.sc, sizeof(DevScalars), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    *out = h->sum;
+    return GP_OK;
+  };
+  // the scalars start at zero: a fill kernel, not a memset node
+  launch_zero_scalars(w.sc, s);
+  if (hipMemcpyAsync(r, b, n * 4, hipMemcpyDeviceToDevice, s) != hipSuccess)
+    return fail(GP_ERR_HIP, "init copy failed");
+  launch_zero_list(r, fixed, n_fixed, s);
+  launch_vec(z, r, diagonal, 0.f, n, 2, s);
+  if (hipMemcpyAsync(p, z, n * 4, hipMemcpyDeviceToDevice, s) != hipSuccess)
+    return fail(GP_ERR_HIP, "init copy failed");
+  int rc = GP_OK;
+  double rz, bn2;
+  if ((rc = dot(r, z, &rz)) || (rc = dot(b, b, &bn2))) return rc;
+  const double bnorm = std::sqrt(bn2);
+  int it = 0;
+  double rr = 0;
+  for (; it < max_iter; ++it) {
+    launch_csr_spmv(rp, ci, v, p, Ap, n, s);
+    launch_zero_list(Ap, fixed, n_fixed, s);
+    double den;
+    if ((rc = dot(p, Ap, &den))) return rc;
+    if (std::fabs(den) < 1e-30) break;
+    const float alpha = (float)(rz / den);
+    launch_vec(x, p, nullptr, alpha, n, 1, s);
+    launch_vec(r, Ap, nullptr, -alpha, n, 1, s);
+    launch_zero_list(r, fixed, n_fixed, s);
+    if ((rc = dot(r, r, &rr))) return rc;
+    if (std::sqrt(rr) < tol * bnorm) {
+      ++it;
+      break;
+    }
+    launch_vec(z, r, diagonal, 0.f, n, 2, s);
+    double rzn;
+    if ((rc = dot(r, z, &rzn))) return rc;
+    launch_vec(p, z, nullptr, (float)(rzn / rz), n, 0, s);
+    rz = rzn;
+  }
+  st->iters = it;
+  st->relres = bnorm > 0 ? std::sqrt(rr) / bnorm : 0.0;
+  st->status = (it >= max_iter) ? GP_NOT_CONVERGED : GP_OK;
+  if (hipStreamSynchronize(s) != hipSuccess) return fail(GP_ERR_HIP, "sync failed");
+  return GP_OK;
+}
+
+// Static device scratch of the no-allocation CSR solve: the scalars, the
+// partials and four vectors of kNoAllocMaxN floats (16 MiB + 8 KiB per GPU,
+// held until the process ends). In host code `scratch` names a host-side
+// stand-in; the device address comes from hipGetSymbolAddress, once per
+// device, and only that address is used.
+constexpr int64_t kNoAllocMaxN = 1 << 20;
+constexpr size_t kScratchScalars = 256;
+constexpr size_t kScratchParts = (kVecBlocks * sizeof(double) + 255) & ~size_t(255);
+static_assert(sizeof(DevScalars) <= kScratchScalars, "scalars fit their slot");
+static __device__ __align__(256) char scratch[kScratchScalars + kScratchParts +
+                                              4 * kNoAllocMaxN * sizeof(float)];
+
+std::mutex g_scratch_mu;
+std::vector<char*> g_scratch_base;  // device address of `scratch` per device
+
+int scratch_base(char** out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if ((int)g_scratch_base.size() <= dev) g_scratch_base.resize(dev + 1, nullptr);
+  if (!g_scratch_base[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(GP_ERR_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch_base[dev] = static_cast<char*>(p);
+  }
+  *out = g_scratch_base[dev];
+  return GP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int gp_generic_sens(const float* xPhys, const float* u, const float* ke, const int32_t* edof,
                     int64_t ne, double penal, float* dc, void* stream) {
   if (!xPhys || !u || !ke || !edof || !dc || ne < 0) return fail(GP_ERR_ARG, "bad argument");
@@ -1365,76 +1472,60 @@ int gp_csr_pcg(const int64_t* rp, const int64_t* ci, const float* v, const float
   std::memset(st, 0, sizeof(*st));
   if (n == 0) return GP_OK;
   hipStream_t s = S(stream);
-  float *r = nullptr, *z = nullptr, *p = nullptr, *Ap = nullptr;
-  DevScalars* sc = nullptr;
-  double* part = nullptr;
+  CsrPcgWork w{};
   DevScalars* h = nullptr;
   int rc = GP_OK;
   auto cleanup = [&]() {
-    for (void* q : {(void*)r, (void*)z, (void*)p, (void*)Ap, (void*)sc, (void*)part})
+    for (void* q : {(void*)w.r, (void*)w.z, (void*)w.p, (void*)w.Ap, (void*)w.sc, (void*)w.part})
       if (q) hipFree(q);
     if (h) hipHostFree(h);
   };
-  auto dot = [&](const float* a, const float* c, double* out) -> int {
-    Parts pp{};
-    pp.S = part;
-    launch_dot<float, float>(a, c, n, pp, sc, s);
-    HIPCHK(hipMemcpyAsync(h, sc, sizeof(DevScalars), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    *out = h->sum;
-    return GP_OK;
-  };
   do {
-    if ((rc = alloc((void**)&r, n * 4)) || (rc = alloc((void**)&z, n * 4)) ||
-        (rc = alloc((void**)&p, n * 4)) || (rc = alloc((void**)&Ap, n * 4)) ||
-        (rc = alloc((void**)&sc, sizeof(DevScalars))) ||
-        (rc = alloc((void**)&part, kVecBlocks * sizeof(double))))
+    if ((rc = alloc((void**)&w.r, n * 4)) || (rc = alloc((void**)&w.z, n * 4)) ||
+        (rc = alloc((void**)&w.p, n * 4)) || (rc = alloc((void**)&w.Ap, n * 4)) ||
+        (rc = alloc((void**)&w.sc, sizeof(DevScalars))) ||
+        (rc = alloc((void**)&w.part, kVecBlocks * sizeof(double))))
       break;
     if (hipHostMalloc((void**)&h, sizeof(DevScalars)) != hipSuccess) {
       rc = fail(GP_ERR_OOM, "hipHostMalloc failed");
       break;
     }
-    if (hipMemsetAsync(sc, 0, sizeof(DevScalars), s) != hipSuccess ||
-        hipMemcpyAsync(r, b, n * 4, hipMemcpyDeviceToDevice, s) != hipSuccess) {
-      rc = fail(GP_ERR_HIP, "init copy failed");
-      break;
-    }
-    launch_zero_list(r, fixed, n_fixed, s);
-    launch_vec(z, r, diagonal, 0.f, n, 2, s);
-    hipMemcpyAsync(p, z, n * 4, hipMemcpyDeviceToDevice, s);
-    double rz, bn2;
-    if ((rc = dot(r, z, &rz)) || (rc = dot(b, b, &bn2))) break;
-    const double bnorm = std::sqrt(bn2);
-    int it = 0;
-    double rr = 0;
-    for (; it < max_iter; ++it) {
-      launch_csr_spmv(rp, ci, v, p, Ap, n, s);
-      launch_zero_list(Ap, fixed, n_fixed, s);
-      double den;
-      if ((rc = dot(p, Ap, &den))) break;
-      if (std::fabs(den) < 1e-30) break;
-      const float alpha = (float)(rz / den);
-      launch_vec(x, p, nullptr, alpha, n, 1, s);
-      launch_vec(r, Ap, nullptr, -alpha, n, 1, s);
-      launch_zero_list(r, fixed, n_fixed, s);
-      if ((rc = dot(r, r, &rr))) break;
-      if (std::sqrt(rr) < tol * bnorm) {
-        ++it;
-        break;
-      }
-      launch_vec(z, r, diagonal, 0.f, n, 2, s);
-      double rzn;
-      if ((rc = dot(r, z, &rzn))) break;
-      launch_vec(p, z, nullptr, (float)(rzn / rz), n, 0, s);
-      rz = rzn;
-    }
-    st->iters = it;
-    st->relres = bnorm > 0 ? std::sqrt(rr) / bnorm : 0.0;
-    st->status = (it >= max_iter) ? GP_NOT_CONVERGED : GP_OK;
-    if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = fail(GP_ERR_HIP, "sync failed");
+    rc = csr_pcg_run(rp, ci, v, diagonal, b, x, fixed, n_fixed, n, max_iter, tol, w, h, st, s);
   } while (false);
   cleanup();
   return rc;
+}
+
+// No counterpart in the reference: gp_csr_pcg for callers that cannot
+// allocate (code under graph capture, tight serving loops). The four PCG
+// vectors, the scalars and the partial sums live in the library's static
+// device array; n is limited to kNoAllocMaxN.
+int gp_csr_pcg_noalloc(const int64_t* rp, const int64_t* ci, const float* v,
+                       const float* diagonal, const float* b, float* x, const int64_t* fixed,
+                       int64_t n_fixed, int64_t n, int32_t max_iter, double tol,
+                       gp_pcg_stats* st, void* stream) {
+  if (!rp || !ci || !v || !diagonal || !b || !x || !st || n < 0 || n_fixed < 0 ||
+      (n_fixed > 0 && !fixed))
+    return fail(GP_ERR_ARG, "bad argument");
+  if (n > kNoAllocMaxN)
+    return fail(GP_ERR_ARG, "n = %lld exceeds the no-allocation limit of %lld unknowns",
+                (long long)n, (long long)kNoAllocMaxN);
+  std::memset(st, 0, sizeof(*st));
+  if (n == 0) return GP_OK;
+  char* base = nullptr;
+  int rc = scratch_base(&base);
+  if (rc) return rc;
+  CsrPcgWork w{};
+  w.sc = reinterpret_cast<DevScalars*>(base);
+  w.part = reinterpret_cast<double*>(base + kScratchScalars);
+  float* vec = reinterpret_cast<float*>(base + kScratchScalars + kScratchParts);
+  w.r = vec;
+  w.z = vec + kNoAllocMaxN;
+  w.p = vec + 2 * kNoAllocMaxN;
+  w.Ap = vec + 3 * kNoAllocMaxN;
+  DevScalars h{};  // pageable: read back with a synchronising copy
+  return csr_pcg_run(rp, ci, v, diagonal, b, x, fixed, n_fixed, n, max_iter, tol, w, &h, st,
+                     S(stream));
 }
 
 }  // extern "C"

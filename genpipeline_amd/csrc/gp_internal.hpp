@@ -372,6 +372,8 @@ template <typename TA, typename T>
 void launch_dot(const TA* a, const T* b, long long n, Parts parts, DevScalars* sc,
                 hipStream_t st);
 void launch_reset_flags(DevScalars* sc, hipStream_t st);
+// every field of *sc to zero (a fill kernel, so it can sit in a captured graph)
+void launch_zero_scalars(DevScalars* sc, hipStream_t st);
 void launch_zero_list(float* v, const int64_t* idx, long long n, hipStream_t st);
 void launch_vec(float* y, const float* x, const float* d, float a, long long n, int mode,
                 hipStream_t st);

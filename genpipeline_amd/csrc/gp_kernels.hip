@@ -565,6 +565,14 @@ void launch_dot(const TA* a, const T* b, long long n, Parts parts, DevScalars* s
 }
 
 __global__ void k_reset_flags(DevScalars* sc) { sc->nan_flag = 0; }
+
+__global__ void k_zero_scalars(DevScalars* sc) {
+  constexpr int n = (int)(sizeof(DevScalars) / sizeof(int));
+  for (int i = threadIdx.x; i < n; i += blockDim.x) reinterpret_cast<int*>(sc)[i] = 0;
+}
+void launch_zero_scalars(DevScalars* sc, hipStream_t st) {
+  hipLaunchKernelGGL(k_zero_scalars, dim3(1), dim3(64), 0, st, sc);
+}
 void launch_reset_flags(DevScalars* sc, hipStream_t st) {
   hipLaunchKernelGGL(k_reset_flags, dim3(1), dim3(1), 0, st, sc);
 }

@@ -338,6 +338,19 @@ int gp_csr_pcg(const int64_t* row_ptr, const int64_t* col_idx, const float* valu
                int64_t n_fixed, int64_t n, int32_t max_iter, double tol,
                gp_pcg_stats* stats, void* stream);
 
+/* gp_csr_pcg for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). No
+ * counterpart in the reference. The same arguments, kernels and results as
+ * gp_csr_pcg; its four PCG vectors, scalars and partial sums live in a
+ * static device array of the library (16 MiB + 8 KiB per GPU, held until
+ * the process ends), so n is at most 1048576 (GP_ERR_ARG beyond). Calls
+ * that overlap on one GPU share that array and race; calls in sequence do
+ * not (the scalars are cleared by a kernel at each call).                  */
+int gp_csr_pcg_noalloc(const int64_t* row_ptr, const int64_t* col_idx, const float* values,
+                       const float* diagonal, const float* b, float* x, const int64_t* fixed,
+                       int64_t n_fixed, int64_t n, int32_t max_iter, double tol,
+                       gp_pcg_stats* stats, void* stream);
+
 /* ---- isosurface ---------------------------------------------------------- *
  * Triangles of the surface density = level of an nx x ny x nz float64 field
  * (C order), the marching-cubes step of density_to_stl
