@@ -31,6 +31,8 @@ import sys
 import time
 from pathlib import Path
 
+import numpy as np
+
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
@@ -50,6 +52,9 @@ def parse():
     ap.add_argument("--no-extra", action="store_true",
                     help="default workload: skip the 48-problem batch and serving summaries")
     ap.add_argument("--cpu-oc-iters", type=int, default=2)
+    ap.add_argument("--cpu-scipy", action="store_true",
+                    help="also time SciPy CSR Jacobi-PCG at the headline grid (about a minute "
+                         "of assembly and iterations)")
     ap.add_argument("--traffic-file",
                     default=str(ROOT / "profiles" / "k_cg_modal_64x64x64_traffic.json"))
     ap.add_argument("--calib-mib", type=int, default=0,
@@ -584,6 +589,58 @@ def accuracy_vs_fp64(P, w, r_head):
     return out
 
 
+def cpu_reference_literal_c1():
+    """The reference CPU solver's own method on its CPU-runnable configuration
+    (32x16x16, the reference's element table, clamp law, SciPy SuperLU, one
+    thread by construction; simp_solver.py:195-213), through the oracle:
+    one direct solve at the uniform start density, timed. A run of C1 is 51
+    such solves (50 OC iterations and the final compliance solve)."""
+    from oracle import simp_oracle as O
+    nx, ny, nz = 32, 16, 16
+    P = O.reference_problem(nx, ny, nz, volfrac=0.3, ke="ref", law="clamp", oc="cpu")
+    x = np.full(P.n_elem, 0.3)
+    P.setup()
+    t0 = time.perf_counter()
+    P.solve_u(x, "direct")
+    dt = time.perf_counter() - t0
+    return {"seconds_per_solve": dt, "solves_per_run": 51, "runs_per_s": 1.0 / (51 * dt),
+            "cores": 1, "kind": "reference-literal (oracle: reference table, SciPy SuperLU)",
+            "grid": [nx, ny, nz]}
+
+
+def cpu_scipy_pcg(w, gpu_pcg_iters, n_iter=30):
+    """The configuration-literal CPU method at the headline grid: FP64
+    Jacobi-PCG on SciPy's assembled CSR K (the oracle's assembly, the
+    reference's SpMV path; one core), timed over n_iter PCG iterations at
+    the uniform start density and scaled by the GPU solve's PCG count.
+    Assembly is reported separately as set-up."""
+    import scipy.sparse.linalg as spla
+    from oracle import simp_oracle as O
+    sp = w.spec
+    KE = O.ke_hex8(sp.nu)
+    ed = O.edof_matrix(sp.nx, sp.ny, sp.nz)
+    Ee = O.stiffness(np.full(sp.n_elem, sp.volfrac), sp.law, sp.penal, sp.E0, sp.Emin)
+    t0 = time.perf_counter()
+    K = O.assemble_K(KE, Ee, ed, sp.n_dof)
+    free = np.ones(sp.n_dof, bool)
+    free[w.fixed] = False
+    Kf = K[free][:, free].tocsr()
+    t_setup = time.perf_counter() - t0
+    d = 1.0 / Kf.diagonal()
+    b = w.f[free]
+    M = spla.LinearOperator(Kf.shape, matvec=lambda r: d * r)
+    t0 = time.perf_counter()
+    spla.cg(Kf, b, rtol=1e-30, maxiter=n_iter, M=M)
+    dt = time.perf_counter() - t0
+    per = dt / n_iter
+    return {"seconds_per_pcg_iter": per, "setup_seconds": t_setup, "sample_pcg_iters": n_iter,
+            "solves_per_s": 1.0 / (per * gpu_pcg_iters), "cores": 1,
+            "kind": "config-literal (SciPy CSR Jacobi-PCG, FP64)",
+            "sample": f"{n_iter} scipy.sparse.linalg.cg iterations on the assembled free-DOF CSR "
+                      f"K at uniform density, scaled by the GPU solve's {gpu_pcg_iters} PCG "
+                      "iterations"}
+
+
 def _cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -728,6 +785,12 @@ def main():
     if not a.no_cpu:
         print("[bench] cpu baseline sample", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(w, r.pcg_iters, w.n_iters, a.cpu_oc_iters)
+        try:
+            out["cpu_baseline"]["reference_literal_c1"] = cpu_reference_literal_c1()
+        except Exception as e:
+            out["cpu_baseline"]["reference_literal_c1"] = {"error": f"{type(e).__name__}: {e}"}
+        if a.cpu_scipy:
+            out["cpu_baseline"]["scipy_csr_pcg"] = cpu_scipy_pcg(w, r.pcg_iters)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

@@ -113,8 +113,9 @@ SIGNATURES = {
     "gp_sched_tick": [P, I64, P],
     "gp_sched_status": [P, P, P],
     "gp_sched_run": [P, I64, C.POINTER(I64), P, P],
-    "gp_generic_sens": [P, P, P, P, I64, F64, P, P],
-    "gp_generic_spmv": [P, P, P, P, I64, I64, F64, P, P],
+    "gp_generic_sens": [P, P, P, P, I64, F64, I32, P, P],
+    "gp_generic_spmv_workspace": [I64, I64, C.POINTER(I64)],
+    "gp_generic_spmv": [P, P, P, P, I64, I64, F64, I32, P, P, P],
     "gp_csr_pcg": [P, P, P, P, P, P, P, I64, I64, I32, F64, C.POINTER(GpPcgStats), P],
     "gp_csr_pcg_noalloc": [P, P, P, P, P, P, P, I64, I64, I32, F64, C.POINTER(GpPcgStats), P],
     "gp_voxel_prep": [P, I32, I32, I32, I32, I32, I32, I32, P, I32, F64, P, P, P],

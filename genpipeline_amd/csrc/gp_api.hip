@@ -837,7 +837,8 @@ int gp_problem_create(const gp_spec* spec, gp_problem** out) {
   launch_hs(g, P->st, P->Hs, nullptr);
   launch_oc_weights(g, P->st, P->Hs, nullptr, P->w, nullptr);
   e = hipGetLastError();
-  if (e == hipSuccess) e = hipDeviceSynchronize();
+  // the set-up ran on the null stream: wait for that stream only
+  if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
   if (e != hipSuccess) return cleanup_fail(fail(GP_ERR_HIP, "init kernels: %s", hipGetErrorString(e)));
   *out = P;
   return GP_OK;
@@ -846,7 +847,7 @@ int gp_problem_create(const gp_spec* spec, gp_problem** out) {
 int gp_problem_destroy(gp_problem* P) {
   if (!P) return GP_OK;
   hipSetDevice(P->device);
-  hipDeviceSynchronize();
+  // hipFree waits for the work that may still use these buffers
   void* dev[] = {P->arena, P->pres, P->ws32, P->ws64, P->ws16, P->r64, P->y64, P->d32, P->E,
                  P->dc, P->dcf, P->xnew, P->xphys_new, P->ce_sum};
   for (void* p : dev)
@@ -905,7 +906,7 @@ int gp_problem_set_preserved(gp_problem* P, const uint8_t* mask) {
   }
   launch_oc_weights(g, P->st, P->Hs, P->pres, P->w, nullptr);
   LAUNCHCHK();
-  HIPCHK(hipDeviceSynchronize());
+  HIPCHK(hipStreamSynchronize(nullptr));  // the weights were formed on the null stream
   return GP_OK;
 }
 
@@ -1321,21 +1322,41 @@ int scratch_base(char** out) {
 
 extern "C" {
 
-int gp_generic_sens(const float* xPhys, const float* u, const float* ke, const int32_t* edof,
-                    int64_t ne, double penal, float* dc, void* stream) {
+int gp_generic_sens(const void* xPhys, const void* u, const void* ke, const int32_t* edof,
+                    int64_t ne, double penal, int32_t dtype, void* dc, void* stream) {
   if (!xPhys || !u || !ke || !edof || !dc || ne < 0) return fail(GP_ERR_ARG, "bad argument");
+  if (dtype != GP_F32 && dtype != GP_F64) return fail(GP_ERR_ARG, "dtype must be GP_F32 or GP_F64");
   if (ne == 0) return GP_OK;
-  launch_generic_sens(xPhys, u, ke, edof, ne, penal, dc, S(stream));
+  if (dtype == GP_F32)
+    launch_generic_sens<float>((const float*)xPhys, (const float*)u, (const float*)ke, edof, ne,
+                               penal, (float*)dc, S(stream));
+  else
+    launch_generic_sens<double>((const double*)xPhys, (const double*)u, (const double*)ke, edof,
+                                ne, penal, (double*)dc, S(stream));
   LAUNCHCHK();
   return GP_OK;
 }
 
-int gp_generic_spmv(const float* xPhys, const float* u, const float* ke, const int32_t* edof,
-                    int64_t ne, int64_t ndof, double penal, float* y, void* stream) {
-  if (!xPhys || !u || !ke || !edof || !y || ne < 0 || ndof < 0) return fail(GP_ERR_ARG, "bad argument");
-  HIPCHK(hipMemsetAsync(y, 0, ndof * sizeof(float), S(stream)));
-  if (ne == 0) return GP_OK;
-  launch_generic_spmv(xPhys, u, ke, edof, ne, penal, y, S(stream));
+int gp_generic_spmv_workspace(int64_t ne, int64_t ndof, int64_t* n_ints) {
+  if (ne < 0 || ndof < 0 || !n_ints) return fail(GP_ERR_ARG, "bad argument");
+  if (24 * ne + 3 * ndof >= (1LL << 31)) return fail(GP_ERR_ARG, "system too large");
+  *n_ints = generic_spmv_ws_ints(ne, ndof);
+  return GP_OK;
+}
+
+int gp_generic_spmv(const void* xPhys, const void* u, const void* ke, const int32_t* edof,
+                    int64_t ne, int64_t ndof, double penal, int32_t dtype, int32_t* workspace,
+                    void* y, void* stream) {
+  if (!xPhys || !u || !ke || !edof || !y || !workspace || ne < 0 || ndof < 1)
+    return fail(GP_ERR_ARG, "bad argument");
+  if (dtype != GP_F32 && dtype != GP_F64) return fail(GP_ERR_ARG, "dtype must be GP_F32 or GP_F64");
+  if (24 * ne + 3 * ndof >= (1LL << 31)) return fail(GP_ERR_ARG, "system too large");
+  if (dtype == GP_F32)
+    launch_generic_spmv<float>((const float*)xPhys, (const float*)u, (const float*)ke, edof, ne,
+                               ndof, penal, workspace, (float*)y, S(stream));
+  else
+    launch_generic_spmv<double>((const double*)xPhys, (const double*)u, (const double*)ke, edof,
+                                ne, ndof, penal, workspace, (double*)y, S(stream));
   LAUNCHCHK();
   return GP_OK;
 }
@@ -1729,7 +1750,7 @@ int gp_batch_create(gp_problem* const* probs, int32_t n, int32_t tj, int32_t tk,
 int gp_batch_destroy(gp_batch* B) {
   if (!B) return GP_OK;
   hipSetDevice(B->device);
-  hipDeviceSynchronize();
+  // hipFree waits for the work that may still use these buffers
   if (B->d_probs) hipFree(B->d_probs);
   if (B->d_off) hipFree(B->d_off);
   if (B->d_status) hipFree(B->d_status);
@@ -2010,6 +2031,8 @@ int gp_sched_create(int32_t n_slots, const int32_t* slot_dims, int32_t tj, int32
 int gp_sched_destroy(gp_sched* Q) {
   if (!Q) return GP_OK;
   hipSetDevice(Q->device);
+  // tick graphs may still be executing on the caller's streams, which the
+  // scheduler does not know: they end before the graphs are destroyed
   hipDeviceSynchronize();
   sched_free_graph(Q);
   if (Q->cap) hipStreamDestroy(Q->cap);

@@ -410,10 +410,15 @@ void launch_hex_emit(const float* v, int D, int H, int W, int* ws, int* elems, i
 void launch_voxel_prep(const float* in, int batch, int d, int h, int w, int od, int oh, int ow,
                        const double* weights, int radius, double sharpness, float* work,
                        float* out, hipStream_t st);
-void launch_generic_sens(const float* xPhys, const float* u, const float* ke, const int32_t* edof,
-                         long long ne, double penal, float* dc, hipStream_t st);
-void launch_generic_spmv(const float* xPhys, const float* u, const float* ke, const int32_t* edof,
-                         long long ne, double penal, float* y, hipStream_t st);
+template <typename T>
+void launch_generic_sens(const T* xPhys, const T* u, const T* ke, const int32_t* edof,
+                         long long ne, double penal, T* dc, hipStream_t st);
+// deterministic gather-form fused SpMV; ws: generic_spmv_ws_ints(ne, ndof) ints
+long long generic_spmv_ws_ints(long long ne, long long ndof);
+template <typename T>
+void launch_generic_spmv(const T* xPhys, const T* u, const T* ke, const int32_t* edof,
+                         long long ne, long long ndof, double penal, int* ws, T* y,
+                         hipStream_t st);
 void launch_csr_spmv(const int64_t* rp, const int64_t* ci, const float* v, const float* x, float* y,
                      long long n, hipStream_t st);
 

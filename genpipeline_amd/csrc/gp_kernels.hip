@@ -566,6 +566,11 @@ void launch_dot(const TA* a, const T* b, long long n, Parts parts, DevScalars* s
 
 __global__ void k_reset_flags(DevScalars* sc) { sc->nan_flag = 0; }
 
+__global__ void k_fill_int(int* __restrict__ v, long long n, int val) {
+  const long long i = blockIdx.x * (long long)kBlock + threadIdx.x;
+  if (i < n) v[i] = val;
+}
+
 __global__ void k_zero_scalars(DevScalars* sc) {
   constexpr int n = (int)(sizeof(DevScalars) / sizeof(int));
   for (int i = threadIdx.x; i < n; i += blockDim.x) reinterpret_cast<int*>(sc)[i] = 0;
@@ -580,12 +585,16 @@ void launch_reset_flags(DevScalars* sc, hipStream_t st) {
 // ------------------------------------------------- generic-table kernels
 // Reference semantics (cuda_kernels/simp_sensitivity_ptx.cu:163-270, 301-373):
 // arbitrary edof rows, E = max(x,1e-3)^p, FP64 element arithmetic.
-__global__ __launch_bounds__(kBlock) void k_generic_sens(const float* __restrict__ xPhys,
-                                                        const float* __restrict__ u,
-                                                        const float* __restrict__ ke,
+// The reference kernel API (cuda_kernels/__init__.py:286-368) on explicit
+// tables: element products in FP64 whatever the storage type T (float or
+// double, the reference kernels take double).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_generic_sens(const T* __restrict__ xPhys,
+                                                        const T* __restrict__ u,
+                                                        const T* __restrict__ ke,
                                                         const int32_t* __restrict__ edof,
                                                         long long ne, double p,
-                                                        float* __restrict__ dc) {
+                                                        T* __restrict__ dc) {
   const long long e = blockIdx.x * (long long)kBlock + threadIdx.x;
   if (e >= ne) return;
   double ue[24];
@@ -599,39 +608,171 @@ __global__ __launch_bounds__(kBlock) void k_generic_sens(const float* __restrict
     for (int b = 0; b < 24; ++b) t += (double)ke[a * 24 + b] * ue[b];
     ce += ue[a] * t;
   }
-  dc[e] = (float)(-p * pow(fmax((double)xPhys[e], 1e-3), p - 1.0) * ce);
+  dc[e] = (T)(-p * pow(fmax((double)xPhys[e], 1e-3), p - 1.0) * ce);
 }
-void launch_generic_sens(const float* xPhys, const float* u, const float* ke, const int32_t* edof,
-                         long long ne, double penal, float* dc, hipStream_t st) {
-  hipLaunchKernelGGL(k_generic_sens, dim3(grid_blocks(ne)), dim3(kBlock), 0, st, xPhys, u, ke,
-                     edof, ne, penal, dc);
+template <typename T>
+void launch_generic_sens(const T* xPhys, const T* u, const T* ke, const int32_t* edof,
+                         long long ne, double penal, T* dc, hipStream_t st) {
+  hipLaunchKernelGGL((k_generic_sens<T>), dim3(grid_blocks(ne)), dim3(kBlock), 0, st, xPhys, u,
+                     ke, edof, ne, penal, dc);
 }
+template void launch_generic_sens<float>(const float*, const float*, const float*, const int32_t*,
+                                         long long, double, float*, hipStream_t);
+template void launch_generic_sens<double>(const double*, const double*, const double*,
+                                          const int32_t*, long long, double, double*, hipStream_t);
 
-__global__ __launch_bounds__(kBlock) void k_generic_spmv(const float* __restrict__ xPhys,
-                                                        const float* __restrict__ u,
-                                                        const float* __restrict__ ke,
-                                                        const int32_t* __restrict__ edof,
-                                                        long long ne, double p,
-                                                        float* __restrict__ y) {
-  const long long e = blockIdx.x * (long long)kBlock + threadIdx.x;
-  if (e >= ne) return;
-  const double Ee = pow(fmax((double)xPhys[e], 1e-3), p);
-  double ue[24];
-#pragma unroll
-  for (int a = 0; a < 24; ++a) ue[a] = (double)u[edof[e * 24 + a]];
-#pragma unroll 4
-  for (int a = 0; a < 24; ++a) {
-    double t = 0.0;
-#pragma unroll
-    for (int b = 0; b < 24; ++b) t += (double)ke[a * 24 + b] * ue[b];
-    atomicAdd(&y[edof[e * 24 + a]], (float)(Ee * t));
+// y = sum_e max(x_e, 1e-3)^p KE u_e scattered by an arbitrary edof, in gather
+// form so that every DOF sums its terms in one fixed order (element, then
+// local index). The DOF -> (element, local index) lists are built on the
+// device for each call: integer counts, an exclusive scan, a fill, and a
+// per-DOF sort of its (short) list by key 24 e + a.
+__global__ void k_t_count(const int32_t* __restrict__ edof, long long nk, int* __restrict__ cnt) {
+  const long long q = blockIdx.x * (long long)kBlock + threadIdx.x;
+  if (q < nk) atomicAdd(&cnt[edof[q]], 1);
+}
+// per-block sums of cnt[0..n) into bs[blk]
+__global__ __launch_bounds__(kBlock) void k_t_bsum(const int* __restrict__ cnt, long long n,
+                                                  int* __restrict__ bs) {
+  const long long q = blockIdx.x * (long long)kBlock + threadIdx.x;
+  __shared__ int sh[kBlock];
+  sh[threadIdx.x] = q < n ? cnt[q] : 0;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int i = 0; i < kBlock; ++i) t += sh[i];
+    bs[blockIdx.x] = t;
   }
 }
-void launch_generic_spmv(const float* xPhys, const float* u, const float* ke, const int32_t* edof,
-                         long long ne, double penal, float* y, hipStream_t st) {
-  hipLaunchKernelGGL(k_generic_spmv, dim3(grid_blocks(ne)), dim3(kBlock), 0, st, xPhys, u, ke,
-                     edof, ne, penal, y);
+// exclusive scan of bs[0..nb) in place (one block, contiguous chunks)
+__global__ __launch_bounds__(kBlock) void k_t_bscan(int* __restrict__ bs, int nb) {
+  __shared__ int sh[kBlock];
+  const int t = threadIdx.x;
+  const int chunk = (nb + kBlock - 1) / kBlock;
+  const int b0 = t * chunk, b1 = b0 + chunk < nb ? b0 + chunk : nb;
+  int s = 0;
+  for (int b = b0; b < b1; ++b) s += bs[b];
+  sh[t] = s;
+  __syncthreads();
+  if (t == 0) {
+    int acc = 0;
+    for (int u = 0; u < kBlock; ++u) {
+      const int x = sh[u];
+      sh[u] = acc;
+      acc += x;
+    }
+  }
+  __syncthreads();
+  int acc = sh[t];
+  for (int b = b0; b < b1; ++b) {
+    const int x = bs[b];
+    bs[b] = acc;
+    acc += x;
+  }
 }
+// off[d] = exclusive prefix of cnt; fillpos[d] = 0
+__global__ __launch_bounds__(kBlock) void k_t_apply(const int* __restrict__ cnt, long long n,
+                                                   const int* __restrict__ bs,
+                                                   int* __restrict__ off, int* __restrict__ pos) {
+  const long long q = blockIdx.x * (long long)kBlock + threadIdx.x;
+  __shared__ int sh[kBlock];
+  sh[threadIdx.x] = q < n ? cnt[q] : 0;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int acc = bs[blockIdx.x];
+    for (int i = 0; i < kBlock; ++i) {
+      const int x = sh[i];
+      sh[i] = acc;
+      acc += x;
+    }
+  }
+  __syncthreads();
+  if (q < n) {
+    off[q] = sh[threadIdx.x];
+    pos[q] = 0;
+  }
+}
+__global__ void k_t_fill(const int32_t* __restrict__ edof, long long nk,
+                         const int* __restrict__ off, int* __restrict__ pos,
+                         int* __restrict__ keys) {
+  const long long q = blockIdx.x * (long long)kBlock + threadIdx.x;
+  if (q >= nk) return;
+  const int d = edof[q];
+  keys[off[d] + atomicAdd(&pos[d], 1)] = (int)q;  // key = 24 e + a
+}
+__global__ void k_t_sort(const int* __restrict__ off, const int* __restrict__ cnt, long long n,
+                         int* __restrict__ keys) {
+  const long long d = blockIdx.x * (long long)kBlock + threadIdx.x;
+  if (d >= n) return;
+  int* k = keys + off[d];
+  const int m = cnt[d];
+  for (int i = 1; i < m; ++i) {
+    const int v = k[i];
+    int j = i - 1;
+    while (j >= 0 && k[j] > v) {
+      k[j + 1] = k[j];
+      --j;
+    }
+    k[j + 1] = v;
+  }
+}
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_t_spmv(const T* __restrict__ xPhys,
+                                                  const T* __restrict__ u,
+                                                  const T* __restrict__ ke,
+                                                  const int32_t* __restrict__ edof,
+                                                  const int* __restrict__ off,
+                                                  const int* __restrict__ cnt,
+                                                  const int* __restrict__ keys, long long n,
+                                                  double p, T* __restrict__ y) {
+  const long long d = blockIdx.x * (long long)kBlock + threadIdx.x;
+  if (d >= n) return;
+  double acc = 0.0;
+  const int o = off[d], m = cnt[d];
+  for (int i = 0; i < m; ++i) {
+    const int key = keys[o + i];
+    const long long e = key / 24;
+    const int a = key - (int)(e * 24);
+    double t = 0.0;
+    for (int b = 0; b < 24; ++b) t += (double)ke[a * 24 + b] * (double)u[edof[e * 24 + b]];
+    acc += pow(fmax((double)xPhys[e], 1e-3), p) * t;
+  }
+  y[d] = (T)acc;
+}
+
+long long generic_spmv_ws_ints(long long ne, long long ndof) {
+  const long long nb = (ndof + kBlock - 1) / kBlock;
+  return 3 * ndof + 24 * ne + nb + 1;
+}
+
+template <typename T>
+void launch_generic_spmv(const T* xPhys, const T* u, const T* ke, const int32_t* edof,
+                         long long ne, long long ndof, double penal, int* ws, T* y,
+                         hipStream_t st) {
+  const long long nk = 24 * ne;
+  const int nb = (int)((ndof + kBlock - 1) / kBlock);
+  int* cnt = ws;
+  int* off = ws + ndof;
+  int* pos = ws + 2 * ndof;
+  int* keys = ws + 3 * ndof;
+  int* bs = keys + nk;
+  hipLaunchKernelGGL(k_fill_int, dim3(grid_blocks(ndof)), dim3(kBlock), 0, st, cnt, ndof, 0);
+  hipLaunchKernelGGL(k_t_count, dim3(grid_blocks(nk)), dim3(kBlock), 0, st, edof, nk, cnt);
+  hipLaunchKernelGGL(k_t_bsum, dim3(nb), dim3(kBlock), 0, st, cnt, ndof, bs);
+  hipLaunchKernelGGL(k_t_bscan, dim3(1), dim3(kBlock), 0, st, bs, nb);
+  hipLaunchKernelGGL(k_t_apply, dim3(nb), dim3(kBlock), 0, st, cnt, ndof, bs, off, pos);
+  hipLaunchKernelGGL(k_t_fill, dim3(grid_blocks(nk)), dim3(kBlock), 0, st, edof, nk, off, pos,
+                     keys);
+  hipLaunchKernelGGL(k_t_sort, dim3(grid_blocks(ndof)), dim3(kBlock), 0, st, off, cnt, ndof,
+                     keys);
+  hipLaunchKernelGGL((k_t_spmv<T>), dim3(grid_blocks(ndof)), dim3(kBlock), 0, st, xPhys, u, ke,
+                     edof, off, cnt, keys, ndof, penal, y);
+}
+template void launch_generic_spmv<float>(const float*, const float*, const float*,
+                                         const int32_t*, long long, long long, double, int*,
+                                         float*, hipStream_t);
+template void launch_generic_spmv<double>(const double*, const double*, const double*,
+                                          const int32_t*, long long, long long, double, int*,
+                                          double*, hipStream_t);
 
 __global__ void k_csr_spmv(const int64_t* __restrict__ rp, const int64_t* __restrict__ ci,
                            const float* __restrict__ v, const float* __restrict__ x,

@@ -8,9 +8,9 @@ reference's kernels do, and run the library's generic gather kernels
 a GPU they raise. Structured-grid callers use ``Problem`` instead, which
 derives element connectivity arithmetically.
 
-Vectors are stored as float32 (the reference kernels take float64 tensors);
-element products are formed in FP64, scattered with float32 atomics, and
-results come back in the dtype of the input they mirror.
+Float64 inputs run in float64, as the reference kernels read double; float32
+inputs in float32. Element products are formed in FP64, and results come back
+in the dtype of the input they mirror.
 """
 from __future__ import annotations
 
@@ -31,20 +31,34 @@ def _stream(t: torch.Tensor):
     return C.c_void_p(torch.cuda.current_stream(t.device).cuda_stream)
 
 
+def _ftype(*ts):
+    """float64 when any float input is float64 (the reference kernels' type),
+    else float32."""
+    return torch.float64 if any(t.dtype == torch.float64 for t in ts) else torch.float32
+
+
+def _edof(edof_mat: torch.Tensor, n_dof: int) -> torch.Tensor:
+    ed = _dev(edof_mat, torch.int32)
+    if ed.numel() and (int(ed.min()) < 0 or int(ed.max()) >= n_dof):
+        raise ValueError("edof_mat entries must lie in [0, n_dof)")
+    return ed
+
+
 def simp_sensitivity(xPhys: torch.Tensor, u: torch.Tensor, Ke: torch.Tensor,
                      edof_mat: torch.Tensor, penal: float, nx: int, ny: int, nz: int) -> torch.Tensor:
     """dc_e = -penal max(x_e, 1e-3)^(penal-1) u_e^T Ke u_e, shaped like xPhys
     (__init__.py:286-326; kernel simp_sensitivity_ptx.cu:163-270)."""
     n = int(nx) * int(ny) * int(nz)
-    x = _dev(xPhys, torch.float32)
+    ft = _ftype(xPhys, u, Ke)
+    x = _dev(xPhys, ft)
     if x.numel() != n or edof_mat.shape[0] != n:
         raise ValueError("xPhys and edof_mat must hold nx*ny*nz elements")
-    uu, ke = _dev(u, torch.float32), _dev(Ke, torch.float32)
-    ed = _dev(edof_mat, torch.int32)
+    uu, ke = _dev(u, ft), _dev(Ke, ft)
+    ed = _edof(edof_mat, uu.numel())
     dc = torch.empty_like(x)
     L.call("gp_generic_sens", C.c_void_p(x.data_ptr()), C.c_void_p(uu.data_ptr()),
            C.c_void_p(ke.data_ptr()), C.c_void_p(ed.data_ptr()), n, float(penal),
-           C.c_void_p(dc.data_ptr()), _stream(x))
+           L.GP_F64 if ft == torch.float64 else L.GP_F32, C.c_void_p(dc.data_ptr()), _stream(x))
     return dc.view(xPhys.shape).to(xPhys.dtype)
 
 
@@ -56,16 +70,23 @@ def simp_sensitivity_aggressive(xPhys, u, Ke, edof_mat, penal, nx, ny, nz) -> to
 def fused_spmv(xPhys: torch.Tensor, p: torch.Tensor, Ke: torch.Tensor, edof_mat: torch.Tensor,
                penal: float, nx: int, ny: int, nz: int) -> torch.Tensor:
     """y = sum_e max(x_e, 1e-3)^penal Ke p_e scattered by edof, shaped like p
-    (__init__.py:329-368; kernel simp_sensitivity_ptx.cu:301-373)."""
+    (__init__.py:329-368; kernel simp_sensitivity_ptx.cu:301-373). Every DOF
+    sums its terms in a fixed order, so the result does not vary run to run
+    (the reference kernel adds them atomically)."""
     n = int(nx) * int(ny) * int(nz)
-    x = _dev(xPhys, torch.float32)
+    ft = _ftype(xPhys, p, Ke)
+    x = _dev(xPhys, ft)
     if x.numel() != n or edof_mat.shape[0] != n:
         raise ValueError("xPhys and edof_mat must hold nx*ny*nz elements")
-    pp, ke = _dev(p, torch.float32), _dev(Ke, torch.float32)
-    ed = _dev(edof_mat, torch.int32)
+    pp, ke = _dev(p, ft), _dev(Ke, ft)
+    ed = _edof(edof_mat, pp.numel())
+    nw = C.c_int64()
+    L.call("gp_generic_spmv_workspace", n, pp.numel(), C.byref(nw))
+    ws = torch.empty(int(nw.value), dtype=torch.int32, device=x.device)
     y = torch.empty_like(pp)
     L.call("gp_generic_spmv", C.c_void_p(x.data_ptr()), C.c_void_p(pp.data_ptr()),
            C.c_void_p(ke.data_ptr()), C.c_void_p(ed.data_ptr()), n, pp.numel(), float(penal),
+           L.GP_F64 if ft == torch.float64 else L.GP_F32, C.c_void_p(ws.data_ptr()),
            C.c_void_p(y.data_ptr()), _stream(x))
     return y.view(p.shape).to(p.dtype)
 

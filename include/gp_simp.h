@@ -318,18 +318,23 @@ int gp_sched_run(gp_sched* s, int64_t max_ticks, int64_t* ticks_run, gp_slot_sta
                  void* stream);
 
 /* ---- generic-table kernels (reference kernel API) --------------------- *
- * Explicit KE (24x24 float32 row-major) and edof (int32 n_elem x 24) as the
- * reference kernels take them; edof may be arbitrary (no grid assumption).
+ * Explicit KE (24x24 row-major) and edof (int32 n_elem x 24, every entry in
+ * [0, n_dof)) as the reference kernels take them; edof may be arbitrary (no
+ * grid assumption). dtype GP_F32 or GP_F64 names the type of every float
+ * array (the reference kernels take double); element products are FP64.
  * simp_sensitivity: genpipeline/cuda_kernels/__init__.py:286-326,
  *   kernel simp_sensitivity_ptx.cu:163-270 -> dc = -p max(x,1e-3)^(p-1) u_e^T KE u_e
  * fused_spmv:       __init__.py:329-368, kernel simp_sensitivity_ptx.cu:301-373
- *   -> y = sum_e max(x,1e-3)^p KE u_e scattered (y zero-initialised here)   */
-int gp_generic_sens(const float* xPhys, const float* u, const float* ke,
-                    const int32_t* edof, int64_t n_elem, double penal, float* dc,
-                    void* stream);
-int gp_generic_spmv(const float* xPhys, const float* u, const float* ke,
-                    const int32_t* edof, int64_t n_elem, int64_t n_dof, double penal,
-                    float* y, void* stream);
+ *   -> y = sum_e max(x,1e-3)^p KE u_e scattered by edof. The reference scatters
+ *   with atomic adds (summation order varies run to run); here every DOF
+ *   gathers its terms in a fixed order (element, then local index), so the
+ *   result is deterministic. workspace: gp_generic_spmv_workspace ints.     */
+int gp_generic_sens(const void* xPhys, const void* u, const void* ke, const int32_t* edof,
+                    int64_t n_elem, double penal, int32_t dtype, void* dc, void* stream);
+int gp_generic_spmv_workspace(int64_t n_elem, int64_t n_dof, int64_t* n_ints);
+int gp_generic_spmv(const void* xPhys, const void* u, const void* ke, const int32_t* edof,
+                    int64_t n_elem, int64_t n_dof, double penal, int32_t dtype,
+                    int32_t* workspace, void* y, void* stream);
 /* Jacobi-PCG on a CSR matrix (float32 values, int64 row_ptr/col_idx), fixed
  * DOFs (int64 list) held at zero. Replaces pcg_solve_cuda
  * (__init__.py:443-473; simp_pcg_cuda.cu:66-124). x: in x0, out solution.  */

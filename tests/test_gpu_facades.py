@@ -202,7 +202,13 @@ def test_voxel_fem_evaluator_gpu_branch(gpu):
 
 
 # ------------------------------------------------------------- kernels.py
-def test_kernel_functions_match_oracle(gpu):
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+def test_kernel_functions_match_oracle(gpu, dtype):
+    """simp_sensitivity and fused_spmv on explicit tables. float64 tensors
+    run in float64 as the reference kernels do (1e-12 against the FP64
+    oracle), float32 tensors in float32 storage (1e-5); fused_spmv gathers
+    in a fixed order, so repeated calls agree bit for bit, also on a
+    permuted edof that is no structured grid."""
     from genpipeline_amd import kernels as K
     nx, ny, nz = 5, 4, 3
     rng = np.random.default_rng(7)
@@ -211,20 +217,30 @@ def test_kernel_functions_match_oracle(gpu):
     x = rng.uniform(0.0, 1.0, (nx, ny, nz))
     u = rng.standard_normal(nd)
     KE = O.ke_hex8(0.3)
-    t = lambda a, dt=torch.float64: torch.as_tensor(a, dtype=dt, device=gpu)
+    dt = getattr(torch, dtype)
+    tol = 1e-12 if dtype == "float64" else 1e-5
+    q = (lambda a: a) if dtype == "float64" else (lambda a: a.astype(np.float32).astype(np.float64))
+    t = lambda a, d=dt: torch.as_tensor(a, dtype=d, device=gpu)
     dc = K.simp_sensitivity(t(x), t(u), t(KE), t(ed, torch.int64), 3.0, nx, ny, nz)
-    ref = O.sensitivity(KE, u.astype(np.float32).astype(np.float64), ed, x.ravel(), "clamp", 3.0)
-    assert dc.shape == (nx, ny, nz)
-    assert np.abs(dc.cpu().numpy().ravel() - ref).max() <= 1e-5 * np.abs(ref).max()
+    assert dc.dtype == dt and dc.shape == (nx, ny, nz)
+    ref = O.sensitivity(q(KE), q(u), ed, q(x.ravel()), "clamp", 3.0)
+    assert np.abs(dc.cpu().numpy().ravel() - ref).max() <= tol * np.abs(ref).max()
     z = K.simp_sensitivity_aggressive(t(x), t(np.zeros(nd)), t(KE), t(ed, torch.int64), 3.0,
                                       nx, ny, nz)
     assert torch.count_nonzero(z) == 0  # tests/test_aggressive_kernel.py:106-125
     A = rng.standard_normal((24, 24))  # asymmetric: pins y_e = A p_e, not A^T p_e
-    y = K.fused_spmv(t(x), t(u), t(A), t(ed, torch.int64), 3.0, nx, ny, nz)
-    E = np.maximum(x.ravel(), 1e-3) ** 3
-    ye = (u.astype(np.float32).astype(np.float64)[ed] @ A.astype(np.float32).astype(np.float64).T)
-    yref = np.bincount(ed.ravel(), weights=(ye * E[:, None]).ravel(), minlength=nd)
-    assert np.abs(y.cpu().numpy() - yref).max() <= 1e-5 * np.abs(yref).max()
+    perm = rng.permutation(nd)
+    for edm in (ed, perm[ed]):
+        y = K.fused_spmv(t(x), t(u), t(A), t(edm, torch.int64), 3.0, nx, ny, nz)
+        assert y.dtype == dt
+        E = np.maximum(q(x.ravel()), 1e-3) ** 3
+        ye = q(u)[edm] @ q(A).T
+        yref = np.bincount(edm.ravel(), weights=(ye * E[:, None]).ravel(), minlength=nd)
+        assert np.abs(y.cpu().numpy() - yref).max() <= tol * np.abs(yref).max()
+        y2 = K.fused_spmv(t(x), t(u), t(A), t(edm, torch.int64), 3.0, nx, ny, nz)
+        assert torch.equal(y, y2)
+    with pytest.raises(ValueError):
+        K.fused_spmv(t(x), t(u), t(A), t(ed + nd, torch.int64), 3.0, nx, ny, nz)
 
 
 def test_pcg_solve_cuda_matches_direct(gpu):
