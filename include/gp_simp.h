@@ -37,9 +37,11 @@ extern "C" {
 /* ---- enums ----------------------------------------------------------- */
 #define GP_F32 0              /* PCG vectors float32, FP64 dots              */
 #define GP_F64 1              /* PCG vectors float64                          */
-#define GP_F16 2              /* PCG vectors and KE coefficients stored fp16, */
-                              /* fp32 math, FP64 dots; runs as the inner     */
-                              /* solve of FP64 iterative refinement          */
+#define GP_F16 2              /* the PCG iterate x and search direction p    */
+                              /* stored fp16; the recurrence vectors r, w, s, */
+                              /* M^-1 and the eight element coefficients stay */
+                              /* float; fp32 math, FP64 dots; runs only as    */
+                              /* the inner solve of FP64 iterative refinement */
 #define GP_BF16 3             /* as GP_F16 with bfloat16 storage (the fp32   */
                               /* exponent range: safe for Emin = 1e-9)        */
 
