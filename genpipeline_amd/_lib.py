@@ -50,7 +50,7 @@ class GpSimpOpts(C.Structure):
     _fields_ = [("n_iters", C.c_int32), ("volfrac", C.c_double), ("pcg", GpPcgOpts),
                 ("tol_schedule", C.c_int32), ("early_stop", C.c_int32),
                 ("compliance_each_iter", C.c_int32), ("warm_start", C.c_int32),
-                ("warm_compliance", C.c_int32)]
+                ("warm_compliance", C.c_int32), ("direct", C.c_int32)]
 
 
 class GpSimpStats(C.Structure):
@@ -95,6 +95,7 @@ SIGNATURES = {
     "gp_ku": [P, P, P, P, I32, I32, P],
     "gp_diag": [P, P, P, I32, P],
     "gp_pcg": [P, P, P, P, C.POINTER(GpPcgOpts), C.POINTER(GpPcgStats), P],
+    "gp_direct": [P, P, P, P, C.POINTER(GpPcgStats), P],
     "gp_sens": [P, P, P, I32, P, P, P],
     "gp_filter": [P, P, P, I32, P],
     "gp_oc": [P, P, P, F64, P, P, C.POINTER(GpOcStats), P],

@@ -163,6 +163,15 @@ struct gp_problem {
   double* xnew = nullptr;
   double* xphys_new = nullptr;
   double* ce_sum = nullptr;
+  // exact direct solve: host copy of the supports (the free numbering is
+  // built from it) and the banded-LU workspace, one allocation
+  std::vector<uint8_t> fixmask_h;
+  std::vector<int> fi_h;
+  DirectWs dws{};
+  void* dws_mem = nullptr;
+  size_t dws_bytes = 0;
+  int* dws_info_h = nullptr;    // pinned
+  bool dws_valid = false;
   // sampled iteration timing: events bracket every `prof_every`-th chunk of
   // PCG iteration launches
   int iter_hint[3] = {0, 0, 0};  // last PCG iteration count (fp32, fp64, 16-bit)
@@ -374,6 +383,87 @@ int ensure_refine_ws(gp_problem* P) {
   if (!P->r64) ALLOC(P->r64, nd * sizeof(double));
   if (!P->y64) ALLOC(P->y64, nd * sizeof(double));
   if (!P->d32) ALLOC(P->d32, nd * sizeof(float));
+  return GP_OK;
+}
+
+// Largest band slot (the LDS of the LU kernels holds a slot's worth of
+// doubles) and largest band (device bytes) of an exact direct solve.
+constexpr int kDirectMaxW = 8000;
+constexpr size_t kDirectMaxBytes = size_t(16) << 30;
+
+int ensure_direct_ws(gp_problem* P, hipStream_t s) {
+  if (P->dws_valid) return GP_OK;
+  const Grid& g = P->g;
+  const int bw = direct_band_width(g);
+  const int W = 2 * bw + 2;
+  if (W > kDirectMaxW)
+    return fail(GP_ERR_ARG,
+                "direct solve: band slot of %d doubles exceeds %d (node strides %d, %d); "
+                "grids with (ny+1)(nz+1) up to about 1300 are supported",
+                W, kDirectMaxW, g.sI, g.sJ);
+  P->fi_h.assign(g.ndof, -1);
+  int nf = 0;
+  for (long long d = 0; d < g.ndof; ++d)
+    if (!((P->fixmask_h[d / 3] >> (d % 3)) & 1)) P->fi_h[d] = nf++;
+  const size_t band = (size_t)nf * W * sizeof(double);
+  if (band > kDirectMaxBytes)
+    return fail(GP_ERR_ARG, "direct solve: band of %zu bytes exceeds %zu", band, kDirectMaxBytes);
+  auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+  const size_t o_fi = 0, o_band = up(o_fi + g.ndof * sizeof(int));
+  const size_t o_b = up(o_band + band), o_x = up(o_b + nf * sizeof(double));
+  const size_t o_map = up(o_x + nf * sizeof(double)), o_urow = up(o_map + 2 * nf * sizeof(int));
+  const size_t o_cand = up(o_urow + nf * sizeof(int));
+  const size_t o_info = up(o_cand + 2 * (bw + 1) * sizeof(double));
+  const size_t bytes = up(o_info + sizeof(int));
+  if (bytes > P->dws_bytes) {
+    if (P->dws_mem) {
+      HIPCHK(hipStreamSynchronize(s));  // the old workspace may still be in use on s
+      HIPCHK(hipFree(P->dws_mem));
+      P->dws_mem = nullptr;
+      P->dws_bytes = 0;
+    }
+    ALLOC(P->dws_mem, bytes);
+    P->dws_bytes = bytes;
+  }
+  if (!P->dws_info_h) HIPCHK(hipHostMalloc((void**)&P->dws_info_h, sizeof(int)));
+  char* base = (char*)P->dws_mem;
+  DirectWs& w = P->dws;
+  w.nf = nf;
+  w.bw = bw;
+  w.W = W;
+  w.fi = (int*)(base + o_fi);
+  w.band = (double*)(base + o_band);
+  w.b = (double*)(base + o_b);
+  w.x = (double*)(base + o_x);
+  w.map = (int*)(base + o_map);
+  w.urow = (int*)(base + o_urow);
+  w.cand = (double*)(base + o_cand);
+  w.info = (int*)(base + o_info);
+  // fi_h stays alive with the problem, so the copy may complete later
+  HIPCHK(hipMemcpyAsync(w.fi, P->fi_h.data(), g.ndof * sizeof(int), hipMemcpyHostToDevice, s));
+  P->dws_valid = true;
+  return GP_OK;
+}
+
+// u = K(xPhys)^-1 f on the free DOFs (0 on the fixed ones), exact in FP64.
+// Blocks for the pivot check.
+int direct_solve(gp_problem* P, const double* xPhys, const double* f, double* u,
+                 gp_pcg_stats* st, hipStream_t s) {
+  std::memset(st, 0, sizeof(*st));
+  int rc = ensure_direct_ws(P, s);
+  if (rc) return rc;
+  const gp_spec& sp = P->spec;
+  launch_direct(P->g, P->ke64, xPhys, sp.law, sp.penal, sp.E0, sp.Emin, f, P->dws, u, s);
+  LAUNCHCHK();
+  *P->dws_info_h = 0;
+  if (P->dws.nf > 0)
+    HIPCHK(hipMemcpyAsync(P->dws_info_h, P->dws.info, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (*P->dws_info_h) {
+    st->status = GP_BREAKDOWN;
+    return fail(GP_BREAKDOWN, "direct solve: zero pivot in column %d of %d (singular system)",
+                *P->dws_info_h - 1, P->dws.nf);
+  }
   return GP_OK;
 }
 
@@ -849,10 +939,11 @@ int gp_problem_destroy(gp_problem* P) {
   hipSetDevice(P->device);
   // hipFree waits for the work that may still use these buffers
   void* dev[] = {P->arena, P->pres, P->ws32, P->ws64, P->ws16, P->r64, P->y64, P->d32, P->E,
-                 P->dc, P->dcf, P->xnew, P->xphys_new, P->ce_sum};
+                 P->dc, P->dcf, P->xnew, P->xphys_new, P->ce_sum, P->dws_mem};
   for (void* p : dev)
     if (p) hipFree(p);
   if (P->h_sc) hipHostFree(P->h_sc);
+  if (P->dws_info_h) hipHostFree(P->dws_info_h);
   for (auto& ev : P->ev)
     if (ev) hipEventDestroy(ev);
   for (auto& pr : P->prof_pending) {
@@ -878,6 +969,8 @@ int gp_problem_set_bcs(gp_problem* P, const int64_t* fixed, int64_t n_fixed, con
   }
   HIPCHK(hipMemcpy(P->fixmask, m.data(), g.nn, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(P->f, f_host, g.ndof * sizeof(double), hipMemcpyHostToDevice));
+  P->fixmask_h = std::move(m);
+  P->dws_valid = false;  // the free numbering follows the supports
   P->has_bcs = true;
   return GP_OK;
 }
@@ -1028,6 +1121,14 @@ int gp_pcg(gp_problem* P, const float* E, const double* f, void* u, const gp_pcg
   return GP_OK;
 }
 
+int gp_direct(gp_problem* P, const double* xPhys, const double* f, double* u, gp_pcg_stats* st,
+              void* stream) {
+  if (!P || !xPhys || !f || !u || !st) return fail(GP_ERR_ARG, "NULL argument");
+  if (!P->has_bcs) return fail(GP_ERR_STATE, "boundary conditions not set");
+  HIPCHK(hipSetDevice(P->device));
+  return direct_solve(P, xPhys, f, u, st, S(stream));
+}
+
 int gp_sens(gp_problem* P, const double* xPhys, const void* u, int32_t dtype, double* dc,
             double* ce_sum, void* stream) {
   if (!P || !xPhys || !u || !dc) return fail(GP_ERR_ARG, "NULL argument");
@@ -1097,7 +1198,10 @@ int gp_simp_run(gp_problem* P, const gp_simp_opts* o, double* x, double* xPhys, 
   if (rc) return rc;
   const Grid& g = P->g;
   const gp_spec& sp = P->spec;
-  const bool refine = o->pcg.refine != 0 || o->pcg.precision == GP_F64 ||
+  const bool direct = o->direct != 0;
+  // u is the float64 state whenever the solves are FP64 (refined, FP64 PCG
+  // or direct)
+  const bool refine = direct || o->pcg.refine != 0 || o->pcg.precision == GP_F64 ||
                       o->pcg.precision == GP_F16 || o->pcg.precision == GP_BF16;
   // u is float64 state; plain FP32 PCG works on a float copy
   float* u32 = nullptr;
@@ -1114,7 +1218,9 @@ int gp_simp_run(gp_problem* P, const gp_simp_opts* o, double* x, double* xPhys, 
     po.use_x0 = warm ? 1 : 0;
     int r;
     std::memset(ps, 0, sizeof(*ps));
-    if (refine) {
+    if (direct) {
+      r = direct_solve(P, xPhys, P->f, u, ps, s);
+    } else if (refine) {
       if (po.precision == GP_F64 && !po.refine) {
         r = ensure_pcg_ws(P, GP_F64);
         if (!r) r = pcg_core<double>(P, P->E, P->f, u, tol, max_iter, warm, ps, s);

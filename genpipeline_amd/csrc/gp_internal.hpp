@@ -422,6 +422,26 @@ void launch_generic_spmv(const T* xPhys, const T* u, const T* ke, const int32_t*
 void launch_csr_spmv(const int64_t* rp, const int64_t* ci, const float* v, const float* x, float* y,
                      long long n, hipStream_t st);
 
+// Exact FP64 banded-LU solve of the free-DOF system (device buffers of one
+// problem; layout described with the kernels)
+struct DirectWs {
+  int nf = 0;               // free DOFs
+  int bw = 0;               // lower bandwidth bound, 3 (sI + sJ + 1) + 2
+  int W = 0;                // band row slot, 2 bw + 2 doubles
+  int* fi = nullptr;        // ndof: free index of each DOF, -1 when fixed
+  double* band = nullptr;   // nf * W
+  double* b = nullptr;      // nf, right-hand side per physical slot
+  double* x = nullptr;      // nf, solution per free index
+  int* map = nullptr;       // 2 nf, logical row -> physical slot (two copies)
+  int* urow = nullptr;      // nf, physical slot of U row k
+  double* cand = nullptr;   // 2 (bw + 1), pivot candidates of the next column
+  int* info = nullptr;      // 0, or k + 1 for a zero pivot in column k
+};
+int direct_band_width(const Grid& g);
+void launch_direct(const Grid& g, const double* ke, const double* xPhys, int law, double penal,
+                   double E0, double Emin, const double* f, const DirectWs& w, double* u,
+                   hipStream_t st);
+
 inline int grid_blocks(long long n, int per_thread = 1) {
   long long b = (n + (long long)kBlock * per_thread - 1) / ((long long)kBlock * per_thread);
   if (b < 1) b = 1;

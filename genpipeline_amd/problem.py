@@ -280,6 +280,16 @@ class Problem:
         L.call("gp_pcg", self._h, _ptr(E), _ptr(f), _ptr(u), C.byref(o), C.byref(st), _stream(self.device))
         return u, PcgResult(st.status, st.iters, st.outer, st.relres)
 
+    def direct(self, xPhys, f=None):
+        """Exact FP64 solve K(xPhys) u = f (banded LU with partial pivoting,
+        gp_direct); returns (u float64, PcgResult with status only)."""
+        x = self._f64(xPhys, self.spec.n_elem)
+        f = self.f if f is None else self._f64(f, self.spec.n_dof)
+        u = torch.empty(self.spec.n_dof, dtype=torch.float64, device=self.device)
+        st = L.GpPcgStats()
+        L.call("gp_direct", self._h, _ptr(x), _ptr(f), _ptr(u), C.byref(st), _stream(self.device))
+        return u, PcgResult(st.status, st.iters, st.outer, st.relres)
+
     def sens(self, xPhys, u, with_energy=False):
         """dc = -dE/dx * u_e^T KE u_e (float64). With with_energy also returns
         sum_e E_e u_e^T KE u_e."""
@@ -322,10 +332,11 @@ class Problem:
     def run(self, n_iters, x0=None, volfrac=None, tol=1e-6, max_iter=10000, precision="fp32",
             refine=False, inner_tol=1e-3, max_outer=50, warm_start=True, tol_schedule=False,
             early_stop=False, compliance_each_iter=False, u0=None, xphys0=None,
-            warm_compliance=False) -> SimpResult:
+            warm_compliance=False, direct=False) -> SimpResult:
         """Whole SIMP loop on the device (gp_simp_run). warm_compliance: the
         compliance solve after each update starts from the previous u
-        instead of zero."""
+        instead of zero. direct: every solve is the exact FP64 banded LU
+        (the PCG arguments are then ignored)."""
         sp = self.spec
         vf = sp.volfrac if volfrac is None else volfrac
         if x0 is None:
@@ -352,6 +363,7 @@ class Problem:
         o.compliance_each_iter = 1 if compliance_each_iter else 0
         o.warm_start = 1 if warm_start else 0
         o.warm_compliance = 1 if warm_compliance else 0
+        o.direct = 1 if direct else 0
         st = L.GpSimpStats()
         L.call("gp_simp_run", self._h, C.byref(o), _ptr(x), _ptr(xPhys), _ptr(u), C.byref(st),
                _stream(self.device))

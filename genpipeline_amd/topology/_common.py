@@ -40,11 +40,16 @@ def _digest(*arrays) -> str:
 
 
 def cached_problem(owner, law, oc, f, fixed, preserved, volfrac, penal=None, E0=1.0, Emin=1e-9,
-                   filter_weights="f64"):
+                   filter_weights="f64", ke=None):
     """The owner's device problem for these BCs, created on first use and
-    reused while grid, material, loads and supports stay the same."""
+    reused while grid, material, element table, loads and supports stay the
+    same. ke: None (exact hex8) or a 24x24 table in the device orientation
+    (y_e = ke u_e)."""
     penal = owner.penal if penal is None else penal
+    if ke is not None:
+        ke = np.ascontiguousarray(ke, np.float64).reshape(24, 24)
     key = (law, oc, filter_weights, float(penal), float(owner.rmin), float(E0), float(Emin),
+           None if ke is None else ke.tobytes(),
            _digest(np.asarray(f, np.float64), np.sort(np.asarray(fixed, np.int64)),
                    None if preserved is None else np.asarray(preserved, bool)))
     P = getattr(owner, "_problem", None)
@@ -55,7 +60,8 @@ def cached_problem(owner, law, oc, f, fixed, preserved, volfrac, penal=None, E0=
     spec = ProblemSpec(owner.nx, owner.ny, owner.nz, penal=float(penal), rmin=float(owner.rmin),
                        volfrac=float(volfrac), E0=float(E0), Emin=float(Emin), law=law, oc=oc,
                        filter_weights=filter_weights)
-    P = Problem(spec, fixed, f, preserved=preserved, device=getattr(owner, "_torch_device", None))
+    P = Problem(spec, fixed, f, preserved=preserved, ke=ke,
+                device=getattr(owner, "_torch_device", None))
     owner._problem = P
     owner._problem_key = key
     return P

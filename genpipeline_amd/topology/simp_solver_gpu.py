@@ -31,6 +31,13 @@ Differences:
   * ``dtype`` float32 runs FP32 PCG vectors, float64 FP64 vectors; bfloat16
     and float16 (used by the reference only inside the sensitivity product)
     run as float32.
+
+``ke=table`` (keyword) runs any 24x24 element table in the reference GPU
+solver's orientation (K = sum_e E_e KE, simp_solver_gpu.py:313-326), and
+``solver="direct"`` (the default whenever ``ke`` is given) solves every system
+exactly in FP64 by banded LU with partial pivoting on the device (gp_direct):
+the reference's SciPy direct branch for small systems (:571-600), and the only
+solve that works with the reference's indefinite table.
 """
 from __future__ import annotations
 
@@ -47,9 +54,10 @@ class StiffnessOperator:
     device. ``K @ u`` (or ``K.mv(u)``) returns K u without boundary rows
     zeroed, like multiplying the reference's assembled CSR matrix."""
 
-    def __init__(self, problem, E: torch.Tensor):
+    def __init__(self, problem, E: torch.Tensor, xPhys: torch.Tensor | None = None):
         self.problem = problem
         self.E = E
+        self.xPhys = xPhys  # float64 densities (the direct solve reads them)
         n = problem.spec.n_dof
         self.shape = (n, n)
         self.device = E.device
@@ -73,7 +81,8 @@ class SIMPSolverGPU:
                  rmin: float = 1.5, boundary_conditions: dict | None = None,
                  device: str = "cuda", preserved_mask: np.ndarray | None = None,
                  dtype: torch.dtype = torch.float32, *, max_iter: int = 20000,
-                 pcg_tol: float | None = None):
+                 pcg_tol: float | None = None, ke: np.ndarray | None = None,
+                 solver: str | None = None):
         if not torch.cuda.is_available():
             raise RuntimeError("SIMPSolverGPU needs a ROCm GPU (torch.cuda.is_available() is False)")
         self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
@@ -88,7 +97,12 @@ class SIMPSolverGPU:
         # every solve of run() to that relative residual (no counterpart in
         # the reference)
         self.pcg_tol = None if pcg_tol is None else float(pcg_tol)
-        self.Ke = torch.from_numpy(_lib.hex8_ke(0.3)).to(
+        table = _lib.hex8_ke(0.3) if ke is None else np.array(ke, np.float64).reshape(24, 24)
+        self._ke_dev = None if ke is None else np.ascontiguousarray(table)
+        self.solver = solver or ("pcg" if ke is None else "direct")
+        if self.solver not in ("pcg", "direct"):
+            raise ValueError(f"solver must be 'pcg' or 'direct', not {self.solver!r}")
+        self.Ke = torch.from_numpy(table).to(
             torch.float64 if dtype == torch.float64 else torch.float32)
         self.last_compliance = 0.0
         self.preserved_mask = (None if preserved_mask is None
@@ -126,7 +140,7 @@ class SIMPSolverGPU:
         fixed = (fixed.detach().cpu().numpy() if torch.is_tensor(fixed)
                  else np.asarray(fixed)).astype(np.int64)
         return cached_problem(self, "clamp", "gpu", f, fixed, self.preserved_mask, volfrac,
-                              filter_weights="f32")
+                              filter_weights="f32", ke=self._ke_dev)
 
     def _assemble_K(self, xPhys) -> StiffnessOperator:
         """Element moduli max(xPhys, 1e-3)^penal on the device, as an operator
@@ -135,7 +149,7 @@ class SIMPSolverGPU:
         if P is None:
             P = self._problem_for(*self._host_bcs(1.0))
         x = torch.as_tensor(xPhys).reshape(-1).to(P.device, torch.float64)
-        return StiffnessOperator(P, P.stiffness(x))
+        return StiffnessOperator(P, P.stiffness(x), x)
 
     def _solve(self, K, f, fixed_dofs, tol: float = 1e-6, max_iter: int = 2000, x0=None):
         """Solve K u = f with u = 0 on ``fixed_dofs`` (simp_solver_gpu.py:571-705).
@@ -145,7 +159,14 @@ class SIMPSolverGPU:
             raise TypeError("_solve takes the operator returned by _assemble_K")
         P = self._problem_for(f, fixed_dofs)
         if P is not K.problem:
-            K = StiffnessOperator(P, K.E)
+            K = StiffnessOperator(P, K.E, K.xPhys)
+        if self.solver == "direct":
+            if K.xPhys is None:
+                raise ValueError("the direct solve needs the operator of _assemble_K(xPhys)")
+            u, _ = P.direct(K.xPhys, f=torch.as_tensor(f).to(P.device, torch.float64))
+            u = u if self.precision == "fp64" else u.float()
+            self._u_prev = u
+            return u
         cap = max(int(max_iter), self.max_iter)
         u, _ = P.pcg(K.E, f=torch.as_tensor(f).to(P.device, torch.float64), u0=x0, tol=float(tol),
                      max_iter=cap, precision=self.precision)
@@ -169,7 +190,8 @@ class SIMPSolverGPU:
         res = P.run(int(n_iters), x0=x, xphys0=x, volfrac=float(volfrac),
                     tol=self.pcg_tol if fixed_tol else 1e-6, max_iter=self.max_iter,
                     precision=self.precision, warm_start=True, tol_schedule=not fixed_tol,
-                    early_stop=True, compliance_each_iter=True)
+                    early_stop=True, compliance_each_iter=True,
+                    direct=self.solver == "direct")
         self.last_result = res
         self._u_prev = res.u
         self.last_compliance = float(res.compliance)

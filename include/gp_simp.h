@@ -115,6 +115,9 @@ typedef struct gp_simp_opts {
                               /*    the compliance solve also starts from the */
                               /*    previous u (the reference solves it from  */
                               /*    zero, simp_solver_gpu.py:747-751)         */
+  int32_t direct;             /* 1: every solve is gp_direct's exact FP64     */
+                              /*    banded LU (the reference CPU solver's     */
+                              /*    SuperLU, simp_solver.py:213); pcg ignored */
 } gp_simp_opts;
 
 typedef struct gp_simp_stats {
@@ -194,6 +197,19 @@ int gp_diag(gp_problem* p, const float* E, void* diag, int32_t dtype, void* stre
  * host until the solve finishes (the iteration count is a host value).     */
 int gp_pcg(gp_problem* p, const float* E, const double* f, void* u,
            const gp_pcg_opts* opts, gp_pcg_stats* stats, void* stream);
+
+/* Exact solve of K(xPhys) u = f on the free DOFs (u = 0 on the fixed ones)
+ * by banded LU with partial pivoting in FP64, the moduli E(xPhys) taken in
+ * FP64 as well. Replaces SciPy SuperLU's spsolve in SIMPSolver._solve
+ * (simp_solver.py:195-213); like it, it needs no symmetry or definiteness of
+ * the element table (the reference's own table is neither). xPhys (n_elem),
+ * f, u (n_dof): float64 device. Band of n_free * (6 (sI + sJ + 1) + 6)
+ * doubles, sI = (ny+1)(nz+1), sJ = nz+1: grids up to (ny+1)(nz+1) ~ 1300
+ * and a band of 16 GiB; larger grids are refused (GP_ERR_ARG). A zero pivot
+ * (singular system) returns GP_BREAKDOWN. One launch per free DOF; blocks.
+ * stats: status only (iters = outer = 0, relres = 0).                      */
+int gp_direct(gp_problem* p, const double* xPhys, const double* f, double* u,
+              gp_pcg_stats* stats, void* stream);
 
 /* ---- SIMP pieces ------------------------------------------------------ */
 /* dc[e] = -dE/dx(xPhys[e]) * u_e^T KE u_e (float64 out); ce_sum (device
