@@ -31,6 +31,8 @@ namespace {
 
 constexpr int kStepBlock = 256;  // 4 waves, one band row each
 constexpr int kSubBlock = 1024;  // back substitution: one workgroup
+constexpr int kStepStage = 32;   // band columns a lane of the LU step holds at once
+constexpr int kSubStage = 16;    // ... and of the back substitution
 
 __global__ void k_zero_f64(double* __restrict__ v, long long n) {
   for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n;
@@ -116,7 +118,10 @@ __global__ void k_cand0(const double* __restrict__ band, int W, int nf, int bw,
 // that now holds U row k, and subtract l_i times the pivot row from each
 // row i in (k, kend] and from its right-hand side. One wave per row; wave gw
 // always takes the rows i = gw (mod 4 gridDim.x), so a row stays with one
-// workgroup (and one L2) for the steps it is active in.
+// workgroup (and one L2) for the steps it is active in. A wave loads its
+// row's first kStepStage * 64 columns before the pivot search, so that load
+// overlaps the search and the pivot-row load; the one row that trades places
+// with the pivot row loads again from its new slot.
 __global__ __launch_bounds__(kStepBlock) void k_lu_step(
     double* __restrict__ band, int W, int nf, int bw, int k, const int* __restrict__ sm,
     int* __restrict__ dm, int* __restrict__ urow, double* __restrict__ b,
@@ -127,13 +132,34 @@ __global__ __launch_bounds__(kStepBlock) void k_lu_step(
   const int uw = 2 * bw;
   const int kend = min(k + bw, nf - 1), kendc = min(k + uw, nf - 1);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // columns k+1+c, c < ncols, sit at positions (k+1+c) mod W; positions
+  // advance incrementally by (stride mod W), which wraps at most once
+  const int ncols = kendc - k;
+  const int wrap = 64 % W;
+  const int pos0 = (k + 1 + lane) % W;
+  const int nw = 4 * gridDim.x;
+  const int gw = blockIdx.x * 4 + wave;
+  const int i = k + 1 + ((gw - (k + 1)) % nw + nw) % nw;
+  const bool own = i <= kend;
+  const int slot_i = own ? sm[i] : 0;
+  double v[kStepStage];
+  if (own) {
+    const double* row = band + (long long)slot_i * W;
+    int pos = pos0;
+#pragma unroll
+    for (int t = 0; t < kStepStage; ++t) {
+      v[t] = lane + 64 * t < ncols ? row[pos] : 0.0;
+      pos += wrap;
+      if (pos >= W) pos -= W;
+    }
+  }
   // pivot search (every workgroup, same order: the same answer)
   double best = -1.0;
   int bt = 0;
   for (int t = threadIdx.x; t <= kend - k; t += kStepBlock) {
-    const double v = fabs(cand[t]);
-    if (v > best) {
-      best = v;
+    const double cv = fabs(cand[t]);
+    if (cv > best) {
+      best = cv;
       bt = t;
     }
   }
@@ -169,58 +195,132 @@ __global__ __launch_bounds__(kStepBlock) void k_lu_step(
   }
   if (!(fabs(piv) > 0.0)) return;  // uniform: every thread read the same piv
   const double* urow_p = band + (long long)slot_p * W;
-  for (int j = k + 1 + threadIdx.x; j <= kendc; j += kStepBlock) s_u[j - k - 1] = urow_p[j % W];
+  {
+    const int swrap = kStepBlock % W;
+    int pos = (k + 1 + (int)threadIdx.x) % W;
+    for (int c = threadIdx.x; c < ncols; c += kStepBlock) {
+      s_u[c] = urow_p[pos];
+      pos += swrap;
+      if (pos >= W) pos -= W;
+    }
+  }
   const double bp = b[slot_p];
   __syncthreads();
-  const int nw = 4 * gridDim.x;
-  const int gw = blockIdx.x * 4 + wave;
-  const int i = k + 1 + ((gw - (k + 1)) % nw + nw) % nw;
-  if (i > kend) return;
-  const int slot = i == p ? slot_k : sm[i];
+  if (!own) return;
+  const int slot = i == p ? slot_k : slot_i;
+  double* row = band + (long long)slot * W;
+  if (i == p) {  // wave-uniform
+    int pos = pos0;
+#pragma unroll
+    for (int t = 0; t < kStepStage; ++t) {
+      v[t] = lane + 64 * t < ncols ? row[pos] : 0.0;
+      pos += wrap;
+      if (pos >= W) pos -= W;
+    }
+  }
   const double a = i == p ? cand[0] : cand[i - k];
   const double l = a / piv;
-  double* row = band + (long long)slot * W;
   if (lane == 0 && k >= 1) row[(k - 1) % W] = 0.0;
-  if (l != 0.0) {
-    for (int j = k + 1 + lane; j <= kendc; j += 64) {
-      const int pos = j % W;
-      row[pos] = row[pos] - l * s_u[j - k - 1];
+  int pos = pos0;
+#pragma unroll
+  for (int t = 0; t < kStepStage; ++t) {
+    const int c = lane + 64 * t;
+    if (l != 0.0 && c < ncols) {
+      v[t] = v[t] - l * s_u[c];
+      row[pos] = v[t];
+    }
+    pos += wrap;
+    if (pos >= W) pos -= W;
+  }
+  const double first = v[0];  // column k+1 after the update (lane 0)
+  // columns beyond the prefetched ones (bands wider than 64 kStepStage):
+  // loaded kStepStage at a time before any of them is stored
+  for (int c0 = 64 * kStepStage; c0 < ncols; c0 += 64 * kStepStage) {
+    int ps[kStepStage];
+#pragma unroll
+    for (int t = 0; t < kStepStage; ++t) {
+      ps[t] = pos;
+      v[t] = c0 + lane + 64 * t < ncols ? row[pos] : 0.0;
+      pos += wrap;
+      if (pos >= W) pos -= W;
+    }
+    if (l != 0.0) {
+#pragma unroll
+      for (int t = 0; t < kStepStage; ++t) {
+        const int c = c0 + lane + 64 * t;
+        if (c < ncols) row[ps[t]] = v[t] - l * s_u[c];
+      }
     }
   }
   if (lane == 0) {
     if (l != 0.0) b[slot] = b[slot] - l * bp;
     dm[i] = slot;
-    cand_next[i - k - 1] = row[(k + 1) % W];  // this lane wrote column k+1 itself
+    cand_next[i - k - 1] = first;
   }
 }
 
-// U x = y from the bottom row up; y and the rows of U are in the slots urow
-// names. x also goes to a ring of W doubles in LDS (x[j] at j mod W), which
-// holds every x[j] the dot of row k reads (j <= k + 2 bw < k + W).
+// U x = y from the bottom row up, 64 rows (a tile [k0, k1]) at a time. For
+// each row r of the tile, the part of its dot over the columns already solved
+// (k1 < j <= r + 2 bw) is summed by the 16 waves in parallel (4 rows each;
+// lane-strided, then a fixed shuffle tree); then wave 0 solves the tile's
+// 64 x 64 upper triangle from LDS, lane q holding row k0 + q and adding
+// U[r][j] x_j for j from k1 down to r + 1 as each x_j is found. x is read
+// back from global memory by the same workgroup after a barrier.
 __global__ __launch_bounds__(kSubBlock) void k_lu_backsub(const double* __restrict__ band, int W,
                                                           int nf, int bw,
                                                           const int* __restrict__ urow,
                                                           const double* __restrict__ b,
                                                           double* __restrict__ x) {
-  extern __shared__ double s_x[];
-  __shared__ double s_part[kSubBlock / 64];
+  __shared__ double s_tri[64][65];  // U[k0+q][k0+c], 0 outside the band or below the diagonal
+  __shared__ double s_known[64];
+  __shared__ int s_slot[64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int k = nf - 1; k >= 0; --k) {
-    const int slot = urow[k];
-    const double* row = band + (long long)slot * W;
-    const int kendc = min(k + 2 * bw, nf - 1);
-    double s = 0.0;
-    for (int j = k + 1 + threadIdx.x; j <= kendc; j += kSubBlock) s += row[j % W] * s_x[j % W];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-    if (lane == 0) s_part[wave] = s;
+  const int uw = 2 * bw;
+  for (int k1 = nf - 1; k1 >= 0; k1 -= 64) {
+    const int k0 = max(0, k1 - 63), nt = k1 - k0 + 1;
+    if (threadIdx.x < nt) s_slot[threadIdx.x] = urow[k0 + threadIdx.x];
     __syncthreads();
-    if (threadIdx.x == 0) {
-      double t = 0.0;
-      for (int w = 0; w < kSubBlock / 64; ++w) t += s_part[w];
-      const double xk = (b[slot] - t) / row[k % W];
-      s_x[k % W] = xk;
-      x[k] = xk;
+    // the tile's triangle
+    for (int e = threadIdx.x; e < 64 * 64; e += kSubBlock) {
+      const int q = e >> 6, c = e & 63;
+      double v = 0.0;
+      if (q < nt && c < nt && c >= q && c - q <= uw)
+        v = band[(long long)s_slot[q] * W + (k0 + c) % W];
+      s_tri[q][c] = v;
+    }
+    // the solved part of each row's dot
+    for (int q = wave; q < nt; q += kSubBlock / 64) {
+      const int r = k0 + q;
+      const int jend = min(r + uw, nf - 1);
+      const double* row = band + (long long)s_slot[q] * W;
+      double acc = 0.0;
+      for (int j0 = k1 + 1; j0 <= jend; j0 += 64 * kSubStage) {
+        double u[kSubStage], xv[kSubStage];
+#pragma unroll
+        for (int t = 0; t < kSubStage; ++t) {
+          const int j = j0 + lane + 64 * t;
+          u[t] = j <= jend ? row[j % W] : 0.0;
+          xv[t] = j <= jend ? x[j] : 0.0;
+        }
+#pragma unroll
+        for (int t = 0; t < kSubStage; ++t) acc += u[t] * xv[t];
+      }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+      if (lane == 0) s_known[q] = acc;
+    }
+    __syncthreads();
+    if (wave == 0) {
+      const int q = lane;
+      double acc = q < nt ? s_known[q] : 0.0;
+      const double y = q < nt ? b[s_slot[q]] : 0.0;
+      double xq = 0.0;
+      for (int c = nt - 1; c >= 0; --c) {
+        if (q == c) xq = (y - acc) / s_tri[q][q];
+        const double xc = __shfl(xq, c, 64);
+        if (q < c) acc += s_tri[q][c] * xc;
+      }
+      if (q < nt) x[k0 + q] = xq;
     }
     __syncthreads();
   }
@@ -259,8 +359,8 @@ void launch_direct(const Grid& g, const double* ke, const double* xPhys, int law
     hipLaunchKernelGGL(k_lu_step, dim3(blocks), dim3(kStepBlock), lds, st, w.band, W, nf, bw, k,
                        sm, dm, w.urow, w.b, c0, c1, w.info);
   }
-  hipLaunchKernelGGL(k_lu_backsub, dim3(1), dim3(kSubBlock), sizeof(double) * (size_t)W, st,
-                     w.band, W, nf, bw, w.urow, w.b, w.x);
+  hipLaunchKernelGGL(k_lu_backsub, dim3(1), dim3(kSubBlock), 0, st, w.band, W, nf, bw, w.urow,
+                     w.b, w.x);
   hipLaunchKernelGGL(k_scatter_free, dim3(grid_blocks(g.ndof)), dim3(kBlock), 0, st, w.x, w.fi,
                      (long long)g.ndof, u);
 }
