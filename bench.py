@@ -601,11 +601,47 @@ def cpu_reference_literal_c1():
     x = np.full(P.n_elem, 0.3)
     P.setup()
     t0 = time.perf_counter()
-    P.solve_u(x, "direct")
+    u, _ = P.solve_u(x, "direct")
     dt = time.perf_counter() - t0
     return {"seconds_per_solve": dt, "solves_per_run": 51, "runs_per_s": 1.0 / (51 * dt),
             "cores": 1, "kind": "reference-literal (oracle: reference table, SciPy SuperLU)",
-            "grid": [nx, ny, nz]}
+            "grid": [nx, ny, nz]}, u
+
+
+REF_TABLE = Path(__file__).resolve().parent / "tests" / "golden" / "ref_simp_golden.npz"
+
+
+def gpu_reference_literal_c1(n_solves=3):
+    """Reference-literal mode on the device at the reference's CPU-runnable
+    configuration (32x16x16, the reference's own element table -- the data
+    of tests/golden -- clamp law, default cantilever supports and load):
+    exact FP64 banded-LU solves (gp_direct) at the uniform start density,
+    one warm-up solve and n_solves timed ones. A run of the configuration is
+    51 such solves. Returns (figures, u)."""
+    import torch
+    from genpipeline_amd.topology import SIMPSolver
+    from genpipeline_amd.topology._common import cached_problem
+    with np.load(REF_TABLE, allow_pickle=False) as d:
+        table = np.array(d["ke_ref_nu0.3"])
+    nx, ny, nz = 32, 16, 16
+    s = SIMPSolver(nx, ny, nz, ke=table)
+    f, fixed = s._get_bcs(1.0)
+    P = cached_problem(s, "clamp", "cpu", f, fixed, None, 0.3, ke=s._ke_dev)
+    x = torch.full((nx * ny * nz,), 0.3, dtype=torch.float64, device=P.device)
+    u, res = P.direct(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(n_solves):
+        u, res = P.direct(x)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / n_solves
+    out = {"seconds_per_solve": dt, "solves_per_run": 51, "runs_per_s": 1.0 / (51 * dt),
+           "status": res.status, "grid": [nx, ny, nz],
+           "kind": "reference-literal on the device (reference table, FP64 banded LU with "
+                   "partial pivoting, one launch per free DOF)"}
+    u = u.cpu().numpy()
+    P.close()
+    return out, u
 
 
 def cpu_scipy_pcg(w, gpu_pcg_iters, n_iter=30):
@@ -776,7 +812,13 @@ def main():
     print("[bench] accuracy against FP64", file=sys.stderr, flush=True)
     out["extra"]["accuracy"] = accuracy_vs_fp64(P, w, r)
     P.close()
+    u_lit = None
     if not a.no_extra:
+        print("[bench] reference-literal mode on the device", file=sys.stderr, flush=True)
+        try:
+            out["extra"]["reference_literal_c1"], u_lit = gpu_reference_literal_c1()
+        except Exception as e:
+            out["extra"]["reference_literal_c1"] = {"error": f"{type(e).__name__}: {e}"}
         for key, fn in (("batch48", lambda: batch48_legs(a)), ("serve", lambda: serve_legs(a))):
             try:
                 out["extra"][key] = fn()
@@ -786,7 +828,14 @@ def main():
         print("[bench] cpu baseline sample", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(w, r.pcg_iters, w.n_iters, a.cpu_oc_iters)
         try:
-            out["cpu_baseline"]["reference_literal_c1"] = cpu_reference_literal_c1()
+            lit, u_ref = cpu_reference_literal_c1()
+            out["cpu_baseline"]["reference_literal_c1"] = lit
+            g = out["extra"].get("reference_literal_c1", {})
+            if u_lit is not None and "seconds_per_solve" in g:
+                # the same solve on both sides: SuperLU (CPU) against the device
+                g["max_rel_diff_vs_superlu"] = float(np.abs(u_lit - u_ref).max()
+                                                     / np.abs(u_ref).max())
+                g["speedup_vs_superlu"] = lit["seconds_per_solve"] / g["seconds_per_solve"]
         except Exception as e:
             out["cpu_baseline"]["reference_literal_c1"] = {"error": f"{type(e).__name__}: {e}"}
         if a.cpu_scipy:
