@@ -227,9 +227,18 @@ def slot_dims_for(problems: Sequence[Problem], ni: int | None = None):
 def run_batch(problems: Sequence[Problem], n_iters: int, volfracs=None, tile=(8, 32), ni=None,
               graph_ticks: int = 32, **job) -> tuple[list[SimpResult], int]:
     """One job per problem, all slots admitted at once, run to completion.
-    Returns (results, ticks)."""
-    S = Scheduler(slot_dims_for(problems, ni), tile=tile, device=problems[0].device,
-                  graph_ticks=graph_ticks)
+    Returns (results, ticks). tile="auto" takes the workgroup shape and the
+    per-grid sweep depths the autotuner picked for this mix of grid sizes
+    (tuning it first when none is cached); ni is then ignored."""
+    if isinstance(tile, str):
+        if tile != "auto":
+            raise ValueError(f"tile must be one of {TILES} or 'auto'")
+        from . import autotune
+        tile, nis = autotune.sched_shape(problems)
+        dims = [(p.spec.nx, p.spec.ny, p.spec.nz, n) for p, n in zip(problems, nis)]
+    else:
+        dims = slot_dims_for(problems, ni)
+    S = Scheduler(dims, tile=tile, device=problems[0].device, graph_ticks=graph_ticks)
     try:
         vfs = [p.spec.volfrac for p in problems] if volfracs is None else list(volfracs)
         for i, p in enumerate(problems):

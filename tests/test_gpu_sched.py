@@ -304,3 +304,26 @@ def test_warm_compliance_job_equals_alone(gpu):
     _same(S.result(0, st[0]), warm)
     torch.cuda.synchronize()
     S.close()
+
+
+def test_autotuned_shape_batch_equals_alone(gpu):
+    """The autotuner picks the scheduler's tile and a sweep depth per grid size
+    from timed trial runs of the batch; a batch run with tile="auto" then uses
+    that shape, and every job is bitwise Problem.run with the tile and depth
+    the autotuner gave its grid."""
+    from genpipeline_amd import autotune
+    from genpipeline_amd.sched import run_batch
+    shapes = [(16, 8, 8), (10, 6, 6), (16, 8, 8), (12, 6, 5)]
+    Ps = [_problem(gpu, *s, seed=20 + i, volfrac=0.3) for i, s in enumerate(shapes)]
+    tile, depth, timings = autotune.tune_sched(Ps, n_iters=1, tiles=[(8, 32), (14, 18)],
+                                               nis=(2, 4), persist=False, max_iter=5000)
+    grids = {tuple(s) for s in shapes}
+    assert tile in ((8, 32), (14, 18)) and set(depth) == grids
+    assert all(depth[g] in (2, 4, autotune.SCHED_START_NI) for g in grids)
+    # two tiles at the start depth, then up to two depths per grid size
+    assert 2 <= len(timings) <= 2 + 2 * len(grids)
+    assert all(t > 0 for t in timings.values())
+    assert autotune.lookup_sched(shapes, gpu) == (tile, depth)
+    res, _ = run_batch(Ps, 2, tile="auto", graph_ticks=0, tol=1e-6, max_iter=5000)
+    for P, s, r in zip(Ps, shapes, res):
+        _same(r, _alone(P, tile, depth[tuple(s)], 2, tol=1e-6, max_iter=5000))
