@@ -49,6 +49,9 @@ def parse():
     ap.add_argument("--grid", type=int, nargs=3, default=[64, 64, 64])
     ap.add_argument("--oc-iters", type=int, default=100)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--recycle", type=int, default=6,
+                    help="warm starts from the Galerkin combination of the last m solutions "
+                         "(0: from the last solution alone)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extra", action="store_true",
                     help="default workload: skip the 48-problem batch and serving summaries")
@@ -567,20 +570,23 @@ def cpu_baseline(w, gpu_pcg_iters, gpu_oc_iters, n_sample):
     }
 
 
-def accuracy_vs_fp64(P, w, r_head):
+def accuracy_vs_fp64(P, w, r_head, plain_kw=None):
     """The headline solve's error against the same SIMP run in FP64 at PCG
     tolerance 1e-10 (compliance relative, density max-abs), and the error and
     rate of FP64 refinement around FP32 inner solves at an outer tolerance of
-    1e-9, the mode that meets 1e-6 / 1e-5."""
+    1e-9, the mode that meets 1e-6 / 1e-5. plain_kw: also the FP32 run with
+    plain warm starts (each solve from the last u alone)."""
     import torch
     out = {}
     ref = None
-    for name, kw in (("fp64_tol1e-10", dict(precision="fp64", tol=1e-10)),
-                     ("fp32+ir_tol1e-9", dict(precision="fp32", refine=True, tol=1e-9,
-                                              inner_tol=1e-3))):
+    legs = [("fp64_tol1e-10", dict(precision="fp64", tol=1e-10)),
+            ("fp32+ir_tol1e-9", dict(precision="fp32", refine=True, tol=1e-9, inner_tol=1e-3))]
+    if plain_kw is not None:
+        legs.append(("fp32_plain_warm_start", plain_kw))
+    for name, kw in legs:
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        rr = P.run(w.n_iters, max_iter=40000, warm_start=True, **kw)
+        rr = P.run(w.n_iters, **{**dict(max_iter=40000, warm_start=True), **kw})
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         if ref is None:
@@ -737,7 +743,8 @@ def main():
               flush=True)
         del ca, cb
     P = Problem(w.spec, w.fixed, w.f, device=dev)
-    run_kw = dict(tol=w.tol, max_iter=20000, precision="fp32", warm_start=True)
+    run_kw = dict(tol=w.tol, max_iter=20000, precision="fp32", warm_start=True,
+                  recycle=a.recycle)
     for k in range(a.warmup):
         rw = P.run(w.n_iters, **run_kw)
         print(f"[bench] warmup {k}: {rw.seconds:.2f} s, {rw.pcg_iters} PCG its, "
@@ -788,7 +795,9 @@ def main():
             "workload": f"simp_{nx}x{ny}x{nz}_point_loads4_seed{a.seed}",
             "grid": [nx, ny, nz],
             "oc_iters": w.n_iters,
-            "pcg": "jacobi, fp32 vectors, fp64 dots, rel tol 1e-6, warm start",
+            "pcg": ("jacobi, fp32 vectors, fp64 dots, rel tol 1e-6, warm start"
+                    + (f" from the Galerkin combination of the last {a.recycle} solutions"
+                       if a.recycle else "")),
             "material": "hex8 E0=1 Emin=1e-9 nu=0.3 p=3 volfrac=0.3 rmin=1.5",
             "parallelism": "single problem, one GPU",
         },
@@ -818,7 +827,8 @@ def main():
         },
     }
     print("[bench] accuracy against FP64", file=sys.stderr, flush=True)
-    out["extra"]["accuracy"] = accuracy_vs_fp64(P, w, r)
+    plain_kw = dict(run_kw, recycle=0) if a.recycle else None
+    out["extra"]["accuracy"] = accuracy_vs_fp64(P, w, r, plain_kw)
     P.close()
     u_lit = None
     if not a.no_extra:

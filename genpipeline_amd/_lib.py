@@ -50,7 +50,7 @@ class GpSimpOpts(C.Structure):
     _fields_ = [("n_iters", C.c_int32), ("volfrac", C.c_double), ("pcg", GpPcgOpts),
                 ("tol_schedule", C.c_int32), ("early_stop", C.c_int32),
                 ("compliance_each_iter", C.c_int32), ("warm_start", C.c_int32),
-                ("warm_compliance", C.c_int32), ("direct", C.c_int32)]
+                ("warm_compliance", C.c_int32), ("direct", C.c_int32), ("recycle", C.c_int32)]
 
 
 class GpSimpStats(C.Structure):
@@ -101,6 +101,7 @@ SIGNATURES = {
     "gp_oc": [P, P, P, F64, P, P, C.POINTER(GpOcStats), P],
     "gp_compliance": [P, P, I32, C.POINTER(F64), P],
     "gp_simp_run": [P, C.POINTER(GpSimpOpts), P, P, P, C.POINTER(GpSimpStats), P],
+    "gp_recycle_guess": [P, P, P, I32, P, C.POINTER(F64), P],
     "gp_batch_create": [P, I32, I32, I32, C.POINTER(P)],
     "gp_batch_destroy": [P],
     "gp_batch_simp_run": [P, C.POINTER(GpSimpOpts), P, P, P, P, P, P],

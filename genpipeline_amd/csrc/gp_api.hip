@@ -163,6 +163,15 @@ struct gp_problem {
   double* xnew = nullptr;
   double* xphys_new = nullptr;
   double* ce_sum = nullptr;
+  // recycled starts: a ring of the last rc_cap FP32 solutions, their K v at
+  // the current moduli, the Gram partials and the coefficients (one
+  // allocation); rc_n solutions stored, the next one goes to slot rc_head
+  void* rc_mem = nullptr;
+  float* rc_v = nullptr;
+  float* rc_kv = nullptr;
+  double* rc_part = nullptr;
+  double* rc_coef = nullptr;
+  int rc_cap = 0, rc_n = 0, rc_head = 0;
   // exact direct solve: host copy of the supports (the free numbering is
   // built from it) and the banded-LU workspace, one allocation
   std::vector<uint8_t> fixmask_h;
@@ -383,6 +392,47 @@ int ensure_refine_ws(gp_problem* P) {
   if (!P->r64) ALLOC(P->r64, nd * sizeof(double));
   if (!P->y64) ALLOC(P->y64, nd * sizeof(double));
   if (!P->d32) ALLOC(P->d32, nd * sizeof(float));
+  return GP_OK;
+}
+
+// Recycled-start workspace for up to m stored solutions (kept when large
+// enough; the ring restarts empty).
+int ensure_recycle_ws(gp_problem* P, int m) {
+  if (P->rc_cap >= m) return GP_OK;
+  if (P->rc_mem) {
+    HIPCHK(hipFree(P->rc_mem));
+    P->rc_mem = nullptr;
+  }
+  const size_t nd = (size_t)P->g.ndof;
+  const size_t vb = (2 * (size_t)m * nd * sizeof(float) + 255) & ~size_t(255);
+  ALLOC(P->rc_mem, vb + recycle_part_doubles() * sizeof(double));
+  char* b = static_cast<char*>(P->rc_mem);
+  P->rc_v = reinterpret_cast<float*>(b);
+  P->rc_kv = P->rc_v + (size_t)m * nd;
+  P->rc_part = reinterpret_cast<double*>(b + vb);
+  P->rc_coef = P->rc_part + recycle_part_doubles() - (kRecycleMax + 1);
+  P->rc_cap = m;
+  P->rc_n = P->rc_head = 0;
+  return GP_OK;
+}
+
+// Galerkin start over the n vectors V[i] (device, n_dof floats each) at the
+// moduli E, into u0; K V[i] go to the workspace's product slots.
+int recycle_guess(gp_problem* P, const float* E, const float* const* V, int n, float* u0,
+                  hipStream_t s) {
+  if (n < 1 || n > kRecycleMax) return fail(GP_ERR_ARG, "recycle: 1 to %d vectors", kRecycleMax);
+  int rc = ensure_recycle_ws(P, n);
+  if (rc) return rc;
+  const size_t nd = (size_t)P->g.ndof;
+  RecycleSet rs{};
+  rs.n = n;
+  for (int i = 0; i < n; ++i) {
+    rs.v[i] = V[i];
+    rs.kv[i] = P->rc_kv + i * nd;
+    ku_dispatch<float>(P, E, V[i], P->rc_kv + i * nd, true, false, s);
+  }
+  launch_recycle_guess(rs, P->f, (long long)nd, P->rc_part, P->rc_coef, u0, s);
+  LAUNCHCHK();
   return GP_OK;
 }
 
@@ -939,7 +989,7 @@ int gp_problem_destroy(gp_problem* P) {
   hipSetDevice(P->device);
   // hipFree waits for the work that may still use these buffers
   void* dev[] = {P->arena, P->pres, P->ws32, P->ws64, P->ws16, P->r64, P->y64, P->d32, P->E,
-                 P->dc, P->dcf, P->xnew, P->xphys_new, P->ce_sum, P->dws_mem};
+                 P->dc, P->dcf, P->xnew, P->xphys_new, P->ce_sum, P->dws_mem, P->rc_mem};
   for (void* p : dev)
     if (p) hipFree(p);
   if (P->h_sc) hipHostFree(P->h_sc);
@@ -1203,6 +1253,16 @@ int gp_simp_run(gp_problem* P, const gp_simp_opts* o, double* x, double* xPhys, 
   // or direct)
   const bool refine = direct || o->pcg.refine != 0 || o->pcg.precision == GP_F64 ||
                       o->pcg.precision == GP_F16 || o->pcg.precision == GP_BF16;
+  const int recycle = o->recycle;
+  if (recycle < 0 || recycle > kRecycleMax)
+    return fail(GP_ERR_ARG, "recycle must be 0..%d", kRecycleMax);
+  if (recycle > 0 && refine)
+    return fail(GP_ERR_ARG, "recycled starts apply to plain FP32 PCG solves");
+  if (recycle > 0) {
+    rc = ensure_recycle_ws(P, recycle);
+    if (rc) return rc;
+    P->rc_n = P->rc_head = 0;  // a run's history starts empty
+  }
   // u is float64 state; plain FP32 PCG works on a float copy
   float* u32 = nullptr;
   if (!refine) {
@@ -1229,7 +1289,22 @@ int gp_simp_run(gp_problem* P, const gp_simp_opts* o, double* x, double* xPhys, 
       }
     } else {
       r = ensure_pcg_ws(P, GP_F32);
+      if (!r && recycle > 0 && warm && P->rc_n > 0) {
+        // start from the Galerkin combination of the stored solutions
+        // (oldest first) instead of the last one alone
+        const float* V[kRecycleMax];
+        const size_t nd = (size_t)g.ndof;
+        for (int i = 0; i < P->rc_n; ++i)
+          V[i] = P->rc_v + (size_t)((P->rc_head - P->rc_n + i + recycle) % recycle) * nd;
+        r = recycle_guess(P, P->E, V, P->rc_n, u32, s);
+      }
       if (!r) r = pcg_core<float>(P, P->E, P->f, u32, tol, max_iter, warm, ps, s);
+      if (!r && recycle > 0) {
+        HIPCHK(hipMemcpyAsync(P->rc_v + (size_t)P->rc_head * g.ndof, u32, g.ndof * sizeof(float),
+                              hipMemcpyDeviceToDevice, s));
+        P->rc_head = (P->rc_head + 1) % recycle;
+        P->rc_n = std::min(P->rc_n + 1, recycle);
+      }
     }
     if (r) return r;
     st->pcg_iters_total += ps->iters;
@@ -1318,6 +1393,25 @@ int gp_simp_run(gp_problem* P, const gp_simp_opts* o, double* x, double* xPhys, 
   LAUNCHCHK();
   HIPCHK(hipStreamSynchronize(s));
   st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return GP_OK;
+}
+
+int gp_recycle_guess(gp_problem* P, const float* E, const float* const* V, int32_t n, float* u0,
+                     double* coef, void* stream) {
+  if (!P || !E || !V || !u0) return fail(GP_ERR_ARG, "NULL argument");
+  if (!P->has_bcs) return fail(GP_ERR_STATE, "boundary conditions not set");
+  if (n < 1 || n > kRecycleMax) return fail(GP_ERR_ARG, "n must be 1..%d", kRecycleMax);
+  for (int i = 0; i < n; ++i)
+    if (!V[i]) return fail(GP_ERR_ARG, "V[%d] is NULL", i);
+  HIPCHK(hipSetDevice(P->device));
+  hipStream_t s = S(stream);
+  int rc = recycle_guess(P, E, V, n, u0, s);
+  if (rc) return rc;
+  if (coef) {
+    HIPCHK(hipMemcpyAsync(coef, P->rc_coef, (kRecycleMax + 1) * sizeof(double),
+                          hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
   return GP_OK;
 }
 
@@ -1872,6 +1966,8 @@ int gp_batch_simp_run(gp_batch* B, const gp_simp_opts* o, const double* volfracs
   if (!B || !o || !volfracs || !x || !xPhys || !u || !stats) return fail(GP_ERR_ARG, "NULL argument");
   if (o->pcg.refine || o->pcg.precision != GP_F32)
     return fail(GP_ERR_ARG, "batched runs use float32 PCG vectors (precision GP_F32, refine 0)");
+  if (o->recycle != 0)
+    return fail(GP_ERR_ARG, "batched runs start each solve from the last u (recycle 0)");
   const int n = (int)B->probs.size();
   for (int b = 0; b < n; ++b)
     if (!x[b] || !xPhys[b] || !u[b]) return fail(GP_ERR_ARG, "NULL state pointer for problem %d", b);

@@ -448,4 +448,19 @@ inline int grid_blocks(long long n, int per_thread = 1) {
   return (int)b;
 }
 
+// ------------------------------------------------------- recycled starts
+// Galerkin start u0 = V c, (V^T K V) c = V^T f, over up to kRecycleMax stored
+// solutions v_i with their products K v_i at the current moduli
+// (gp_recycle.hip). coef receives c (kRecycleMax entries, 0 for solutions
+// left out as nearly dependent) and then the number kept.
+constexpr int kRecycleMax = 8;
+struct RecycleSet {
+  const float* v[kRecycleMax];
+  const float* kv[kRecycleMax];
+  int n;
+};
+size_t recycle_part_doubles();
+bool launch_recycle_guess(const RecycleSet& rs, const double* f, long long nd, double* part,
+                          double* coef, float* u0, hipStream_t st);
+
 }  // namespace gp

@@ -1,0 +1,185 @@
+// Recycled starting guess for a sequence of related solves.
+//
+// Consecutive OC iterations solve K(E_k) u = f with slowly changing moduli.
+// Instead of starting PCG from the last solution alone, the start is the
+// Galerkin (energy-optimal) combination of the last n solutions
+// V = [v_1 .. v_n]:
+//   u0 = V c,   (V^T K V) c = V^T f,
+// which minimises the K-norm of the error over span(V) (span(V) contains the
+// last solution, so u0 is never worse than it in that norm). Three launches:
+//   k_gram      FP64 per-block partials of v_i.(K v_j) (i <= j) and f.v_i,
+//   k_gram_solve one block: fixed-order sums, then a Cholesky factorisation
+//               with diagonal pivoting that stops at pivots below 1e-12 of
+//               the largest diagonal (nearly dependent solutions are left
+//               out), and the triangular solves for c,
+//   k_combine   u0 = sum_i c_i v_i in FP64, rounded once to float.
+// Every sum runs in a fixed order, so the start is bitwise reproducible.
+#pragma clang fp contract(on)
+#include "gp_bodies.hpp"
+
+namespace gp {
+
+namespace {
+
+constexpr int kGramMax = kRecycleMax * (kRecycleMax + 1) / 2 + kRecycleMax;
+
+constexpr int gram_vals(int n) { return n * (n + 1) / 2 + n; }
+
+// Index of v_i.(K v_j), i <= j, in the packed list; f.v_i follow at
+// n(n+1)/2 + i.
+__host__ __device__ inline int gram_idx(int n, int i, int j) {
+  return i * n - i * (i - 1) / 2 + (j - i);
+}
+
+template <int N>
+__global__ __launch_bounds__(kBlock) void k_gram(RecycleSet rs, const double* __restrict__ f,
+                                                long long nd, double* __restrict__ part) {
+  constexpr int NV = gram_vals(N);
+  double acc[NV];
+#pragma unroll
+  for (int t = 0; t < NV; ++t) acc[t] = 0.0;
+  const long long stride = (long long)gridDim.x * kBlock;
+  for (long long d = blockIdx.x * (long long)kBlock + threadIdx.x; d < nd; d += stride) {
+    double v[N], kv[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      v[i] = (double)rs.v[i][d];
+      kv[i] = (double)rs.kv[i][d];
+    }
+    const double fd = f[d];
+    int t = 0;
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+      for (int j = i; j < N; ++j) acc[t++] += v[i] * kv[j];
+#pragma unroll
+    for (int i = 0; i < N; ++i) acc[t++] += fd * v[i];
+  }
+  __shared__ double sh[NV][kBlock / 64];
+#pragma unroll
+  for (int t = 0; t < NV; ++t) {
+    const double s = wave_sum_nb<kBlock>(acc[t]);
+    if ((threadIdx.x & 63) == 0) sh[t][threadIdx.x >> 6] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    double s = 0.0;
+    for (int w = 0; w < kBlock / 64; ++w) s += sh[threadIdx.x][w];
+    part[(long long)threadIdx.x * gridDim.x + blockIdx.x] = s;
+  }
+}
+
+// One block: reduce the partials (value t by thread t, blocks in order),
+// then thread 0 solves the small system.
+__global__ void k_gram_solve(int n, int nb, const double* __restrict__ part,
+                             double* __restrict__ coef) {
+  __shared__ double val[kGramMax];
+  const int nv = n * (n + 1) / 2 + n;
+  for (int t = threadIdx.x; t < nv; t += blockDim.x) {
+    double s = 0.0;
+    for (int b = 0; b < nb; ++b) s += part[(long long)t * nb + b];
+    val[t] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double M[kRecycleMax][kRecycleMax], r[kRecycleMax], L[kRecycleMax][kRecycleMax];
+  double res[kRecycleMax], c[kRecycleMax];
+  int sel[kRecycleMax];
+  bool used[kRecycleMax];
+  double dmax = 0.0;
+  for (int i = 0; i < n; ++i) {
+    for (int j = i; j < n; ++j) M[i][j] = M[j][i] = val[gram_idx(n, i, j)];
+    r[i] = val[n * (n + 1) / 2 + i];
+    res[i] = M[i][i];
+    used[i] = false;
+    c[i] = 0.0;
+    if (M[i][i] > dmax) dmax = M[i][i];
+    for (int k = 0; k < n; ++k) L[i][k] = 0.0;
+  }
+  // Cholesky with diagonal pivoting: column m takes the unused vector with
+  // the largest remaining (Schur complement) diagonal
+  int m = 0;
+  const double drop = 1e-12 * dmax;
+  while (m < n) {
+    int p = -1;
+    double best = 0.0;
+    for (int i = 0; i < n; ++i)
+      if (!used[i] && res[i] > best) {
+        best = res[i];
+        p = i;
+      }
+    if (p < 0 || !(best > drop)) break;
+    used[p] = true;
+    sel[m] = p;
+    const double lpp = sqrt(best);
+    L[p][m] = lpp;
+    for (int q = 0; q < n; ++q) {
+      if (used[q]) continue;
+      double s = M[q][p];
+      for (int k = 0; k < m; ++k) s -= L[q][k] * L[p][k];
+      L[q][m] = s / lpp;
+      res[q] -= L[q][m] * L[q][m];
+    }
+    ++m;
+  }
+  // (L_s L_s^T) y = r_s over the selected vectors, rows sel[0..m)
+  double z[kRecycleMax];
+  for (int a = 0; a < m; ++a) {
+    double s = r[sel[a]];
+    for (int k = 0; k < a; ++k) s -= L[sel[a]][k] * z[k];
+    z[a] = s / L[sel[a]][a];
+  }
+  for (int a = m - 1; a >= 0; --a) {
+    double s = z[a];
+    for (int k = a + 1; k < m; ++k) s -= L[sel[k]][a] * c[sel[k]];
+    c[sel[a]] = s / L[sel[a]][a];
+  }
+  for (int i = 0; i < kRecycleMax; ++i) coef[i] = i < n ? c[i] : 0.0;
+  coef[kRecycleMax] = (double)m;  // rank used
+}
+
+template <int N>
+__global__ __launch_bounds__(kBlock) void k_combine(RecycleSet rs, const double* __restrict__ coef,
+                                                   long long nd, float* __restrict__ u0) {
+  double c[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) c[i] = coef[i];
+  const long long stride = (long long)gridDim.x * kBlock;
+  for (long long d = blockIdx.x * (long long)kBlock + threadIdx.x; d < nd; d += stride) {
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) s += c[i] * (double)rs.v[i][d];
+    u0[d] = (float)s;
+  }
+}
+
+template <int N>
+void launch_n(const RecycleSet& rs, const double* f, long long nd, double* part, double* coef,
+              float* u0, hipStream_t st) {
+  int nb = grid_blocks(nd);
+  nb = nb < kVecBlocks ? nb : kVecBlocks;
+  hipLaunchKernelGGL((k_gram<N>), dim3(nb), dim3(kBlock), 0, st, rs, f, nd, part);
+  hipLaunchKernelGGL(k_gram_solve, dim3(1), dim3(64), 0, st, N, nb, part, coef);
+  hipLaunchKernelGGL((k_combine<N>), dim3(nb), dim3(kBlock), 0, st, rs, coef, nd, u0);
+}
+
+}  // namespace
+
+size_t recycle_part_doubles() { return (size_t)kGramMax * kVecBlocks + kRecycleMax + 1; }
+
+bool launch_recycle_guess(const RecycleSet& rs, const double* f, long long nd, double* part,
+                          double* coef, float* u0, hipStream_t st) {
+  switch (rs.n) {
+    case 1: launch_n<1>(rs, f, nd, part, coef, u0, st); return true;
+    case 2: launch_n<2>(rs, f, nd, part, coef, u0, st); return true;
+    case 3: launch_n<3>(rs, f, nd, part, coef, u0, st); return true;
+    case 4: launch_n<4>(rs, f, nd, part, coef, u0, st); return true;
+    case 5: launch_n<5>(rs, f, nd, part, coef, u0, st); return true;
+    case 6: launch_n<6>(rs, f, nd, part, coef, u0, st); return true;
+    case 7: launch_n<7>(rs, f, nd, part, coef, u0, st); return true;
+    case 8: launch_n<8>(rs, f, nd, part, coef, u0, st); return true;
+    default: return false;
+  }
+}
+
+}  // namespace gp

@@ -280,6 +280,21 @@ class Problem:
         L.call("gp_pcg", self._h, _ptr(E), _ptr(f), _ptr(u), C.byref(o), C.byref(st), _stream(self.device))
         return u, PcgResult(st.status, st.iters, st.outer, st.relres)
 
+    def recycle_guess(self, E, V):
+        """Galerkin start u0 = V c, (V^T K V) c = V^T f, over the float32
+        vectors V (1..8, n_dof each) at moduli E (gp_recycle_guess). Returns
+        (u0 float32, c as a list of len(V) floats, number of vectors kept)."""
+        E = torch.as_tensor(E, device=self.device).to(torch.float32).contiguous()
+        V = [torch.as_tensor(v, device=self.device).to(torch.float32).contiguous() for v in V]
+        if any(v.numel() != self.spec.n_dof for v in V):
+            raise ValueError("every vector needs n_dof entries")
+        ptrs = (C.c_void_p * len(V))(*[v.data_ptr() for v in V])
+        u0 = torch.empty(self.spec.n_dof, dtype=torch.float32, device=self.device)
+        coef = (C.c_double * 9)()
+        L.call("gp_recycle_guess", self._h, _ptr(E), ptrs, len(V), _ptr(u0), coef,
+               _stream(self.device))
+        return u0, [coef[i] for i in range(len(V))], int(coef[8])
+
     def direct(self, xPhys, f=None):
         """Exact FP64 solve K(xPhys) u = f (banded LU with partial pivoting,
         gp_direct); returns (u float64, PcgResult with status only)."""
@@ -332,11 +347,13 @@ class Problem:
     def run(self, n_iters, x0=None, volfrac=None, tol=1e-6, max_iter=10000, precision="fp32",
             refine=False, inner_tol=1e-3, max_outer=50, warm_start=True, tol_schedule=False,
             early_stop=False, compliance_each_iter=False, u0=None, xphys0=None,
-            warm_compliance=False, direct=False) -> SimpResult:
+            warm_compliance=False, direct=False, recycle=0) -> SimpResult:
         """Whole SIMP loop on the device (gp_simp_run). warm_compliance: the
         compliance solve after each update starts from the previous u
         instead of zero. direct: every solve is the exact FP64 banded LU
-        (the PCG arguments are then ignored)."""
+        (the PCG arguments are then ignored). recycle = m (1..8, plain FP32
+        PCG): each warm-started solve starts from the Galerkin combination of
+        the run's last m solutions instead of the last one alone."""
         sp = self.spec
         vf = sp.volfrac if volfrac is None else volfrac
         if x0 is None:
@@ -364,6 +381,7 @@ class Problem:
         o.warm_start = 1 if warm_start else 0
         o.warm_compliance = 1 if warm_compliance else 0
         o.direct = 1 if direct else 0
+        o.recycle = int(recycle)
         st = L.GpSimpStats()
         L.call("gp_simp_run", self._h, C.byref(o), _ptr(x), _ptr(xPhys), _ptr(u), C.byref(st),
                _stream(self.device))

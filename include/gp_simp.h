@@ -118,6 +118,11 @@ typedef struct gp_simp_opts {
   int32_t direct;             /* 1: every solve is gp_direct's exact FP64     */
                               /*    banded LU (the reference CPU solver's     */
                               /*    SuperLU, simp_solver.py:213); pcg ignored */
+  int32_t recycle;            /* 0: a warm-started solve starts from the last */
+                              /*    u (simp_solver_gpu.py:593,704). 1..8: it  */
+                              /*    starts from the Galerkin combination of   */
+                              /*    the last `recycle` solutions of the run   */
+                              /*    (gp_recycle_guess); plain FP32 PCG only   */
 } gp_simp_opts;
 
 typedef struct gp_simp_stats {
@@ -235,6 +240,19 @@ int gp_compliance(gp_problem* p, const void* u, int32_t dtype, double* out_host,
  * u: float64 device (n_dof) warm-start state in/out.                       */
 int gp_simp_run(gp_problem* p, const gp_simp_opts* opts, double* x, double* xPhys,
                 double* u, gp_simp_stats* stats, void* stream);
+
+/* Recycled starting guess: u0 = V c with (V^T K V) c = V^T f, K the
+ * stiffness at moduli E (float32 device, n_elem) with the supports applied
+ * and f the problem's load, over n (1..8) float32 device vectors V[i]
+ * (n_dof each, zero on fixed DOFs; V is a host array of device pointers).
+ * It minimises the K-norm of the error over span(V); nearly dependent
+ * vectors (Cholesky pivots below 1e-12 of the largest diagonal of V^T K V)
+ * are left out. u0: float32 device (n_dof). coef: NULL, or host doubles
+ * receiving c (8 entries) and the number of vectors kept (after a stream
+ * sync). The start gp_simp_run's recycle option uses; no counterpart in the
+ * reference, which starts from the previous u (simp_solver_gpu.py:593,704). */
+int gp_recycle_guess(gp_problem* p, const float* E, const float* const* V, int32_t n,
+                     float* u0, double* coef, void* stream);
 
 /* ---- batches ----------------------------------------------------------- *
  * Many independent problems (any grid sizes, hex8 KE, BCs set) run as one:

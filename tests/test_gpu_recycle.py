@@ -1,0 +1,122 @@
+"""Recycled starting guesses (gp_recycle_guess, gp_simp_run's recycle
+option): the device start is the Galerkin solution over the stored vectors,
+checked against an FP64 solve of the same small system with torch; nearly
+dependent vectors are left out; SIMP runs with recycled starts need fewer PCG
+iterations and stay as close to the tight FP64 answer as plain warm starts."""
+import numpy as np
+import pytest
+import torch
+
+from genpipeline_amd import Problem, ProblemSpec
+from genpipeline_amd._lib import GpError
+from genpipeline_amd.bcs import clamp_x0
+from genpipeline_amd.problems import point_load_vector
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(dev, nx, ny, nz, seed, volfrac=0.3):
+    rng = np.random.default_rng(seed)
+    f = point_load_vector(nx, ny, nz, rng, 3)
+    return Problem(ProblemSpec(nx, ny, nz, volfrac=volfrac), clamp_x0(nx, ny, nz), f, device=dev)
+
+
+def _free(P):
+    fixed = torch.zeros(P.spec.n_dof, dtype=torch.bool, device=P.device)
+    fixed[torch.from_numpy(clamp_x0(P.spec.nx, P.spec.ny, P.spec.nz)).to(P.device)] = True
+    return ~fixed
+
+
+def _galerkin64(P, E, V):
+    """FP64 Galerkin start with torch: c = (V^T K V)^-1 V^T f."""
+    V64 = [v.double() for v in V]
+    KV = [P.ku(E, v) for v in V64]
+    M = torch.stack([torch.stack([torch.dot(a, kb) for kb in KV]) for a in V64])
+    r = torch.stack([torch.dot(P.f, a) for a in V64])
+    c = torch.linalg.solve(M, r)
+    return c, sum(ci * v for ci, v in zip(c, V64))
+
+
+def test_guess_is_the_galerkin_solution(gpu):
+    P = _problem(gpu, 10, 6, 5, 1)
+    rng = np.random.default_rng(3)
+    E = torch.from_numpy(rng.uniform(1e-3, 1.0, P.spec.n_elem).astype(np.float32)).to(gpu)
+    free = _free(P)
+    V = [torch.from_numpy(rng.standard_normal(P.spec.n_dof).astype(np.float32)).to(gpu) * free
+         for _ in range(4)]
+    u0, c, kept = P.recycle_guess(E, V)
+    assert kept == 4
+    c_ref, u_ref = _galerkin64(P, E, V)
+    np.testing.assert_allclose(np.asarray(c), c_ref.cpu().numpy(), rtol=1e-4, atol=0)
+    scale = float(u_ref.abs().max())
+    assert float((u0.double() - u_ref).abs().max()) <= 1e-5 * scale
+    # Galerkin condition: V^T (f - K u0) = 0
+    res = P.f - P.ku(E, u0.double())
+    proj = torch.stack([torch.dot(v.double(), res) for v in V])
+    rhs = torch.stack([torch.dot(v.double(), P.f) for v in V])
+    assert float(proj.abs().max()) <= 1e-4 * float(rhs.abs().max())
+    # fixed DOFs stay zero
+    assert float(u0[~free].abs().max()) == 0.0
+    # bitwise reproducible
+    u1, c1, _ = P.recycle_guess(E, V)
+    assert torch.equal(u0, u1) and c == c1
+    P.close()
+
+
+def test_nearly_dependent_vectors_are_left_out(gpu):
+    """A vector that is (to float rounding) the sum of two others adds
+    nothing: it is left out, and the start is the two-vector solution."""
+    P = _problem(gpu, 8, 5, 5, 2)
+    rng = np.random.default_rng(4)
+    E = torch.from_numpy(rng.uniform(1e-2, 1.0, P.spec.n_elem).astype(np.float32)).to(gpu)
+    free = _free(P)
+    a, b = [torch.from_numpy(rng.standard_normal(P.spec.n_dof).astype(np.float32)).to(gpu) * free
+            for _ in range(2)]
+    u0, c, kept = P.recycle_guess(E, [a, b, a + b])
+    assert kept == 2
+    _, u_ref = _galerkin64(P, E, [a, b])
+    assert float((u0.double() - u_ref).abs().max()) <= 1e-5 * float(u_ref.abs().max())
+    P.close()
+
+
+def test_zero_vector_gives_zero_start(gpu):
+    P = _problem(gpu, 6, 4, 4, 5)
+    E = torch.full((P.spec.n_elem,), 0.5, dtype=torch.float32, device=gpu)
+    u0, c, kept = P.recycle_guess(E, [torch.zeros(P.spec.n_dof, device=gpu)])
+    assert kept == 0 and c == [0.0] and float(u0.abs().max()) == 0.0
+    P.close()
+
+
+def test_simp_run_with_recycled_starts(gpu):
+    """24x12x12, 3 point loads, 30 OC iterations at FP32 tol 1e-6: recycled
+    starts over 6 solutions take fewer PCG iterations than plain warm starts
+    (14,472 against 15,176 here; the gain grows with the grid and the run:
+    35% on the 64^3 benchmark problem), and both are as close to the FP64
+    tol-1e-10 run (compliance 1e-5 relative, density 1e-2 max-abs: the spread
+    FP32 at tol 1e-6 has on this problem)."""
+    P = _problem(gpu, 24, 12, 12, 7)
+    kw = dict(tol=1e-6, max_iter=20000, precision="fp32", warm_start=True)
+    plain = P.run(30, **kw)
+    rec = P.run(30, recycle=6, **kw)
+    rec2 = P.run(30, recycle=6, **kw)
+    ref = P.run(30, tol=1e-10, max_iter=40000, precision="fp64", warm_start=True)
+    assert plain.status == 0 and rec.status == 0
+    assert rec.pcg_iters < plain.pcg_iters, (rec.pcg_iters, plain.pcg_iters)
+    for r in (plain, rec):
+        assert abs(r.compliance - ref.compliance) <= 1e-5 * abs(ref.compliance)
+        assert float((r.xPhys - ref.xPhys).abs().max()) <= 1e-2
+    # fixed-order sums: a repeated run gives the same bits
+    assert rec2.compliance == rec.compliance and rec2.pcg_iters == rec.pcg_iters
+    assert torch.equal(rec2.xPhys, rec.xPhys)
+    P.close()
+
+
+def test_recycle_argument_errors(gpu):
+    P = _problem(gpu, 6, 4, 4, 6)
+    with pytest.raises(GpError):
+        P.run(2, recycle=4, refine=True)
+    with pytest.raises(GpError):
+        P.run(2, recycle=9)
+    with pytest.raises(GpError):
+        P.run(2, recycle=2, precision="fp64")
+    P.close()
