@@ -54,7 +54,8 @@ def parse():
                          "(0: from the last solution alone)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extra", action="store_true",
-                    help="default workload: skip the 48-problem batch and serving summaries")
+                    help="default workload: skip the side legs (accuracy against FP64, "
+                         "reference-literal solve, 48-problem batch, serving)")
     ap.add_argument("--cpu-oc-iters", type=int, default=2)
     ap.add_argument("--cpu-scipy", action="store_true",
                     help="also time SciPy CSR Jacobi-PCG at the headline grid (about a minute "
@@ -826,9 +827,10 @@ def main():
             "ku_tile_fp32": list(ku_tile),
         },
     }
-    print("[bench] accuracy against FP64", file=sys.stderr, flush=True)
-    plain_kw = dict(run_kw, recycle=0) if a.recycle else None
-    out["extra"]["accuracy"] = accuracy_vs_fp64(P, w, r, plain_kw)
+    if not a.no_extra:
+        print("[bench] accuracy against FP64", file=sys.stderr, flush=True)
+        plain_kw = dict(run_kw, recycle=0) if a.recycle else None
+        out["extra"]["accuracy"] = accuracy_vs_fp64(P, w, r, plain_kw)
     P.close()
     u_lit = None
     if not a.no_extra:
