@@ -69,61 +69,88 @@ __global__ __launch_bounds__(kBlock) void k_gram(RecycleSet rs, const double* __
   }
 }
 
-// One block: reduce the partials (value t by thread t, blocks in order),
-// then thread 0 solves the small system.
-__global__ void k_gram_solve(int n, int nb, const double* __restrict__ part,
-                             double* __restrict__ coef) {
-  __shared__ double val[kGramMax];
-  const int nv = n * (n + 1) / 2 + n;
-  for (int t = threadIdx.x; t < nv; t += blockDim.x) {
-    double s = 0.0;
-    for (int b = 0; b < nb; ++b) s += part[(long long)t * nb + b];
-    val[t] = s;
+// One block: every thread sums its share of the partials of each value
+// (blocks t, t + kBlock, ...), then wave sums and the waves in order give the
+// Gram values; thread 0 solves the small system with register arrays.
+template <int N>
+__global__ __launch_bounds__(kBlock) void k_gram_solve(int nb, const double* __restrict__ part,
+                                                      double* __restrict__ coef) {
+  constexpr int NV = gram_vals(N);
+  double acc[NV];
+#pragma unroll
+  for (int t = 0; t < NV; ++t) acc[t] = 0.0;
+  for (int b = threadIdx.x; b < nb; b += kBlock) {
+#pragma unroll
+    for (int t = 0; t < NV; ++t) acc[t] += part[(long long)t * nb + b];
+  }
+  __shared__ double sh[NV][kBlock / 64];
+#pragma unroll
+  for (int t = 0; t < NV; ++t) {
+    const double v = wave_sum_nb<kBlock>(acc[t]);
+    if ((threadIdx.x & 63) == 0) sh[t][threadIdx.x >> 6] = v;
   }
   __syncthreads();
   if (threadIdx.x != 0) return;
-  double M[kRecycleMax][kRecycleMax], r[kRecycleMax], L[kRecycleMax][kRecycleMax];
-  double res[kRecycleMax], c[kRecycleMax];
-  int sel[kRecycleMax];
-  bool used[kRecycleMax];
+  double val[NV];
+#pragma unroll
+  for (int t = 0; t < NV; ++t) {
+    double v = 0.0;
+    for (int w = 0; w < kBlock / 64; ++w) v += sh[t][w];
+    val[t] = v;
+  }
+  double M[N][N], r[N], L[N][N], res[N], c[N], z[N];
+  int sel[N];
+  bool used[N];
   double dmax = 0.0;
-  for (int i = 0; i < n; ++i) {
-    for (int j = i; j < n; ++j) M[i][j] = M[j][i] = val[gram_idx(n, i, j)];
-    r[i] = val[n * (n + 1) / 2 + i];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+#pragma unroll
+    for (int j = i; j < N; ++j) M[i][j] = M[j][i] = val[gram_idx(N, i, j)];
+    r[i] = val[N * (N + 1) / 2 + i];
     res[i] = M[i][i];
     used[i] = false;
     c[i] = 0.0;
-    if (M[i][i] > dmax) dmax = M[i][i];
-    for (int k = 0; k < n; ++k) L[i][k] = 0.0;
+    sel[i] = 0;
+    dmax = fmax(dmax, M[i][i]);
+#pragma unroll
+    for (int k = 0; k < N; ++k) L[i][k] = 0.0;
   }
   // Cholesky with diagonal pivoting: column m takes the unused vector with
   // the largest remaining (Schur complement) diagonal
-  int m = 0;
   const double drop = 1e-12 * dmax;
-  while (m < n) {
+  int m = 0;
+#pragma unroll
+  for (int col = 0; col < N; ++col) {
     int p = -1;
     double best = 0.0;
-    for (int i = 0; i < n; ++i)
+#pragma unroll
+    for (int i = 0; i < N; ++i)
       if (!used[i] && res[i] > best) {
         best = res[i];
         p = i;
       }
     if (p < 0 || !(best > drop)) break;
     used[p] = true;
-    sel[m] = p;
+    sel[col] = p;
     const double lpp = sqrt(best);
-    L[p][m] = lpp;
-    for (int q = 0; q < n; ++q) {
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+      if (q == p) L[q][col] = lpp;
       if (used[q]) continue;
       double s = M[q][p];
-      for (int k = 0; k < m; ++k) s -= L[q][k] * L[p][k];
-      L[q][m] = s / lpp;
-      res[q] -= L[q][m] * L[q][m];
+#pragma unroll
+      for (int k = 0; k < col; ++k) {
+        double lp = 0.0;
+#pragma unroll
+        for (int i = 0; i < N; ++i) lp = (i == p) ? L[i][k] : lp;
+        s -= L[q][k] * lp;
+      }
+      L[q][col] = s / lpp;
+      res[q] -= L[q][col] * L[q][col];
     }
-    ++m;
+    m = col + 1;
   }
   // (L_s L_s^T) y = r_s over the selected vectors, rows sel[0..m)
-  double z[kRecycleMax];
   for (int a = 0; a < m; ++a) {
     double s = r[sel[a]];
     for (int k = 0; k < a; ++k) s -= L[sel[a]][k] * z[k];
@@ -134,8 +161,8 @@ __global__ void k_gram_solve(int n, int nb, const double* __restrict__ part,
     for (int k = a + 1; k < m; ++k) s -= L[sel[k]][a] * c[sel[k]];
     c[sel[a]] = s / L[sel[a]][a];
   }
-  for (int i = 0; i < kRecycleMax; ++i) coef[i] = i < n ? c[i] : 0.0;
-  coef[kRecycleMax] = (double)m;  // rank used
+  for (int i = 0; i < kRecycleMax; ++i) coef[i] = i < N ? c[i] : 0.0;
+  coef[kRecycleMax] = (double)m;  // vectors kept
 }
 
 template <int N>
@@ -159,7 +186,7 @@ void launch_n(const RecycleSet& rs, const double* f, long long nd, double* part,
   int nb = grid_blocks(nd);
   nb = nb < kVecBlocks ? nb : kVecBlocks;
   hipLaunchKernelGGL((k_gram<N>), dim3(nb), dim3(kBlock), 0, st, rs, f, nd, part);
-  hipLaunchKernelGGL(k_gram_solve, dim3(1), dim3(64), 0, st, N, nb, part, coef);
+  hipLaunchKernelGGL((k_gram_solve<N>), dim3(1), dim3(kBlock), 0, st, nb, part, coef);
   hipLaunchKernelGGL((k_combine<N>), dim3(nb), dim3(kBlock), 0, st, rs, coef, nd, u0);
 }
 
