@@ -571,17 +571,19 @@ def cpu_baseline(w, gpu_pcg_iters, gpu_oc_iters, n_sample):
     }
 
 
-def accuracy_vs_fp64(P, w, r_head, plain_kw=None):
+def accuracy_vs_fp64(P, w, r_head, plain_kw=None, recycle=0):
     """The headline solve's error against the same SIMP run in FP64 at PCG
     tolerance 1e-10 (compliance relative, density max-abs), and the error and
     rate of FP64 refinement around FP32 inner solves at an outer tolerance of
-    1e-9, the mode that meets 1e-6 / 1e-5. plain_kw: also the FP32 run with
-    plain warm starts (each solve from the last u alone)."""
+    1e-9, the mode that meets 1e-6 / 1e-5 (with the headline's recycled
+    starts). plain_kw: also the FP32 run with plain warm starts (each solve
+    from the last u alone)."""
     import torch
     out = {}
     ref = None
     legs = [("fp64_tol1e-10", dict(precision="fp64", tol=1e-10)),
-            ("fp32+ir_tol1e-9", dict(precision="fp32", refine=True, tol=1e-9, inner_tol=1e-3))]
+            ("fp32+ir_tol1e-9", dict(precision="fp32", refine=True, tol=1e-9, inner_tol=1e-3,
+                                     recycle=recycle))]
     if plain_kw is not None:
         legs.append(("fp32_plain_warm_start", plain_kw))
     for name, kw in legs:
@@ -830,7 +832,7 @@ def main():
     if not a.no_extra:
         print("[bench] accuracy against FP64", file=sys.stderr, flush=True)
         plain_kw = dict(run_kw, recycle=0) if a.recycle else None
-        out["extra"]["accuracy"] = accuracy_vs_fp64(P, w, r, plain_kw)
+        out["extra"]["accuracy"] = accuracy_vs_fp64(P, w, r, plain_kw, a.recycle)
     P.close()
     u_lit = None
     if not a.no_extra:

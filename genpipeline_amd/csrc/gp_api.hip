@@ -1256,8 +1256,8 @@ int gp_simp_run(gp_problem* P, const gp_simp_opts* o, double* x, double* xPhys, 
   const int recycle = o->recycle;
   if (recycle < 0 || recycle > kRecycleMax)
     return fail(GP_ERR_ARG, "recycle must be 0..%d", kRecycleMax);
-  if (recycle > 0 && refine)
-    return fail(GP_ERR_ARG, "recycled starts apply to plain FP32 PCG solves");
+  if (recycle > 0 && direct)
+    return fail(GP_ERR_ARG, "recycled starts apply to PCG solves, not the direct solve");
   if (recycle > 0) {
     rc = ensure_recycle_ws(P, recycle);
     if (rc) return rc;
@@ -1271,13 +1271,40 @@ int gp_simp_run(gp_problem* P, const gp_simp_opts* o, double* x, double* xPhys, 
     u32 = P->d32;
     launch_cast_f64_to_f32(u, u32, g.ndof, s);
   }
+  // Recycled starts: the Galerkin combination of the stored solutions
+  // (oldest first) replaces the last solution as the start of a warm solve.
+  // The ring holds float copies; FP64-state solves (refined, 16-bit inner,
+  // FP64 PCG) form the start in the FP32 correction buffer and widen it.
+  auto recycled_start = [&]() -> int {
+    const float* V[kRecycleMax];
+    const size_t nd = (size_t)g.ndof;
+    for (int i = 0; i < P->rc_n; ++i)
+      V[i] = P->rc_v + (size_t)((P->rc_head - P->rc_n + i + recycle) % recycle) * nd;
+    if (!refine) return recycle_guess(P, P->E, V, P->rc_n, u32, s);
+    int r = ensure_refine_ws(P);
+    if (!r) r = recycle_guess(P, P->E, V, P->rc_n, P->d32, s);
+    if (!r) launch_cast_f32_to_f64(P->d32, u, g.ndof, s);
+    return r;
+  };
+  auto store_solution = [&]() -> int {
+    float* slot = P->rc_v + (size_t)P->rc_head * g.ndof;
+    if (refine)
+      launch_cast_f64_to_f32(u, slot, g.ndof, s);
+    else
+      HIPCHK(hipMemcpyAsync(slot, u32, g.ndof * sizeof(float), hipMemcpyDeviceToDevice, s));
+    P->rc_head = (P->rc_head + 1) % recycle;
+    P->rc_n = std::min(P->rc_n + 1, recycle);
+    return GP_OK;
+  };
   auto solve = [&](double tol, int max_iter, bool warm, gp_pcg_stats* ps) -> int {
     gp_pcg_opts po = o->pcg;
     po.tol = tol;
     po.max_iter = max_iter;
     po.use_x0 = warm ? 1 : 0;
-    int r;
+    int r = GP_OK;
     std::memset(ps, 0, sizeof(*ps));
+    if (!direct && recycle > 0 && warm && P->rc_n > 0) r = recycled_start();
+    if (r) return r;
     if (direct) {
       r = direct_solve(P, xPhys, P->f, u, ps, s);
     } else if (refine) {
@@ -1289,23 +1316,9 @@ int gp_simp_run(gp_problem* P, const gp_simp_opts* o, double* x, double* xPhys, 
       }
     } else {
       r = ensure_pcg_ws(P, GP_F32);
-      if (!r && recycle > 0 && warm && P->rc_n > 0) {
-        // start from the Galerkin combination of the stored solutions
-        // (oldest first) instead of the last one alone
-        const float* V[kRecycleMax];
-        const size_t nd = (size_t)g.ndof;
-        for (int i = 0; i < P->rc_n; ++i)
-          V[i] = P->rc_v + (size_t)((P->rc_head - P->rc_n + i + recycle) % recycle) * nd;
-        r = recycle_guess(P, P->E, V, P->rc_n, u32, s);
-      }
       if (!r) r = pcg_core<float>(P, P->E, P->f, u32, tol, max_iter, warm, ps, s);
-      if (!r && recycle > 0) {
-        HIPCHK(hipMemcpyAsync(P->rc_v + (size_t)P->rc_head * g.ndof, u32, g.ndof * sizeof(float),
-                              hipMemcpyDeviceToDevice, s));
-        P->rc_head = (P->rc_head + 1) % recycle;
-        P->rc_n = std::min(P->rc_n + 1, recycle);
-      }
     }
+    if (!r && recycle > 0) r = store_solution();
     if (r) return r;
     st->pcg_iters_total += ps->iters;
     if (ps->status > st->status) st->status = ps->status;

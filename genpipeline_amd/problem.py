@@ -351,9 +351,10 @@ class Problem:
         """Whole SIMP loop on the device (gp_simp_run). warm_compliance: the
         compliance solve after each update starts from the previous u
         instead of zero. direct: every solve is the exact FP64 banded LU
-        (the PCG arguments are then ignored). recycle = m (1..8, plain FP32
-        PCG): each warm-started solve starts from the Galerkin combination of
-        the run's last m solutions instead of the last one alone."""
+        (the PCG arguments are then ignored). recycle = m (1..8, any PCG
+        mode): each warm-started solve starts from the Galerkin combination of
+        the run's last m solutions (float copies) instead of the last one
+        alone."""
         sp = self.spec
         vf = sp.volfrac if volfrac is None else volfrac
         if x0 is None:

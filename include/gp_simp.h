@@ -122,7 +122,8 @@ typedef struct gp_simp_opts {
                               /*    u (simp_solver_gpu.py:593,704). 1..8: it  */
                               /*    starts from the Galerkin combination of   */
                               /*    the last `recycle` solutions of the run   */
-                              /*    (gp_recycle_guess); plain FP32 PCG only   */
+                              /*    (gp_recycle_guess, float copies); any PCG */
+                              /*    mode, not with direct                     */
 } gp_simp_opts;
 
 typedef struct gp_simp_stats {

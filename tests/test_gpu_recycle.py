@@ -111,12 +111,28 @@ def test_simp_run_with_recycled_starts(gpu):
     P.close()
 
 
+def test_refined_run_with_recycled_starts(gpu):
+    """FP64 refinement around FP32 inner solves (outer 1e-9) started from the
+    recycled combination: the same answer as with plain warm starts to the
+    refinement's accuracy, with fewer inner iterations."""
+    P = _problem(gpu, 24, 12, 12, 8)
+    kw = dict(tol=1e-9, max_iter=20000, precision="fp32", refine=True, inner_tol=1e-3,
+              warm_start=True)
+    plain = P.run(20, **kw)
+    rec = P.run(20, recycle=6, **kw)
+    assert plain.status == 0 and rec.status == 0
+    assert rec.pcg_iters < plain.pcg_iters, (rec.pcg_iters, plain.pcg_iters)
+    assert abs(rec.compliance - plain.compliance) <= 1e-8 * abs(plain.compliance)
+    assert float((rec.xPhys - plain.xPhys).abs().max()) <= 1e-6
+    P.close()
+
+
 def test_recycle_argument_errors(gpu):
     P = _problem(gpu, 6, 4, 4, 6)
     with pytest.raises(GpError):
-        P.run(2, recycle=4, refine=True)
-    with pytest.raises(GpError):
         P.run(2, recycle=9)
     with pytest.raises(GpError):
-        P.run(2, recycle=2, precision="fp64")
+        P.run(2, recycle=-1)
+    with pytest.raises(GpError):
+        P.run(2, recycle=2, direct=True)
     P.close()
