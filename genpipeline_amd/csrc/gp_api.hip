@@ -693,15 +693,22 @@ int pcg_core(gp_problem* P, const float* E, const double* f, T* u, double tol, i
   LAUNCHCHK();
 
   const int hint_slot = std::is_same<T, float>::value ? 0 : 1;
-  if (modal)
-    return pcg_drive(
+  if (modal) {
+    // x and p updated two iterations at a time (bitwise the every-iteration
+    // x), then the pending update of an odd last iteration
+    const int rc = pcg_drive(
         P, hint_slot, max_iter, true,
         [&](int k) {
           const int c = k & 1, o = c ^ 1;
           launch_cg_modal(g, tile, kc, prec, E, rb[o], wb[o], sb[c], minv, rb[c], wb[c], sb[o], u,
-                          pc, P->fixmask, sc, P->parts, k, s);
+                          pc, P->fixmask, sc, P->parts, k, s, true);
         },
         st, s);
+    if (rc) return rc;
+    launch_cg_xp_fix<T>(g, sc, rb[0], minv, u, pc, s);
+    LAUNCHCHK();
+    return GP_OK;
+  }
   return pcg_drive(
       P, hint_slot, max_iter, false,
       [&](int k) {

@@ -6,6 +6,8 @@
 // rounding of a body does not depend on where the compiler inlines it (the
 // single-problem and batched kernels then agree bit for bit).
 #pragma clang fp contract(on)
+#include <type_traits>
+
 #include "gp_bodies.hpp"
 #include "gp_modal.hpp"
 
@@ -119,6 +121,7 @@ template <typename T>
 struct StageIn {
   RawNode<T> ro, rh;  // owned node, halo node
   T pv[3], xv[3];     // p, x of the owned node (owned planes only)
+  T rq[3];            // r two iterations back (two-step x and p update)
   uint8_t fm;         // its support bits
 };
 
@@ -135,8 +138,16 @@ struct StageIn {
 // NB: threads per block; NT = TJ x TK of them are active and the rest only
 // join the barriers and add zero to the block sums (a padded block gives the
 // same results as one of exactly NT threads).
+// XP: how this launch treats the iterate x and the search direction p, which
+// feed only the x update. 0: both updated every launch (p_{k-1}, x_k). 1: not
+// touched (no load, no store). 2: two updates at once, p_{k-2}, x_{k-1} from
+// the memory's p_{k-3}, x_{k-2} and from r_{k-2}, still in the r buffer this
+// launch writes, then p_{k-1}, x_k as with XP 0; each value is formed with the
+// same operations as by the launch that would have stored it, so alternating
+// XP 1 and 2 gives bitwise the x of XP 0 at every stored launch, moving 18 B
+// per node and iteration less. Needs S == T (no storage rounding between).
 template <typename T, int TJ, int TK, typename S = T, int NB = whole_waves(TJ * TK),
-          bool UNROLL2 = true>
+          bool UNROLL2 = true, int XP = 0>
 __device__ inline bool body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
                                      const ModalCoef<T>& kc, const float* __restrict__ E,
                                      const T* __restrict__ rp, const T* __restrict__ wp,
@@ -152,6 +163,7 @@ __device__ inline bool body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
   constexpr bool NM = sizeof(T) == 4;
   static_assert(NB >= NT && (NB + 63) / 64 == (NT + 63) / 64, "padding within the last wave");
   static_assert(NB % 64 == 0, "whole waves: the block sums use the DPP wave sum");
+  static_assert(XP == 0 || std::is_same<S, T>::value, "two-step update needs S == T");
   __shared__ T zs[2][3][TJ + 1][TK + 1];
   const int tid = threadIdx.x;
   const bool act = NB == NT || tid < NT;
@@ -173,6 +185,7 @@ __device__ inline bool body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
   const int hb = tid <= TK ? tid : 0;
 
   T alpha = T(0), beta = T(0);
+  T alpha2 = T(0), beta2 = T(0);  // alpha_{k-2}, beta_{k-2} (XP 2)
   auto Eat = [&](int ie) -> T {
     const bool ok = col_ok && (unsigned)ie < (unsigned)g.nx;
     const int e = ok ? ie * g.eI + je * g.eJ + ke : 0;
@@ -191,10 +204,17 @@ __device__ inline bool body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
     for (int q = 0; q < 3; ++q) in.pv[q] = in.xv[q] = T(0);
     in.fm = 0;
     if (fin) {
+      if constexpr (XP != 1) {
 #pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        in.pv[q] = ld_s<T>(p[3 * n + q]);
-        in.xv[q] = ld_s<T>(x[3 * n + q]);
+        for (int q = 0; q < 3; ++q) {
+          in.pv[q] = ld_s<T>(p[3 * n + q]);
+          in.xv[q] = ld_s<T>(x[3 * n + q]);
+        }
+      }
+      if constexpr (XP == 2) {
+        // r_{k-2}: this launch overwrites it with r_k only after this load
+#pragma unroll
+        for (int q = 0; q < 3; ++q) in.rq[q] = rn[3 * n + q];
       }
       if (fixmask != nullptr) in.fm = fixmask[n];
     }
@@ -224,10 +244,20 @@ __device__ inline bool body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
     if (fin) {
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
-        const T pk = o.zp[q] + beta * in.pv[q];  // p_{k-1}
-        const T xk = in.xv[q] + alpha * pk;      // x_k
-        p[3 * n + q] = st_s<S>(pk);
-        x[3 * n + q] = st_s<S>(xk);
+        if constexpr (XP == 2) {
+          const T zq = in.ro.m[q] * in.rq[q];      // z_{k-2}
+          const T pm = zq + beta2 * in.pv[q];      // p_{k-2}
+          const T xm = in.xv[q] + alpha2 * pm;     // x_{k-1}
+          const T pk = o.zp[q] + beta * pm;        // p_{k-1}
+          const T xk = xm + alpha * pk;            // x_k
+          p[3 * n + q] = st_s<S>(pk);
+          x[3 * n + q] = st_s<S>(xk);
+        } else if constexpr (XP == 0) {
+          const T pk = o.zp[q] + beta * in.pv[q];  // p_{k-1}
+          const T xk = in.xv[q] + alpha * pk;      // x_k
+          p[3 * n + q] = st_s<S>(pk);
+          x[3 * n + q] = st_s<S>(xk);
+        }
         rn[3 * n + q] = o.r[q];
         sn[3 * n + q] = o.s[q];
         gk += (double)o.r[q] * (double)o.z[q];
@@ -264,9 +294,15 @@ __device__ inline bool body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
       return true;
     }
     const double alp = gam / den;
+    if constexpr (XP == 2) {
+      // written by launch k-1 into the slots this launch does not write
+      alpha2 = (T)sc->cga[k & 1];
+      beta2 = (T)sc->cgb[k & 1];
+    }
     if (lb == 0 && tid == 0) {
       sc->cgg[(k - 1) & 1] = gam;
       sc->cga[(k - 1) & 1] = alp;
+      sc->cgb[(k - 1) & 1] = bet;
       sc->iter = k - 1;
       sc->rr = rho;
     }

@@ -31,7 +31,7 @@ namespace gp {
 
 // One workgroup of TJ x TK lanes rounded up to whole waves (the block sums
 // use the DPP wave sum); the extra lanes idle.
-template <typename T, int TJ, int TK, typename S>
+template <typename T, int TJ, int TK, typename S, int XP>
 __global__ __launch_bounds__(whole_waves(TJ* TK)) void k_cg_modal(Grid g, int NI, int ntj, int ntk,
                                                     ModalCoef<T> kc,
                                                     const float* __restrict__ E,
@@ -45,29 +45,47 @@ __global__ __launch_bounds__(whole_waves(TJ* TK)) void k_cg_modal(Grid g, int NI
                                                     const uint8_t* __restrict__ fixmask,
                                                     DevScalars* sc, Parts parts, int k) {
   __shared__ CornerLds<T, TJ * TK> cv;
-  body_cg_modal<T, TJ, TK, S>(g, NI, ntj, ntk, kc, E, rp, wp, sp, minv, rn, wn, sn, x, p,
-                              fixmask, sc, parts, k, xcd_block(blockIdx.x, gridDim.x), gridDim.x,
-                              cv);
+  body_cg_modal<T, TJ, TK, S, whole_waves(TJ * TK), true, XP>(
+      g, NI, ntj, ntk, kc, E, rp, wp, sp, minv, rn, wn, sn, x, p, fixmask, sc, parts, k,
+      xcd_block(blockIdx.x, gridDim.x), gridDim.x, cv);
 }
 
-template <int TJ, int TK, typename T, typename S>
+template <int TJ, int TK, typename T, typename S, int XP = 0>
 static void launch_cg_ts(const dim3& grid, int NI, int ntj, int ntk, const Grid& g, const void* kc,
                          const float* E, const void* rp, const void* wp, const void* sp,
                          const void* minv, void* rn, void* wn, void* sn, void* x, void* p,
                          const uint8_t* fixmask, DevScalars* sc, Parts parts, int k,
                          hipStream_t st) {
-  hipLaunchKernelGGL((k_cg_modal<T, TJ, TK, S>), grid, dim3(whole_waves(TJ * TK)), 0, st, g, NI,
-                     ntj, ntk,
+  hipLaunchKernelGGL((k_cg_modal<T, TJ, TK, S, XP>), grid, dim3(whole_waves(TJ * TK)), 0, st, g,
+                     NI, ntj, ntk,
                      *reinterpret_cast<const ModalCoef<T>*>(kc), E, (const T*)rp, (const T*)wp,
                      (const T*)sp, (const T*)minv, (T*)rn, (T*)wn, (T*)sn, (S*)x, (S*)p, fixmask,
                      sc, parts, k);
+}
+
+// Launch k of a solve: with two_step, k = 0 updates x and p as usual, odd k
+// leaves them, even k >= 2 applies both pending updates.
+template <int TJ, int TK, typename T>
+static void launch_cg_xp(bool two_step, const dim3& grid, int NI, int ntj, int ntk, const Grid& g,
+                         const void* kc, const float* E, const void* rp, const void* wp,
+                         const void* sp, const void* minv, void* rn, void* wn, void* sn, void* x,
+                         void* p, const uint8_t* fixmask, DevScalars* sc, Parts parts, int k,
+                         hipStream_t st) {
+#define GP_XARGS grid, NI, ntj, ntk, g, kc, E, rp, wp, sp, minv, rn, wn, sn, x, p, fixmask, sc, parts, k, st
+  if (!two_step || k == 0)
+    launch_cg_ts<TJ, TK, T, T, 0>(GP_XARGS);
+  else if (k & 1)
+    launch_cg_ts<TJ, TK, T, T, 1>(GP_XARGS);
+  else
+    launch_cg_ts<TJ, TK, T, T, 2>(GP_XARGS);
+#undef GP_XARGS
 }
 
 template <int TJ, int TK>
 static void launch_cg_t(const Grid& g, int NI, const void* kc, int prec, const float* E,
                         const void* rp, const void* wp, const void* sp, const void* minv,
                         void* rn, void* wn, void* sn, void* x, void* p, const uint8_t* fixmask,
-                        DevScalars* sc, Parts parts, int k, hipStream_t st) {
+                        DevScalars* sc, Parts parts, int k, hipStream_t st, bool two_step) {
   const int ntj = (g.ny + 1 + (TJ - 2)) / (TJ - 1);
   const int ntk = (g.nz + 1 + (TK - 2)) / (TK - 1);
   const int nti = (g.nx + 1 + NI - 1) / NI;
@@ -75,8 +93,8 @@ static void launch_cg_t(const Grid& g, int NI, const void* kc, int prec, const f
 #define GP_ARGS \
   grid, NI, ntj, ntk, g, kc, E, rp, wp, sp, minv, rn, wn, sn, x, p, fixmask, sc, parts, k, st
   switch (prec) {
-    case GP_F32: launch_cg_ts<TJ, TK, float, float>(GP_ARGS); break;
-    case GP_F64: launch_cg_ts<TJ, TK, double, double>(GP_ARGS); break;
+    case GP_F32: launch_cg_xp<TJ, TK, float>(two_step, GP_ARGS); break;
+    case GP_F64: launch_cg_xp<TJ, TK, double>(two_step, GP_ARGS); break;
     case GP_F16: launch_cg_ts<TJ, TK, float, _Float16>(GP_ARGS); break;
     case GP_BF16: launch_cg_ts<TJ, TK, float, bf16>(GP_ARGS); break;
     default: break;
@@ -87,11 +105,11 @@ static void launch_cg_t(const Grid& g, int NI, const void* kc, int prec, const f
 bool launch_cg_modal(const Grid& g, const KuTile& t, const void* kc, int prec, const float* E,
                      const void* rp, const void* wp, const void* sp, const void* minv, void* rn,
                      void* wn, void* sn, void* x, void* p, const uint8_t* fixmask,
-                     DevScalars* sc, Parts parts, int k, hipStream_t st) {
+                     DevScalars* sc, Parts parts, int k, hipStream_t st, bool two_step) {
 #define GP_TILE(TJ_, TK_)                                                                   \
   if (t.tj == TJ_ && t.tk == TK_) {                                                         \
     launch_cg_t<TJ_, TK_>(g, t.ni, kc, prec, E, rp, wp, sp, minv, rn, wn, sn, x, p, fixmask, \
-                          sc, parts, k, st);                                                \
+                          sc, parts, k, st, two_step);                                      \
     return true;                                                                            \
   }
   GP_TILE(8, 32)
@@ -105,6 +123,42 @@ bool launch_cg_modal(const Grid& g, const KuTile& t, const void* kc, int prec, c
 #undef GP_TILE
   return false;
 }
+
+// The pending x and p updates of an odd final iteration of a two-step solve
+// (see launch_cg_modal), with the operations the sweep uses.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_xp_fix(int nn, const DevScalars* __restrict__ sc,
+                                                     const T* __restrict__ r0,
+                                                     const T* __restrict__ minv,
+                                                     T* __restrict__ x, T* __restrict__ p) {
+  const int K = sc->iter;
+  if ((K & 1) == 0) return;  // iteration K's launch stored x and p
+  const T alpha = (T)sc->cga[(K - 1) & 1], beta = (T)sc->cgb[(K - 1) & 1];
+  constexpr bool NM = sizeof(T) == 4;  // float sweeps: M^-1 per node
+  for (int n = blockIdx.x * kBlock + threadIdx.x; n < nn; n += gridDim.x * kBlock) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const long long d = 3ll * n + q;
+      const T m = NM ? minv[n] : minv[d];
+      const T z = m * r0[d];
+      const T pk = z + beta * p[d];
+      const T xk = x[d] + alpha * pk;
+      p[d] = pk;
+      x[d] = xk;
+    }
+  }
+}
+
+template <typename T>
+void launch_cg_xp_fix(const Grid& g, const DevScalars* sc, const T* r0, const T* minv, T* x,
+                      T* p, hipStream_t st) {
+  hipLaunchKernelGGL((k_cg_xp_fix<T>), dim3(vec_blocks(g)), dim3(kBlock), 0, st, g.nn, sc, r0,
+                     minv, x, p);
+}
+template void launch_cg_xp_fix<float>(const Grid&, const DevScalars*, const float*, const float*,
+                                      float*, float*, hipStream_t);
+template void launch_cg_xp_fix<double>(const Grid&, const DevScalars*, const double*,
+                                       const double*, double*, double*, hipStream_t);
 
 template <typename T, typename S>
 __global__ __launch_bounds__(kBlock) void k_cg_init(int nn, const double* __restrict__ f,

@@ -70,6 +70,7 @@ struct DevScalars {
   double volfrac, move, oc_eps, oc_tol, oc_floor;
   double sum;                // one-shot reduction result
   double cgg[2], cga[2];     // single-kernel PCG: gamma and alpha by parity
+  double cgb[2];             // and beta (read by the two-step x and p update)
   double ir_rr, ir_bb;       // FP64 refinement: |r|^2 and |f_free|^2 of the last residual
   int iter, max_iter, done, status;
   int oc_steps, oc_max, oc_done, nan_flag;
@@ -316,11 +317,21 @@ void launch_cg_init(const Grid& g, const double* f, const T* Ax, bool use_x0, S*
 // One Jacobi-PCG iteration k (Chronopoulos-Gear form) on the modal sweep
 // (prec: GP_F32 / GP_F64, or GP_F16 / GP_BF16 16-bit storage of x and p):
 // reads r_{k-1} (rp), w_{k-1} (wp), s_{k-2} (sp); writes r_k, w_k, s_{k-1}
-// (rn, wn, sn); updates x and p in place.
+// (rn, wn, sn); updates x and p in place. two_step (GP_F32 / GP_F64 only):
+// odd launches leave x and p alone and even launches k >= 2 apply both
+// updates, bitwise the same x at every even launch; after the last launch
+// launch_cg_xp_fix completes an odd final iteration.
 bool launch_cg_modal(const Grid& g, const KuTile& t, const void* kc, int prec, const float* E,
                      const void* rp, const void* wp, const void* sp, const void* minv, void* rn,
                      void* wn, void* sn, void* x, void* p, const uint8_t* fixmask,
-                     DevScalars* sc, Parts parts, int k, hipStream_t st);
+                     DevScalars* sc, Parts parts, int k, hipStream_t st, bool two_step = false);
+// After a two-step solve: when the last iteration K (sc->iter) is odd, x_K =
+// x_{K-1} + alpha_{K-1} p_{K-1} with p_{K-1} = M r_{K-1} + beta_{K-1} p_{K-2},
+// as iteration K's launch would have stored them (r0: the r buffer of even
+// iterations).
+template <typename T>
+void launch_cg_xp_fix(const Grid& g, const DevScalars* sc, const T* r0, const T* minv, T* x,
+                      T* p, hipStream_t st);
 template <typename T>
 void launch_pcg_b(const Grid& g, int it, int nA, T* x, T* r, T* z, const T* p, const T* Ap,
                   const T* minv, Parts parts, DevScalars* sc, hipStream_t st);
