@@ -64,7 +64,7 @@ class GpJob(C.Structure):
                 ("max_iter", C.c_int32), ("tol_schedule", C.c_int32), ("early_stop", C.c_int32),
                 ("compliance_each_iter", C.c_int32), ("warm_start", C.c_int32),
                 ("refine", C.c_int32), ("inner_tol", C.c_double), ("max_outer", C.c_int32),
-                ("warm_compliance", C.c_int32)]
+                ("warm_compliance", C.c_int32), ("recycle", C.c_int32)]
 
 
 class GpSlotStatus(C.Structure):

@@ -2289,7 +2289,15 @@ int gp_sched_admit(gp_sched* Q, int32_t slot, gp_problem* P, const gp_job* job, 
   if (job->refine != 0 && job->refine != 1) return fail(GP_ERR_ARG, "refine must be 0 or 1");
   if (job->refine && (!(job->inner_tol >= 0) || job->max_outer < 0))
     return fail(GP_ERR_ARG, "bad refinement parameters");
+  if (job->recycle < 0 || job->recycle > kRecycleMax)
+    return fail(GP_ERR_ARG, "recycle must be 0..%d", kRecycleMax);
   HIPCHK(hipSetDevice(Q->device));
+  // recycled starts use the problem's own ring (allocated on first use; the
+  // job's history starts empty)
+  if (job->recycle > 0) {
+    int rr = ensure_recycle_ws(P, job->recycle);
+    if (rr) return rr;
+  }
   SchedSlot& q = Q->h_slots[slot];
   const int ni = Q->dims[4 * slot + 3];
   int rc = fill_batch_prob(P, Q->tj, Q->tk, ni, q.P);
@@ -2322,6 +2330,13 @@ int gp_sched_admit(gp_sched* Q, int32_t slot, gp_problem* P, const gp_job* job, 
   q.cfg.inner_tol = job->inner_tol > 0 ? job->inner_tol : 1e-3;
   q.cfg.max_outer = job->max_outer > 0 ? job->max_outer : 50;
   q.cfg.warm_compl = job->warm_compliance != 0 ? 1 : 0;
+  q.cfg.recycle = job->recycle;
+  if (job->recycle > 0) {
+    q.P.rc_v = P->rc_v;
+    q.P.rc_kv = P->rc_kv;
+    q.P.rc_part = P->rc_part;
+    q.P.rc_coef = P->rc_coef;
+  }
   SchedState s0 = idle_state();
   // an FP64-refined job works on the caller's u directly: no float copy in
   s0.phase = job->warm_start && !job->refine ? PH_CAST_IN : PH_STIFF;

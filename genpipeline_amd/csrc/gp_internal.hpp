@@ -101,6 +101,12 @@ struct BatchProb {
   DevScalars* sc;               // PCG scalars
   DevScalars* sc_oc;            // OC / one-shot scalars
   Parts parts;
+  // recycled starts (scheduler jobs with recycle > 0): the problem's ring of
+  // stored float solutions, their products K v, Gram partials, coefficients
+  float* rc_v;
+  float* rc_kv;
+  double* rc_part;
+  double* rc_coef;
 };
 
 // Launch description of a batch: problem table and, per kernel kind, the
@@ -157,7 +163,13 @@ enum SchedPhase : int {
   PH_Z64,        // u64 <- 0
   PH_RES,        // r64 = f - K u64 on free DOFs (FP64), |r|^2 and |f|^2 partials
   PH_RESFIN,     // residual sums; converged, capped, broken down, or an inner solve
-  PH_CORR        // u64 += correction
+  PH_CORR,       // u64 += correction
+  // recycled starts (jobs with recycle > 0; bodies in gp_recycle.hpp)
+  PH_RC_KV,      // K v for stored solution sub (oldest first)
+  PH_RC_GRAM,    // Gram partials v_i.(K v_j), f.v_i
+  PH_RC_SOLVE,   // Gram sums and coefficients (the slot's first block)
+  PH_RC_COMB,    // start = V c (refined jobs: widened into u64)
+  PH_RC_STORE    // the finished solve's solution into the ring
 };
 enum SolveKind : int { SK_OC = 0, SK_COMPL = 1, SK_FINAL = 2 };
 
@@ -166,13 +178,16 @@ struct SchedCfg {
   double tol;
   int refine, max_outer;  // FP64 refinement (outer tolerance: tol, as scheduled)
   double inner_tol;
-  int warm_compl, pad;    // compliance solves warm-started too
+  int warm_compl;         // compliance solves warm-started too
+  int recycle;            // warm solves start from the Galerkin combination of
+                          // the job's last `recycle` solutions (0: the last u)
 };
 
 struct SchedState {
   int phase, k, oc_it, sub;
   int solve, small, status, pcg_total;
   int bis_total, nan, outer, ir_brk;  // refinement sweep; inner breakdown seen
+  int rc_n, rc_head;                  // recycled starts: solutions stored, next slot
   double prev_c, curr_c;
 };
 

@@ -19,6 +19,7 @@
 // partial sum covers the same index set in the same order and a problem's
 // results are bitwise those of Problem.run with the same K.u tile.
 #include "gp_cg_body.hpp"
+#include "gp_recycle.hpp"
 
 namespace gp {
 
@@ -42,6 +43,35 @@ __device__ inline int sched_end_phase(const SchedCfg& c) {
 // First phase after a finished solve.
 __device__ inline int sched_post_solve(const SchedState& s) {
   return s.solve == SK_OC ? PH_SENS : PH_DOT;
+}
+
+// A warm solve of a job with recycled starts begins from the Galerkin
+// combination once the ring holds a solution (gp_simp_run's order: the start
+// replaces the last solution, then the solve proceeds as a warm one).
+__device__ inline bool sched_recycled(const SchedState& s, const SchedCfg& c) {
+  return c.recycle > 0 && s.rc_n > 0 && sched_use_x0(s, c);
+}
+
+// Phase after a solve ends, whatever its status: a recycling job stores the
+// solution in its ring first.
+__device__ inline int sched_solved(const SchedState& s, const SchedCfg& c) {
+  return c.recycle > 0 ? PH_RC_STORE : sched_post_solve(s);
+}
+
+// Ring slot of stored solution i (0 = oldest) and the job's stored set.
+__device__ inline int sched_ring_slot(const SchedState& s, const SchedCfg& c, int i) {
+  return (s.rc_head - s.rc_n + i + c.recycle) % c.recycle;
+}
+__device__ inline RecycleSet sched_ring(const BatchProb& P, const SchedState& s,
+                                        const SchedCfg& c) {
+  RecycleSet rs;
+  rs.n = s.rc_n;
+  const long long nd = P.g.ndof;
+  for (int i = 0; i < kRecycleMax; ++i) {
+    rs.v[i] = i < s.rc_n ? P.rc_v + sched_ring_slot(s, c, i) * nd : nullptr;
+    rs.kv[i] = i < s.rc_n ? P.rc_kv + i * nd : nullptr;
+  }
+  return rs;
 }
 
 // State after a finished OC update (iteration count oc_it already advanced).
@@ -75,10 +105,35 @@ __device__ inline SchedState sched_next(const SchedState& cur, const SchedCfg& C
       } else {
         nx.phase = sched_use_x0(cur, C) ? PH_KX0 : PH_CGINIT;
       }
+      if (sched_recycled(cur, C)) {
+        nx.phase = PH_RC_KV;
+        nx.sub = 0;
+      }
+      break;
+    case PH_RC_KV:
+      if (cur.sub + 1 < cur.rc_n)
+        nx.sub = cur.sub + 1;
+      else
+        nx.phase = PH_RC_GRAM;
+      break;
+    case PH_RC_GRAM:
+      nx.phase = PH_RC_SOLVE;
+      break;
+    case PH_RC_SOLVE:
+      nx.phase = PH_RC_COMB;
+      break;
+    case PH_RC_COMB:
+      // the warm solve from the combined start
+      nx.phase = C.refine ? PH_RES : PH_KX0;
+      break;
+    case PH_RC_STORE:
+      nx.rc_head = (cur.rc_head + 1) % C.recycle;
+      nx.rc_n = cur.rc_n + 1 < C.recycle ? cur.rc_n + 1 : C.recycle;
+      nx.phase = sched_post_solve(cur);
       break;
     case PH_Z64:
       // outer < 0: f has no free part, the solve ends with u = 0
-      nx.phase = cur.outer < 0 ? sched_post_solve(cur) : PH_RES;
+      nx.phase = cur.outer < 0 ? sched_solved(cur, C) : PH_RES;
       break;
     case PH_RES:
       nx.phase = PH_RESFIN;
@@ -101,7 +156,7 @@ __device__ inline SchedState sched_next(const SchedState& cur, const SchedCfg& C
         st = GP_NOT_CONVERGED;
       if (st >= 0) {
         if (st > nx.status) nx.status = st;
-        nx.phase = sched_post_solve(cur);
+        nx.phase = sched_solved(cur, C);
       } else {
         nx.phase = PH_CGINIT;  // inner solve K d = r64 from d = 0
         nx.outer = cur.outer + 1;
@@ -131,7 +186,7 @@ __device__ inline SchedState sched_next(const SchedState& cur, const SchedCfg& C
           nx.phase = PH_CORR;
         } else {
           if (P.sc->status > nx.status) nx.status = P.sc->status;
-          nx.phase = sched_post_solve(cur);
+          nx.phase = sched_solved(cur, C);
         }
       } else {
         nx.k = cur.k + 1;
@@ -214,13 +269,63 @@ __device__ inline SchedState sched_next(const SchedState& cur, const SchedCfg& C
   return nx;
 }
 
+// The recycled-start phases after the products K v (PH_RC_GRAM, PH_RC_SOLVE,
+// PH_RC_COMB, PH_RC_STORE), on the job's blocks lb of nb. Gram and
+// combination run the single-problem pass's gram_blocks layout as virtual
+// blocks, so the start is bitwise gp_simp_run's. gl is the tick's LDS.
+template <int N>
+__device__ void sched_recycle_n(const BatchProb& P, const SchedCfg& C, const SchedState& cur,
+                                int phase, int lb, int nb, GramLds& gl) {
+  const RecycleSet rs = sched_ring(P, cur, C);
+  const long long nd = P.g.ndof;
+  const int gb = gram_blocks(nd);
+  if (phase == PH_RC_GRAM) {
+    for (int v = lb; v < gb; v += nb) body_gram<N>(rs, P.f, nd, P.rc_part, v, gb, gl);
+  } else if (phase == PH_RC_SOLVE) {
+    if (lb == 0) body_gram_solve<N>(gb, P.rc_part, P.rc_coef, gl);
+  } else {
+    // FP32 jobs start from the float combination, refined jobs from it
+    // widened (gp_simp_run widens the float start, and pcg_refine clears the
+    // supported DOFs of a warm start)
+    body_combine<N>(rs, P.rc_coef, nd, C.refine ? nullptr : P.u, C.refine ? P.u64 : nullptr,
+                    P.fixmask, lb, nb);
+  }
+}
+
+// Kept out of line: the tick's sweep phases hold the kernel's register budget.
+__device__ __attribute__((noinline)) void sched_recycle_phase(const BatchProb& P,
+                                                              const SchedCfg& C,
+                                                              const SchedState& cur, int phase,
+                                                              int lb, int nb, GramLds& gl) {
+  if (phase == PH_RC_STORE) {
+    float* __restrict__ slot = P.rc_v + (long long)cur.rc_head * P.g.ndof;
+    for (long long d = lb * (long long)kBlock + threadIdx.x; d < P.g.ndof;
+         d += (long long)nb * kBlock)
+      slot[d] = C.refine ? (float)P.u64[d] : P.u[d];
+    return;
+  }
+  switch (cur.rc_n) {
+    case 1: sched_recycle_n<1>(P, C, cur, phase, lb, nb, gl); break;
+    case 2: sched_recycle_n<2>(P, C, cur, phase, lb, nb, gl); break;
+    case 3: sched_recycle_n<3>(P, C, cur, phase, lb, nb, gl); break;
+    case 4: sched_recycle_n<4>(P, C, cur, phase, lb, nb, gl); break;
+    case 5: sched_recycle_n<5>(P, C, cur, phase, lb, nb, gl); break;
+    case 6: sched_recycle_n<6>(P, C, cur, phase, lb, nb, gl); break;
+    case 7: sched_recycle_n<7>(P, C, cur, phase, lb, nb, gl); break;
+    case 8: sched_recycle_n<8>(P, C, cur, phase, lb, nb, gl); break;
+    default: break;
+  }
+}
+
 // Occupancy floor of 4 waves per SIMD: the 128-register budget the PCG sweep
 // fits in, kept for every phase of the kernel (the FP64 sensitivity phase
 // spills a little; a fourth wave per SIMD is worth more to the sweep).
 template <int TJ, int TK>
 __global__ __launch_bounds__(kBlock, 4) void k_tick(SchedTable tab, int t) {
   static_assert(TJ * TK <= kBlock, "the scheduler runs every body with kBlock threads");
-  __shared__ CornerLds<float, TJ * TK> cv;
+  static_assert(sizeof(CornerLds<float, TJ * TK>) >= sizeof(GramLds),
+                "the recycled-start phases borrow the corner buffer");
+  __shared__ __attribute__((aligned(16))) CornerLds<float, TJ * TK> cv;
   const int xb = xcd_block(blockIdx.x, gridDim.x);
   const int gb = tab.act ? tab.act[xb] : xb;
   const int b = tab.b2s[gb];
@@ -270,8 +375,24 @@ __global__ __launch_bounds__(kBlock, 4) void k_tick(SchedTable tab, int t) {
         P.u64[d] += (double)P.u[d];
       break;
     case PH_KX0:
-      body_ku_sweep<float, TJ, TK, kBlock>(P.g, P.ni, P.ntj, P.ntk, P.mc, P.E, P.u, P.wb[0],
+    case PH_RC_KV: {
+      // w0 = K u of a warm start, or K v of stored solution sub (one sweep
+      // site for both)
+      const float* src = P.u;
+      float* dst = P.wb[0];
+      if (phase == PH_RC_KV) {
+        src = P.rc_v + (long long)sched_ring_slot(cur, S.cfg, cur.sub) * P.g.ndof;
+        dst = P.rc_kv + (long long)cur.sub * P.g.ndof;
+      }
+      body_ku_sweep<float, TJ, TK, kBlock>(P.g, P.ni, P.ntj, P.ntk, P.mc, P.E, src, dst,
                                            P.fixmask, lb, cv);
+      break;
+    }
+    case PH_RC_GRAM:
+    case PH_RC_SOLVE:
+    case PH_RC_COMB:
+    case PH_RC_STORE:
+      sched_recycle_phase(P, S.cfg, cur, phase, lb, nb, *reinterpret_cast<GramLds*>(&cv));
       break;
     case PH_CGINIT: {
       // refined jobs: the inner solve of K d = r64 from d = 0

@@ -115,7 +115,7 @@ class Scheduler:
               early_stop: bool = False, compliance_each_iter: bool = False,
               warm_start: bool = True, x0=None, xphys0=None, u0=None, rhs=None,
               refine: bool = False, inner_tol: float = 1e-3, max_outer: int = 50,
-              warm_compliance: bool = False) -> Job:
+              warm_compliance: bool = False, recycle: int = 0) -> Job:
         """Start a job in a free slot (takes effect at the next tick).
 
         n_iters = 0 runs one solve at the given density (``xphys0``, else
@@ -125,7 +125,10 @@ class Scheduler:
         solve is FP64 iterative refinement around FP32 inner solves at
         ``inner_tol`` (``tol`` the FP64 residual tolerance), bitwise
         ``Problem.run(refine=True)``. ``warm_compliance``: compliance solves
-        start from the previous u (as ``Problem.run``'s option)."""
+        start from the previous u (as ``Problem.run``'s option). ``recycle``
+        = m: warm solves start from the Galerkin combination of the job's last
+        m solutions, bitwise ``Problem.run(recycle=m)``; the ring is the
+        problem's own, so a problem runs in one such job at a time."""
         sp = problem.spec
         vf = sp.volfrac if volfrac is None else float(volfrac)
         dev = self.device
@@ -164,6 +167,7 @@ class Scheduler:
         j.inner_tol = float(inner_tol)
         j.max_outer = int(max_outer)
         j.warm_compliance = 1 if warm_compliance else 0
+        j.recycle = int(recycle)
         with torch.cuda.device(dev):
             L.call("gp_sched_admit", self._h, int(slot), problem.handle, C.byref(j),
                    C.c_void_p(x.data_ptr()), C.c_void_p(xp.data_ptr()), C.c_void_p(u.data_ptr()),

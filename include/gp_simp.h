@@ -303,11 +303,16 @@ typedef struct gp_job {
   double inner_tol;           /* inner PCG tolerance (0: 1e-3)              */
   int32_t max_outer;          /* refinement sweeps cap (0: 50)              */
   int32_t warm_compliance;    /* as gp_simp_opts.warm_compliance            */
+  int32_t recycle;            /* as gp_simp_opts.recycle (0..8). The ring is */
+                              /* the problem's own: a problem runs in one    */
+                              /* such job at a time, and not in gp_simp_run  */
+                              /* meanwhile                                   */
 } gp_job;
 
 typedef struct gp_slot_status {
-  int32_t phase;              /* 0 idle, 18 done, else running (1-17, and   */
-                              /* 19-22 in the refinement sweeps)            */
+  int32_t phase;              /* 0 idle, 18 done, else running (1-17,       */
+                              /* 19-22 in the refinement sweeps, 23-27 in   */
+                              /* recycled starts)                           */
   int32_t done;
   int32_t iters_done;         /* OC iterations completed                    */
   int32_t pcg_iters_total;
