@@ -38,6 +38,7 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "SIMP solves/sec & K·u GB/s vs HBM peak; batched p99 latency @1e-6 compliance"
 SCHED_WARM_NI = 8
+RECYCLE_BATCH = 6  # stored solutions per recycling job in the batch legs
 HBM_PEAK_GBS = 8000.0
 
 
@@ -89,6 +90,9 @@ def parse():
                     help="seconds of arrivals in each below-capacity leg")
     ap.add_argument("--serve-warm-compliance", action="store_true",
                     help="SIMP requests warm-start their compliance solves")
+    ap.add_argument("--serve-recycle", type=int, default=0,
+                    help="SIMP requests start warm OC solves from the Galerkin combination of "
+                         "their last this many solutions (0: the last u)")
     ap.add_argument("--serve-ni", type=int, default=8, help="sweep depth of the serving slots")
     ap.add_argument("--serve-slots", type=int, nargs=2, default=[8, 4],
                     help="serving slots for 32^3 and for 64^3 requests")
@@ -114,11 +118,14 @@ def batch48_legs(a, refine_leg=True, fp64_sample=4, refined_one_by_one=False):
     60 OC iterations, all in one device scheduler (independent progress, ticks
     replayed from a HIP graph), timed beside the same problems run one after
     another through the single-problem FP32 path (PCG at 1e-6). Legs: the
-    FP32 batch at 1e-6, and the mixed-precision batch (FP64 refinement to
-    1e-7 around FP32 inner solves at 1e-2, and to 1e-9 around inner solves
-    at 1e-3). A sample of one problem per size
-    class is also solved in FP64 at 1e-10: each leg's compliance and density
-    error against it."""
+    FP32 batch at 1e-6 with plain warm starts and with recycled starts (the
+    Galerkin combination of each job's last 6 solutions), and the
+    mixed-precision batch with recycled starts (FP64 refinement to 1e-7
+    around FP32 inner solves at 1e-2, and to 1e-9 around inner solves at
+    1e-3). One by one: plain FP32 (the single-problem path as the reference
+    runs it) and FP32 with recycled starts (the same algorithm as the
+    recycled batch). A sample of one problem per size class is also solved in
+    FP64 at 1e-10: each leg's compliance and density error against it."""
     import torch
     from genpipeline_amd import Problem
     from genpipeline_amd.problems import batch_specs
@@ -134,11 +141,12 @@ def batch48_legs(a, refine_leg=True, fp64_sample=4, refined_one_by_one=False):
     shape = {"tile": list(tile), "ni": {f"{P.spec.nx}x{P.spec.ny}x{P.spec.nz}": n
                                         for P, n in zip(Ps, nis)}}
     sched = dict(tile="auto", graph_ticks=32, max_iter=20000)
-    legs = {"fp32": dict(tol=1e-6)}
+    legs = {"fp32": dict(tol=1e-6), "fp32_recycled": dict(tol=1e-6, recycle=RECYCLE_BATCH)}
     if refine_leg:
-        legs["fp32+ir"] = dict(tol=1e-7, refine=True, inner_tol=1e-2)
+        legs["fp32+ir"] = dict(tol=1e-7, refine=True, inner_tol=1e-2, recycle=RECYCLE_BATCH)
         # tight enough for the 1e-6 compliance bar against FP64 at 1e-10
-        legs["fp32+ir_1e-9"] = dict(tol=1e-9, refine=True, inner_tol=1e-3)
+        legs["fp32+ir_1e-9"] = dict(tol=1e-9, refine=True, inner_tol=1e-3,
+                                    recycle=RECYCLE_BATCH)
     run_batch(Ps[:4], 2, tile=tile, ni=SCHED_WARM_NI, graph_ticks=32, max_iter=20000)  # warm-up
     out = {}
     results = {}
@@ -152,6 +160,7 @@ def batch48_legs(a, refine_leg=True, fp64_sample=4, refined_one_by_one=False):
         pcg_bytes = sum(r.pcg_iters * (120 * P.spec.n_nodes + 4 * P.spec.n_elem)
                         for r, P in zip(res, Ps))
         out[name] = {"seconds": t_b, "solves_per_s": len(Ps) / t_b, "ticks": ticks,
+                     "recycle": kw.get("recycle", 0),
                      "us_per_tick": t_b / ticks * 1e6,
                      "pcg_iters_total": sum(r.pcg_iters for r in res),
                      "pcg_algorithmic_gbs": pcg_bytes / t_b / 1e9,
@@ -164,7 +173,19 @@ def batch48_legs(a, refine_leg=True, fp64_sample=4, refined_one_by_one=False):
     torch.cuda.synchronize()
     t_s = time.perf_counter() - t0
     print(f"[bench] batch48 one by one (fp32, 1e-6): {t_s:.2f} s", file=sys.stderr, flush=True)
-    one = {"seconds": t_s, "solves_per_s": len(Ps) / t_s}
+    one = {"seconds": t_s, "solves_per_s": len(Ps) / t_s,
+           "pcg_iters_total": sum(r.pcg_iters for r in seq)}
+    t0 = time.perf_counter()
+    seq_rc = [P.run(a.batch_oc_iters, tol=1e-6, max_iter=20000, precision="fp32",
+                    warm_start=True, recycle=RECYCLE_BATCH) for P in Ps]
+    torch.cuda.synchronize()
+    t_rc = time.perf_counter() - t0
+    print(f"[bench] batch48 one by one (fp32, 1e-6, recycled): {t_rc:.2f} s", file=sys.stderr,
+          flush=True)
+    one_rc = {"seconds": t_rc, "solves_per_s": len(Ps) / t_rc,
+              "pcg_iters_total": sum(r.pcg_iters for r in seq_rc)}
+    out["fp32_recycled"]["speedup_vs_fp32_recycled_one_by_one"] = (
+        t_rc / out["fp32_recycled"]["seconds"])
     one_ir = None
     if refined_one_by_one:
         # the same accuracy one problem at a time: FP64 refinement to 1e-9
@@ -183,6 +204,8 @@ def batch48_legs(a, refine_leg=True, fp64_sample=4, refined_one_by_one=False):
                 t_ir / out["fp32+ir_1e-9"]["seconds"])
     rel = max(abs(x.compliance - y.compliance) / abs(y.compliance)
               for x, y in zip(results["fp32"], seq))
+    rel_rc = max(abs(x.compliance - y.compliance) / abs(y.compliance)
+                 for x, y in zip(results["fp32_recycled"], seq_rc))
     for name in out:
         out[name]["speedup_vs_fp32_one_by_one"] = t_s / out[name]["seconds"]
     # FP64 references for one problem of each size class
@@ -207,23 +230,28 @@ def batch48_legs(a, refine_leg=True, fp64_sample=4, refined_one_by_one=False):
     for P in Ps:
         P.close()
     return {"problems": len(Ps), "oc_iters": a.batch_oc_iters, "scheduler_shape": shape, "legs": out,
-            "fp32_one_by_one": one, "fp32_ir_1e-9_one_by_one": one_ir,
+            "fp32_one_by_one": one, "fp32_recycled_one_by_one": one_rc,
+            "fp32_ir_1e-9_one_by_one": one_ir,
             "max_rel_compliance_diff_fp32_batch_vs_one_by_one": rel,
+            "max_rel_compliance_diff_fp32_recycled_batch_vs_one_by_one": rel_rc,
             "error_vs_fp64_1e-10": err,
             "fp64_sample": [f"{ws[i].spec.nx}x{ws[i].spec.ny}x{ws[i].spec.nz}#{i}" for i in sample]}
 
 
 def bench_batch48(a):
-    """C4 as its own bench line (value: the FP32 batch's solves/s)."""
+    """C4 as its own bench line (value: the FP32 batch with recycled starts,
+    solves/s)."""
     d = batch48_legs(a, refined_one_by_one=True)
-    leg = d["legs"]["fp32"]
+    leg = d["legs"]["fp32_recycled"]
     return _line("SIMP solves/sec (48-problem batch, device scheduler)", leg["solves_per_s"],
                  "solves/s", 1, 1, leg["seconds"] * 1e3, "f32",
                  {"workload": "simp_batch48_c4_seed0", "problems": d["problems"],
                   "oc_iters": a.batch_oc_iters,
                   "grids": "32^3 / 48x24x24 / 64x32x32 / 64^3 by seed",
-                  "pcg": "jacobi, fp32 vectors, fp64 dots, rel tol 1e-6, warm start; "
-                         "fp32+ir leg: FP64 refinement to 1e-7 around FP32 inner solves at 1e-2",
+                  "pcg": "jacobi, fp32 vectors, fp64 dots, rel tol 1e-6, warm start from the "
+                         f"Galerkin combination of each job's last {RECYCLE_BATCH} solutions; "
+                         "fp32 leg: plain warm starts; fp32+ir legs: FP64 refinement to 1e-7 / "
+                         "1e-9 around FP32 inner solves at 1e-2 / 1e-3, recycled starts",
                   "scheduler": "autotuned 256-lane workgroup shape and per-grid sweep depth "
                                "(extra.scheduler_shape), 32-tick HIP graph"},
                  extra=d)
@@ -246,7 +274,8 @@ def serve_legs(a, saturating=(50,)):
     from genpipeline_amd.serve import Server, ServeConfig, latency_summary
 
     n32, n64 = a.serve_slots
-    cfg = ServeConfig(policy=a.serve_policy, simp_warm_compliance=a.serve_warm_compliance)
+    cfg = ServeConfig(policy=a.serve_policy, simp_warm_compliance=a.serve_warm_compliance,
+                      simp_recycle=a.serve_recycle)
     srv = Server({(32, 32, 32): n32, (64, 64, 64): n64}, cfg=cfg, tile=(14, 18), ni=a.serve_ni,
                  chunk_ticks=64)
     srv.run(serving_requests(1000, 6, seed=99))  # warm-up
@@ -290,7 +319,7 @@ def serve_legs(a, saturating=(50,)):
     return {"service_capacity_solves_per_s": mu, "per_rate": per,
             "solves_per_s_overall": total_n / total_t, "slots": {"32^3": n32, "64^3": n64},
             "admission": a.serve_policy, "simp_warm_compliance": a.serve_warm_compliance,
-            "peak_problem_bytes": srv.peak_used}
+            "simp_recycle": a.serve_recycle, "peak_problem_bytes": srv.peak_used}
 
 
 def bench_serve(a):

@@ -80,6 +80,10 @@ class ServeConfig:
     # previous displacement instead of zero (SIMPSolverGPU solves it from zero,
     # simp_solver_gpu.py:747-751; same answers to the tolerance, fewer iterations)
     simp_warm_compliance: bool = False
+    # "simp" requests: each warm OC solve starts from the Galerkin combination
+    # of the request's last simp_recycle solutions (0: from the last u, as
+    # SIMPSolverGPU does); the ring adds 8 * simp_recycle bytes per DOF
+    simp_recycle: int = 0
     max_iter: int = 20000
     # admission order among queued requests that fit: "fifo" (arrival order)
     # or "sjf" (single solves before SIMP runs, each kind in arrival order;
@@ -120,7 +124,18 @@ def job_args(req: ServeRequest, cfg: ServeConfig) -> dict:
     return dict(n_iters=cfg.simp_iters, volfrac=req.volfrac, x0=x, xphys0=x, tol=1e-6,
                 max_iter=cfg.max_iter, tol_schedule=True, early_stop=True,
                 compliance_each_iter=True, warm_start=True,
-                warm_compliance=cfg.simp_warm_compliance)
+                warm_compliance=cfg.simp_warm_compliance, recycle=cfg.simp_recycle)
+
+
+def problem_bytes(shape, kind, cfg: ServeConfig) -> int:
+    """Device bytes a pooled problem of this grid and kind holds, with the
+    recycled-start ring of a "simp" problem (stored solutions and their
+    products in float, the Gram partials)."""
+    spec = _spec(shape, kind)
+    b = footprint_bytes(spec)
+    if kind == "simp" and cfg.simp_recycle > 0:
+        b += 2 * cfg.simp_recycle * spec.n_dof * 4 + (44 * 512 + 9) * 8 + 256
+    return b
 
 
 def make_problem(shape, kind, cfg: ServeConfig, device) -> Problem:
@@ -202,12 +217,12 @@ class Server:
         ps = self.pool.setdefault((shape, kind), [])
         if ps:
             return ps.pop()
-        fp = footprint_bytes(_spec(shape, kind))
+        fp = problem_bytes(shape, kind, self.cfg)
         # make room by freeing idle problems of other kinds or sizes
         for key, idle in self.pool.items():
             while idle and self.used + fp > self.budget:
                 q = idle.pop()
-                self.used -= footprint_bytes(q.spec)
+                self.used -= problem_bytes(*key, self.cfg)
                 q.close()
         if self.used + fp > self.budget:
             return None

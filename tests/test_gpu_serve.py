@@ -25,10 +25,13 @@ def _same(a, b):
         assert torch.equal(a.xPhys, b.xPhys)
 
 
-@pytest.mark.parametrize("policy", ["fifo", "sjf"])
-def test_each_result_same_alone_and_in_mix(gpu, policy):
+@pytest.mark.parametrize("policy,recycle", [("fifo", 0), ("sjf", 0), ("sjf", 4)])
+def test_each_result_same_alone_and_in_mix(gpu, policy, recycle):
+    """recycle: SIMP requests start each warm OC solve from the Galerkin
+    combination of their last solutions (pooled problems carry the ring
+    from request to request; each request's history starts empty)."""
     from genpipeline_amd.serve import Server, ServeConfig, run_alone
-    cfg = ServeConfig(simp_iters=6, policy=policy)
+    cfg = ServeConfig(simp_iters=6, policy=policy, simp_recycle=recycle)
     reqs = _requests(14)
     assert {r.kind for r in reqs} == {"solve", "simp"} and {r.shape for r in reqs} == set(SHAPES)
     srv = Server({SHAPES[0]: 3, SHAPES[1]: 2}, cfg=cfg, tile=(16, 16), ni=4, chunk_ticks=32)
