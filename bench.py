@@ -541,7 +541,8 @@ def bench_ku(a):
     256^3: u and y are 204 MB each in FP32, E 67 MB), so it streams from HBM.
     Reported beside one fused PCG iteration on the same grid and a 1 GiB device
     copy (the bandwidth HBM delivers to a plain read + write stream). Every
-    modal tile and sweep depths 4/8/16 are tried; the best is reported."""
+    modal tile and sweep depths 4/8/16 (K.u) and 8/12/16/24 (PCG iteration,
+    HIP-event timed per launch) are tried; the best is reported."""
     import numpy as np
     import torch
     from genpipeline_amd import Problem, ProblemSpec, autotune
@@ -564,12 +565,15 @@ def bench_ku(a):
     u = torch.from_numpy(rng.standard_normal(spec.n_dof).astype(np.float32)).to(dev)
     ku_bytes = 24 * spec.n_nodes + 4 * spec.n_elem
     it_bytes = 120 * spec.n_nodes + 4 * spec.n_elem
+    f_dense = autotune.dense_rhs(P)  # dense PCG vectors from the first iteration
     ku_t, it_t = {}, {}
     for tj, tk in autotune.TILES:
         for ni in (4, 8, 16):
             P.set_ku_tile(tj, tk, ni, precision="fp32")
             ku_t[(tj, tk, ni)] = _event_us(lambda: P.ku(E, u), 10)
-            it_t[(tj, tk, ni)] = autotune.time_pcg(P, E, (tj, tk, ni), "fp32", iters=60, repeats=1)
+        for ni in (8, 12, 16, 24):
+            it_t[(tj, tk, ni)] = autotune.time_pcg(P, E, (tj, tk, ni), "fp32", iters=120,
+                                                   repeats=2, f=f_dense)
     kb = min(ku_t, key=ku_t.get)
     ib = min(it_t, key=it_t.get)
     P.close()
