@@ -224,7 +224,9 @@ size_t partial_cap(const Grid& g) {
   for (const KuTile& t : kTiles) n = std::max(n, (size_t)modal_blocks(g, t));
   return n + 16;
 }
-constexpr int kRegions = 15;
+// 13 regions of cap entries, then the OC bisection's candidate volumes (two
+// parities of kOcCand x cap)
+constexpr int kRegions = 13 + 2 * kOcCand;
 
 // Khat entry (mode mA comp qA, mode mB comp qB) of a 24x24 table, /64.
 double khat(const double* ke, int mA, int qA, int mB, int qB) {
@@ -833,7 +835,7 @@ int run_oc(gp_problem* P, const double* x, const double* dc, double volfrac, dou
   DevScalars* sc = &P->sc[1];
   launch_oc_init(g, dc, P->spec.oc_variant, volfrac, P->spec.move, P->parts, sc, s);
   const int max_steps = P->spec.oc_variant == GP_OC_GPU ? 30 : 50;
-  for (int i = 0; i < max_steps; ++i)
+  for (int i = 0; i < oc_launches(max_steps); ++i)
     launch_oc_step(g, i, x, dc, P->w, P->pres, P->n_pres, P->parts, sc, s);
   launch_oc_final(g, x, dc, P->pres, sc, xnew, s);
   launch_filter(g, P->st, P->Hs, xnew, xphys, 0, s);
@@ -934,14 +936,13 @@ int gp_problem_create(const gp_spec* spec, gp_problem** out) {
     P->parts.RZ[1] = b + 2 * cap;
     P->parts.RR = b + 3 * cap;
     P->parts.BB = b + 4 * cap;
-    P->parts.O[0] = b + 5 * cap;
-    P->parts.O[1] = b + 6 * cap;
-    P->parts.M = b + 7 * cap;
-    P->parts.S = b + 8 * cap;
+    P->parts.M = b + 5 * cap;
+    P->parts.S = b + 6 * cap;
     for (int q = 0; q < 2; ++q) {
-      P->parts.G[q] = b + (9 + q) * cap;
-      P->parts.D[q] = b + (11 + q) * cap;
-      P->parts.R[q] = b + (13 + q) * cap;
+      P->parts.G[q] = b + (7 + q) * cap;
+      P->parts.D[q] = b + (9 + q) * cap;
+      P->parts.R[q] = b + (11 + q) * cap;
+      P->parts.O[q] = b + (13 + q * kOcCand) * cap;
     }
   }
   {

@@ -186,8 +186,8 @@ void batch_post_solve(const BatchLaunch& bl, int oc_max_steps, hipStream_t st) {
   hipLaunchKernelGGL(k_b_filter, G(bl.tot_elem), dim3(kBlock), 0, st, bl, 1);
   hipLaunchKernelGGL(k_b_oc_max, G(bl.tot_oc), dim3(kBlock), 0, st, bl);
   hipLaunchKernelGGL(k_b_oc_init_fin, G(bl.n), dim3(kBlock), 0, st, bl);
-  for (int s = 0; s < oc_max_steps; ++s)
-    hipLaunchKernelGGL(k_b_oc_step, G(bl.tot_oc), dim3(kBlock), 0, st, bl, s);
+  for (int j = 0; j < oc_launches(oc_max_steps); ++j)
+    hipLaunchKernelGGL(k_b_oc_step, G(bl.tot_oc), dim3(kBlock), 0, st, bl, j);
   hipLaunchKernelGGL(k_b_oc_final, G(bl.tot_elem), dim3(kBlock), 0, st, bl);
   hipLaunchKernelGGL(k_b_filter, G(bl.tot_elem), dim3(kBlock), 0, st, bl, 0);
   hipLaunchKernelGGL(k_b_set_pres, G(bl.tot_elem), dim3(kBlock), 0, st, bl);

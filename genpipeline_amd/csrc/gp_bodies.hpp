@@ -446,63 +446,117 @@ __device__ inline void body_oc_init_fin(const Parts& parts, int nb, int variant,
   }
 }
 
-// Bisection step s (simp_solver.py:230-248). The interval at the start of
-// step s follows from step s-1's interval and its volume sum; the stop test
-// on (l1,l2) precedes the volume evaluation, as in the reference loop body.
-// Returns true when the multiplier is final after this step (every block of
-// the launch reaches the same answer).
-__device__ inline bool body_oc_step(int s, int ne, const double* __restrict__ x,
+// OC bisection (simp_solver.py:230-248), kOcLevels steps per launch. The
+// one-step loop is: at step s with interval (l1, l2), lmid = (l2 + l1) / 2; it
+// stops with lmid if the interval is narrow enough or s is the last permitted
+// step, else it evaluates the volume at lmid and keeps the upper half
+// (l1 = lmid) when the mean density exceeds volfrac, the lower half otherwise.
+// Launch j starts at step s = kOcLevels*j and evaluates the volume at the
+// midpoint of every interval the next kOcLevels steps can reach: a binary
+// tree in heap order, node c with children 2c+1 (lower half) and 2c+2 (upper
+// half); a node where the loop would stop is a leaf and is not evaluated.
+// Launch j+1 reduces those sums and follows the loop's path through the
+// tree. Every interval, midpoint, test and volume sum is formed by the same
+// operations as in the one-step loop (each volume by the same per-block
+// partials and fixed-order reduction), so the multiplier and the step count
+// are bitwise the one-step loop's, in a third of the dependent launches.
+// Returns true when the multiplier is final (every block of the launch
+// reaches the same answer); oc_launches(oc_max) launches always suffice.
+// The one-step loop stops at step s with interval (l1, l2).
+__device__ inline bool oc_leaf(double l1, double l2, int s, const DevScalars* sc) {
+  return (l2 - l1) / (l1 + l2 + sc->oc_eps) < sc->oc_tol || s + 1 >= sc->oc_max;
+}
+
+__device__ inline bool body_oc_step(int j, int ne, const double* __restrict__ x,
                                     const double* __restrict__ dc, const double* __restrict__ w,
                                     const uint8_t* __restrict__ pres, double n_pres,
                                     const Parts& parts, DevScalars* sc, int lb, int nb) {
   if (sc->oc_done) return true;
+  const int s = kOcLevels * j;
   double l1, l2;
-  if (s == 0) {
+  if (j == 0) {
     l1 = sc->ocl1[0];
     l2 = sc->ocl2[0];
   } else {
-    const int pq = (s - 1) & 1;
-    const double pl1 = sc->ocl1[pq], pl2 = sc->ocl2[pq];
-    const double plmid = 0.5 * (pl2 + pl1);
-    const double mean = (reduce_all(parts.O[pq], nb) + n_pres) / (double)ne;
-    if (mean > sc->volfrac) {
-      l1 = plmid;
-      l2 = pl2;
-    } else {
-      l1 = pl1;
-      l2 = plmid;
+    // the loop's path through launch j-1's tree
+    const int pq = (j - 1) & 1;
+    double a = sc->ocl1[pq], b = sc->ocl2[pq];
+    int c = 0;
+    for (int d = 0; d < kOcLevels; ++d) {
+      const double mid = 0.5 * (b + a);
+      if (oc_leaf(a, b, s - kOcLevels + d, sc)) {
+        if (lb == 0 && threadIdx.x == 0) {
+          sc->lmid = mid;
+          sc->oc_steps = s - kOcLevels + d + 1;
+          sc->oc_done = 1;
+        }
+        return true;
+      }
+      const double mean = (reduce_all(parts.O[pq] + (long long)c * nb, nb) + n_pres) / (double)ne;
+      if (mean > sc->volfrac) {
+        a = mid;
+        c = 2 * c + 2;
+      } else {
+        b = mid;
+        c = 2 * c + 1;
+      }
     }
+    l1 = a;
+    l2 = b;
     if (lb == 0 && threadIdx.x == 0) {
-      sc->ocl1[s & 1] = l1;
-      sc->ocl2[s & 1] = l2;
+      sc->ocl1[j & 1] = l1;
+      sc->ocl2[j & 1] = l2;
     }
   }
-  const double lmid = 0.5 * (l2 + l1);
-  if ((l2 - l1) / (l1 + l2 + sc->oc_eps) < sc->oc_tol) {
+  if (oc_leaf(l1, l2, s, sc)) {
     if (lb == 0 && threadIdx.x == 0) {
-      sc->lmid = lmid;
+      sc->lmid = 0.5 * (l2 + l1);
       sc->oc_steps = s + 1;
       sc->oc_done = 1;
     }
     return true;
   }
-  const bool last = s + 1 >= sc->oc_max;
-  if (last && lb == 0 && threadIdx.x == 0) {
-    // last permitted step: its lmid is final (the branch after it is unused)
-    sc->lmid = lmid;
-    sc->oc_steps = s + 1;
-    sc->oc_done = 1;
+  // this launch's tree: intervals, midpoints, evaluated nodes
+  double lo[kOcCand], hi[kOcCand], mid[kOcCand];
+  bool ev[kOcCand];
+  lo[0] = l1;
+  hi[0] = l2;
+#pragma unroll
+  for (int c = 0; c < kOcCand; ++c) {
+    const int d = c >= 3 ? 2 : (c >= 1 ? 1 : 0);  // depth in the tree
+    const bool live = c == 0 || ev[(c - 1) / 2];
+    mid[c] = 0.5 * (hi[c] + lo[c]);
+    ev[c] = live && !oc_leaf(lo[c], hi[c], s + d, sc);
+    if (2 * c + 2 < kOcCand) {
+      lo[2 * c + 1] = lo[c];
+      hi[2 * c + 1] = mid[c];
+      lo[2 * c + 2] = mid[c];
+      hi[2 * c + 2] = hi[c];
+    }
   }
   const double fl = sc->oc_floor, mv = sc->move;
-  double acc = 0.0;
+  double acc[kOcCand];
+#pragma unroll
+  for (int c = 0; c < kOcCand; ++c) acc[c] = 0.0;
   for (int e = lb * kBlock + threadIdx.x; e < ne; e += nb * kBlock) {
-    const double xn = (pres != nullptr && pres[e]) ? 1.0 : oc_xnew(x[e], dc[e], lmid, fl, mv);
-    acc += w[e] * xn;
+    const bool keep = pres != nullptr && pres[e];
+    const double xe = x[e], de = dc[e], we = w[e];
+#pragma unroll
+    for (int c = 0; c < kOcCand; ++c)
+      if (ev[c]) {
+        const double xn = keep ? 1.0 : oc_xnew(xe, de, mid[c], fl, mv);
+        acc[c] += we * xn;
+      }
   }
-  acc = block_sum0(acc);
-  if (threadIdx.x == 0) parts.O[s & 1][lb] = acc;
-  return last;
+#pragma unroll
+  for (int c = 0; c < kOcCand; ++c)
+    if (ev[c]) {
+      const double t = block_sum0(acc[c]);
+      if (threadIdx.x == 0) parts.O[j & 1][(long long)c * nb + lb] = t;
+    }
+  return false;
 }
+static_assert(kOcLevels == 3, "the depth rule in body_oc_step's tree assumes three levels");
 
 __device__ inline void body_oc_final(int ne, const double* __restrict__ x,
                                      const double* __restrict__ dc,

@@ -47,12 +47,18 @@ struct KuTile {
 // finished by its consumers: each consuming block sums the producer's
 // partials in the same fixed order, so all blocks agree bit for bit and no
 // kernel needs device-scope fences or atomics.
+// OC bisection: steps evaluated per launch (body_oc_step), candidate
+// volumes per launch, and the launches that cover oc_max steps.
+constexpr int kOcLevels = 3;
+constexpr int kOcCand = (1 << kOcLevels) - 1;
+__host__ __device__ constexpr int oc_launches(int oc_max) { return (oc_max - 1) / kOcLevels + 2; }
+
 struct Parts {
   double* A;       // p.Ap per K.u block
   double* RZ[2];   // r.z per vector block, by iteration parity
   double* RR;      // r.r per vector block
   double* BB;      // b.b per vector block
-  double* O[2];    // OC volume sums per block, by bisection-step parity
+  double* O[2];    // OC candidate volume sums per block (candidate c at c * nb), by launch parity
   double* M;       // OC max(-dc) per block
   double* S;       // one-shot sums (energy, dot, residual)
   double* G[2];    // single-kernel PCG: (r,z), (z,Az), (r,r) per block, by parity
@@ -65,7 +71,7 @@ struct Parts {
 // are written by block 0 of a kernel and read by later kernels only.
 struct DevScalars {
   double bb, tol2, rr, rz, pAp;
-  double ocl1[2], ocl2[2];   // bisection interval at the start of a step, by parity
+  double ocl1[2], ocl2[2];   // bisection interval at the start of a launch's steps, by parity
   double lmid;               // final multiplier
   double volfrac, move, oc_eps, oc_tol, oc_floor;
   double sum;                // one-shot reduction result
