@@ -83,6 +83,11 @@ def parse():
     ap.add_argument("--mixed-oc-iters", type=int, default=50)
     ap.add_argument("--batch-n", type=int, default=48)
     ap.add_argument("--batch-oc-iters", type=int, default=60)
+    ap.add_argument("--batch-legs", default="fp32,fp32_recycled,fp32+ir,fp32+ir_1e-9",
+                    help="scheduler legs of the batch workload, comma-separated")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="batch workload: the scheduler legs alone (no one-by-one legs, no "
+                         "FP64 sample), e.g. for counter passes")
     ap.add_argument("--serve-n", type=int, default=60, help="requests per saturating rate")
     ap.add_argument("--serve-burst", type=int, default=24,
                     help="requests of the burst that measures the service capacity")
@@ -147,6 +152,11 @@ def batch48_legs(a, refine_leg=True, fp64_sample=4, refined_one_by_one=False):
         # tight enough for the 1e-6 compliance bar against FP64 at 1e-10
         legs["fp32+ir_1e-9"] = dict(tol=1e-9, refine=True, inner_tol=1e-3,
                                     recycle=RECYCLE_BATCH)
+    want = [x for x in a.batch_legs.split(",") if x]
+    unknown = [x for x in want if x not in legs]
+    if unknown:
+        raise SystemExit(f"unknown batch legs {unknown}; choose from {list(legs)}")
+    legs = {k: v for k, v in legs.items() if k in want}
     run_batch(Ps[:4], 2, tile=tile, ni=SCHED_WARM_NI, graph_ticks=32, max_iter=20000)  # warm-up
     out = {}
     results = {}
@@ -167,6 +177,11 @@ def batch48_legs(a, refine_leg=True, fp64_sample=4, refined_one_by_one=False):
                      "worst_status": max(r.status for r in res)}
         print(f"[bench] batch48 scheduler {name}: {t_b:.2f} s, {ticks} ticks", file=sys.stderr,
               flush=True)
+    if a.batch_only:
+        for P in Ps:
+            P.close()
+        return {"problems": len(Ps), "oc_iters": a.batch_oc_iters, "scheduler_shape": shape,
+                "legs": out, "scheduler_legs_only": True}
     t0 = time.perf_counter()
     seq = [P.run(a.batch_oc_iters, tol=1e-6, max_iter=20000, precision="fp32", warm_start=True)
            for P in Ps]
@@ -184,8 +199,9 @@ def batch48_legs(a, refine_leg=True, fp64_sample=4, refined_one_by_one=False):
           flush=True)
     one_rc = {"seconds": t_rc, "solves_per_s": len(Ps) / t_rc,
               "pcg_iters_total": sum(r.pcg_iters for r in seq_rc)}
-    out["fp32_recycled"]["speedup_vs_fp32_recycled_one_by_one"] = (
-        t_rc / out["fp32_recycled"]["seconds"])
+    if "fp32_recycled" in out:
+        out["fp32_recycled"]["speedup_vs_fp32_recycled_one_by_one"] = (
+            t_rc / out["fp32_recycled"]["seconds"])
     one_ir = None
     if refined_one_by_one:
         # the same accuracy one problem at a time: FP64 refinement to 1e-9
@@ -202,10 +218,13 @@ def batch48_legs(a, refine_leg=True, fp64_sample=4, refined_one_by_one=False):
         if "fp32+ir_1e-9" in out:
             out["fp32+ir_1e-9"]["speedup_vs_fp32_ir_1e-9_one_by_one"] = (
                 t_ir / out["fp32+ir_1e-9"]["seconds"])
-    rel = max(abs(x.compliance - y.compliance) / abs(y.compliance)
-              for x, y in zip(results["fp32"], seq))
-    rel_rc = max(abs(x.compliance - y.compliance) / abs(y.compliance)
-                 for x, y in zip(results["fp32_recycled"], seq_rc))
+    def _rel(name, ref):
+        if name not in results:
+            return None
+        return max(abs(x.compliance - y.compliance) / abs(y.compliance)
+                   for x, y in zip(results[name], ref))
+    rel = _rel("fp32", seq)
+    rel_rc = _rel("fp32_recycled", seq_rc)
     for name in out:
         out[name]["speedup_vs_fp32_one_by_one"] = t_s / out[name]["seconds"]
     # FP64 references for one problem of each size class
@@ -242,7 +261,7 @@ def bench_batch48(a):
     """C4 as its own bench line (value: the FP32 batch with recycled starts,
     solves/s)."""
     d = batch48_legs(a, refined_one_by_one=True)
-    leg = d["legs"]["fp32_recycled"]
+    leg = d["legs"].get("fp32_recycled") or next(iter(d["legs"].values()))
     return _line("SIMP solves/sec (48-problem batch, device scheduler)", leg["solves_per_s"],
                  "solves/s", 1, 1, leg["seconds"] * 1e3, "f32",
                  {"workload": "simp_batch48_c4_seed0", "problems": d["problems"],
@@ -762,6 +781,19 @@ def _cpu_model():
     return "unknown"
 
 
+def calibration_add(mib):
+    """A known-size kernel for counter passes: one elementwise add that reads
+    and writes mib MiB of float32."""
+    import torch
+    n = mib * (1 << 20) // 4
+    ca = torch.ones(n, dtype=torch.float32, device=torch.device("cuda", 0))
+    cb = torch.empty_like(ca)
+    torch.add(ca, 1.0, out=cb)
+    torch.cuda.synchronize()
+    print(f"[bench] calibration add: read {4 * n} B, wrote {4 * n} B", file=sys.stderr,
+          flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -769,6 +801,8 @@ def main():
               "problems shard one per GPU, and no multi-GPU figure is measured in this build.",
               file=sys.stderr)
         sys.exit(2)
+    if a.calib_mib:
+        calibration_add(a.calib_mib)
     if a.workload == "batch48":
         print(json.dumps(bench_batch48(a)), flush=True)
         return
@@ -794,15 +828,6 @@ def main():
     nx, ny, nz = a.grid
     w = point_loads(nx, ny, nz, seed=a.seed, n_loads=4, volfrac=0.3, n_iters=a.oc_iters, tol=1e-6)
     dev = torch.device("cuda", 0)
-    if a.calib_mib:
-        n = a.calib_mib * (1 << 20) // 4
-        ca = torch.ones(n, dtype=torch.float32, device=dev)
-        cb = torch.empty_like(ca)
-        torch.add(ca, 1.0, out=cb)
-        torch.cuda.synchronize()
-        print(f"[bench] calibration add: read {4 * n} B, wrote {4 * n} B", file=sys.stderr,
-              flush=True)
-        del ca, cb
     P = Problem(w.spec, w.fixed, w.f, device=dev)
     run_kw = dict(tol=w.tol, max_iter=20000, precision="fp32", warm_start=True,
                   recycle=a.recycle)
