@@ -5,7 +5,9 @@ the shared object is missing or fails to load, every entry point raises.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
+import os
 import threading
 from pathlib import Path
 
@@ -131,6 +133,47 @@ SIGNATURES = {
 }
 EXTRA = {"gp_last_error": ([], C.c_char_p), "gp_version": ([], C.c_int)}
 
+# Device allocator of the library's workspaces: PyTorch's caching allocator
+# (gp_set_allocator), so the framework owns the device memory as with the
+# reference's torch-allocated kernel outputs. GP_TORCH_ALLOCATOR=0 keeps
+# hipMalloc / hipFree.
+GP_ALLOC_FN = C.CFUNCTYPE(C.c_void_p, C.c_size_t, C.c_int32, C.c_void_p)
+GP_FREE_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int32, C.c_void_p)
+SIGNATURES["gp_set_allocator"] = [GP_ALLOC_FN, GP_FREE_FN, C.c_void_p]
+_exiting = False
+
+
+def _torch_alloc(nbytes, device, ctx):
+    try:
+        import torch
+        return torch.cuda.caching_allocator_alloc(int(nbytes), device=int(device))
+    except Exception:
+        return None  # the library reports GP_ERR_OOM
+
+
+def _torch_free(ptr, device, ctx):
+    if _exiting:
+        return  # the process is ending; the allocator may already be gone
+    try:
+        import torch
+        torch.cuda.caching_allocator_delete(ptr)
+    except Exception:
+        pass
+
+
+_TORCH_ALLOC = GP_ALLOC_FN(_torch_alloc)
+_TORCH_FREE = GP_FREE_FN(_torch_free)
+
+
+@atexit.register
+def _at_exit():
+    global _exiting
+    _exiting = True
+
+
+def torch_allocator_enabled() -> bool:
+    return os.environ.get("GP_TORCH_ALLOCATOR", "1") != "0"
+
 
 class GpError(RuntimeError):
     """A non-zero status from the C ABI; ``code`` holds the GP_* value."""
@@ -163,6 +206,10 @@ def load():
             fn = getattr(lib, name)
             fn.argtypes = args
             fn.restype = res
+        if torch_allocator_enabled():
+            rc = lib.gp_set_allocator(_TORCH_ALLOC, _TORCH_FREE, None)
+            if rc != GP_OK:
+                raise GpError(rc, lib.gp_last_error().decode(errors="replace"))
         _lib = lib
         return lib
 

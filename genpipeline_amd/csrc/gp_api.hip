@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -199,13 +200,46 @@ struct gp_problem {
 
 namespace {
 
+// Device allocator: hipMalloc / hipFree, or the caller's pair set by
+// gp_set_allocator (e.g. a framework's caching allocator). g_live counts the
+// allocations alive, so the pair changes only while none is.
+std::atomic<gp_alloc_fn> g_alloc_fn{nullptr};
+std::atomic<gp_free_fn> g_free_fn{nullptr};
+std::atomic<void*> g_alloc_ctx{nullptr};
+std::atomic<long long> g_live{0};
+std::mutex g_alloc_mu;
+
 int alloc(void** p, size_t bytes) {
-  hipError_t e = hipMalloc(p, bytes);
-  if (e != hipSuccess) {
-    *p = nullptr;
-    return fail(GP_ERR_OOM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  const gp_alloc_fn fn = g_alloc_fn.load();
+  if (fn) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    *p = fn(bytes, dev, g_alloc_ctx.load());
+    if (!*p) return fail(GP_ERR_OOM, "the caller's allocator returned NULL for %zu bytes", bytes);
+  } else {
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+      *p = nullptr;
+      return fail(GP_ERR_OOM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
   }
+  ++g_live;
   return GP_OK;
+}
+
+// Frees an allocation made by alloc(). Like hipFree, it first waits for the
+// device's queued work, which may still use the buffer: a caching allocator
+// would otherwise hand the memory out again while a kernel reads it.
+hipError_t dev_free(void* p) {
+  if (!p) return hipSuccess;
+  --g_live;
+  const gp_free_fn fn = g_free_fn.load();
+  if (!fn) return hipFree(p);
+  const hipError_t e = hipDeviceSynchronize();
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+  fn(p, dev, g_alloc_ctx.load());
+  return e;
 }
 #define ALLOC(ptr, bytes)                                      \
   do {                                                         \
@@ -402,7 +436,7 @@ int ensure_refine_ws(gp_problem* P) {
 int ensure_recycle_ws(gp_problem* P, int m) {
   if (P->rc_cap >= m) return GP_OK;
   if (P->rc_mem) {
-    HIPCHK(hipFree(P->rc_mem));
+    HIPCHK(dev_free(P->rc_mem));
     P->rc_mem = nullptr;
   }
   const size_t nd = (size_t)P->g.ndof;
@@ -470,7 +504,7 @@ int ensure_direct_ws(gp_problem* P, hipStream_t s) {
   if (bytes > P->dws_bytes) {
     if (P->dws_mem) {
       HIPCHK(hipStreamSynchronize(s));  // the old workspace may still be in use on s
-      HIPCHK(hipFree(P->dws_mem));
+      HIPCHK(dev_free(P->dws_mem));
       P->dws_mem = nullptr;
       P->dws_bytes = 0;
     }
@@ -859,6 +893,19 @@ int gp_hex8_ke(double nu, double* ke_out) {
 }
 int gp_version(void) { return 100; }
 
+int gp_set_allocator(gp_alloc_fn alloc_fn, gp_free_fn free_fn, void* ctx) {
+  if ((alloc_fn == nullptr) != (free_fn == nullptr))
+    return fail(GP_ERR_ARG, "give both allocator functions, or neither");
+  std::lock_guard<std::mutex> lk(g_alloc_mu);
+  if (g_live.load() != 0)
+    return fail(GP_ERR_STATE, "%lld library allocations are alive; the allocator changes only "
+                "while none is", g_live.load());
+  g_alloc_fn.store(alloc_fn);
+  g_free_fn.store(free_fn);
+  g_alloc_ctx.store(ctx);
+  return GP_OK;
+}
+
 int gp_footprint_bytes(const gp_spec* spec, int32_t precision, size_t* bytes) {
   int rc = validate_spec(spec);
   if (rc) return rc;
@@ -995,11 +1042,11 @@ int gp_problem_create(const gp_spec* spec, gp_problem** out) {
 int gp_problem_destroy(gp_problem* P) {
   if (!P) return GP_OK;
   hipSetDevice(P->device);
-  // hipFree waits for the work that may still use these buffers
+  // dev_free waits for the work that may still use these buffers
   void* dev[] = {P->arena, P->pres, P->ws32, P->ws64, P->ws16, P->r64, P->y64, P->d32, P->E,
                  P->dc, P->dcf, P->xnew, P->xphys_new, P->ce_sum, P->dws_mem, P->rc_mem};
   for (void* p : dev)
-    if (p) hipFree(p);
+    if (p) dev_free(p);
   if (P->h_sc) hipHostFree(P->h_sc);
   if (P->dws_info_h) hipHostFree(P->dws_info_h);
   for (auto& ev : P->ev)
@@ -1038,7 +1085,7 @@ int gp_problem_set_preserved(gp_problem* P, const uint8_t* mask) {
   HIPCHK(hipSetDevice(P->device));
   const Grid& g = P->g;
   if (mask == nullptr) {
-    if (P->pres) HIPCHK(hipFree(P->pres));
+    if (P->pres) HIPCHK(dev_free(P->pres));
     P->pres = nullptr;
     P->n_pres = 0;
   } else {
@@ -1719,7 +1766,7 @@ int gp_csr_pcg(const int64_t* rp, const int64_t* ci, const float* v, const float
   int rc = GP_OK;
   auto cleanup = [&]() {
     for (void* q : {(void*)w.r, (void*)w.z, (void*)w.p, (void*)w.Ap, (void*)w.sc, (void*)w.part})
-      if (q) hipFree(q);
+      if (q) dev_free(q);
     if (h) hipHostFree(h);
   };
   do {
@@ -1971,10 +2018,10 @@ int gp_batch_create(gp_problem* const* probs, int32_t n, int32_t tj, int32_t tk,
 int gp_batch_destroy(gp_batch* B) {
   if (!B) return GP_OK;
   hipSetDevice(B->device);
-  // hipFree waits for the work that may still use these buffers
-  if (B->d_probs) hipFree(B->d_probs);
-  if (B->d_off) hipFree(B->d_off);
-  if (B->d_status) hipFree(B->d_status);
+  // dev_free waits for the work that may still use these buffers
+  if (B->d_probs) dev_free(B->d_probs);
+  if (B->d_off) dev_free(B->d_off);
+  if (B->d_status) dev_free(B->d_status);
   if (B->h_status) hipHostFree(B->h_status);
   for (auto& e : B->ev)
     if (e) hipEventDestroy(e);
@@ -2259,11 +2306,11 @@ int gp_sched_destroy(gp_sched* Q) {
   hipDeviceSynchronize();
   sched_free_graph(Q);
   if (Q->cap) hipStreamDestroy(Q->cap);
-  if (Q->d_slots) hipFree(Q->d_slots);
-  if (Q->d_off) hipFree(Q->d_off);
-  if (Q->d_b2s) hipFree(Q->d_b2s);
-  if (Q->d_act) hipFree(Q->d_act);
-  if (Q->d_status) hipFree(Q->d_status);
+  if (Q->d_slots) dev_free(Q->d_slots);
+  if (Q->d_off) dev_free(Q->d_off);
+  if (Q->d_b2s) dev_free(Q->d_b2s);
+  if (Q->d_act) dev_free(Q->d_act);
+  if (Q->d_status) dev_free(Q->d_status);
   if (Q->h_status) hipHostFree(Q->h_status);
   for (auto& e : Q->ev)
     if (e) hipEventDestroy(e);

@@ -139,6 +139,20 @@ typedef struct gp_simp_stats {
 const char* gp_last_error(void);
 int gp_version(void);                                 /* 100*major + minor */
 
+/* Device allocator of the library's workspaces (problem, batch, scheduler
+ * and CSR-solver buffers). The reference's kernels allocate through
+ * PyTorch (torch::zeros_like, simp_sensitivity_ptx.cu:405,461); this routes
+ * the build's allocations through the caller's allocator in the same way,
+ * e.g. a framework's caching allocator. alloc(bytes, device, ctx) returns
+ * device memory on `device` (the current HIP device) or NULL; release(ptr,
+ * device, ctx) takes it back. The library waits for the device's queued work
+ * before each release, as hipFree does. NULL, NULL restores hipMalloc and
+ * hipFree. Returns GP_ERR_STATE while any allocation made through the
+ * current pair is alive (set it before creating handles).                */
+typedef void* (*gp_alloc_fn)(size_t bytes, int32_t device, void* ctx);
+typedef void (*gp_free_fn)(void* ptr, int32_t device, void* ctx);
+int gp_set_allocator(gp_alloc_fn alloc, gp_free_fn release, void* ctx);
+
 /* Exact unit-cube hex8 element stiffness (E = 1) for Poisson ratio nu into
  * ke_out (host, 24x24 row-major). Replaces SIMPSolver._get_Ke
  * (simp_solver.py:284-1543), whose literal table is not a valid hex8 KE.  */

@@ -1,0 +1,58 @@
+"""The library's device workspaces come from PyTorch's caching allocator
+(gp_set_allocator, registered when the binding loads): a problem's buffers
+count in torch.cuda.memory_allocated and go back when it closes, and the
+allocator cannot change while library allocations are alive."""
+import numpy as np
+import pytest
+import torch
+
+from genpipeline_amd import Problem, ProblemSpec, footprint_bytes
+from genpipeline_amd import _lib as L
+from genpipeline_amd.bcs import clamp_x0
+from genpipeline_amd.problems import point_load_vector
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(dev, nx=16, ny=8, nz=8):
+    f = point_load_vector(nx, ny, nz, np.random.default_rng(0), 2)
+    return Problem(ProblemSpec(nx, ny, nz), clamp_x0(nx, ny, nz), f, device=dev)
+
+
+def test_workspaces_are_torch_allocations(gpu):
+    if not L.torch_allocator_enabled():
+        pytest.skip("GP_TORCH_ALLOCATOR=0: the library uses hipMalloc")
+    torch.cuda.synchronize()
+    before = torch.cuda.memory_allocated(gpu)
+    P = _problem(gpu)
+    torch.cuda.synchronize()
+    during = torch.cuda.memory_allocated(gpu)
+    assert during > before
+    # a solve allocates its PCG workspace lazily, through the same allocator;
+    # the footprint counts the set-up and that workspace
+    x = torch.full((P.spec.n_elem,), 0.3, dtype=torch.float64, device=gpu)
+    u, r = P.pcg(P.stiffness(x), tol=1e-6, max_iter=2000)
+    assert r.status == 0
+    torch.cuda.synchronize()
+    mid = torch.cuda.memory_allocated(gpu)
+    fp = footprint_bytes(P.spec)
+    assert mid - before >= 0.5 * fp, (before, during, mid, fp)
+    with pytest.raises(L.GpError) as e:
+        L.call("gp_set_allocator", L.GP_ALLOC_FN(), L.GP_FREE_FN(), None)
+    assert e.value.code == L.GP_ERR_STATE
+    P.close()
+    torch.cuda.synchronize()
+    # the library's buffers went back to the allocator
+    assert mid - torch.cuda.memory_allocated(gpu) >= 0.5 * fp
+
+
+def test_results_unchanged_by_the_allocator(gpu):
+    """The same solve twice (buffers reused from the cache the second time)
+    gives the same bits."""
+    out = []
+    for _ in range(2):
+        P = _problem(gpu, 12, 6, 6)
+        res = P.run(3, tol=1e-6, max_iter=5000, precision="fp32", warm_start=True)
+        out.append(res)
+        P.close()
+    assert torch.equal(out[0].xPhys, out[1].xPhys) and out[0].compliance == out[1].compliance
