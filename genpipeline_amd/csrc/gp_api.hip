@@ -230,16 +230,22 @@ int alloc(void** p, size_t bytes) {
 // Frees an allocation made by alloc(). Like hipFree, it first waits for the
 // device's queued work, which may still use the buffer: a caching allocator
 // would otherwise hand the memory out again while a kernel reads it.
-hipError_t dev_free(void* p) {
+// synced: the caller has just waited (one wait for several frees).
+hipError_t dev_free(void* p, bool synced = false) {
   if (!p) return hipSuccess;
   --g_live;
   const gp_free_fn fn = g_free_fn.load();
   if (!fn) return hipFree(p);
-  const hipError_t e = hipDeviceSynchronize();
+  const hipError_t e = synced ? hipSuccess : hipDeviceSynchronize();
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) dev = 0;
   fn(p, dev, g_alloc_ctx.load());
   return e;
+}
+
+// One wait before several dev_free(p, true) calls (hipFree waits by itself).
+void sync_for_free() {
+  if (g_free_fn.load()) hipDeviceSynchronize();
 }
 #define ALLOC(ptr, bytes)                                      \
   do {                                                         \
@@ -1042,11 +1048,12 @@ int gp_problem_create(const gp_spec* spec, gp_problem** out) {
 int gp_problem_destroy(gp_problem* P) {
   if (!P) return GP_OK;
   hipSetDevice(P->device);
-  // dev_free waits for the work that may still use these buffers
+  // one wait for the work that may still use these buffers, then the frees
   void* dev[] = {P->arena, P->pres, P->ws32, P->ws64, P->ws16, P->r64, P->y64, P->d32, P->E,
                  P->dc, P->dcf, P->xnew, P->xphys_new, P->ce_sum, P->dws_mem, P->rc_mem};
+  sync_for_free();
   for (void* p : dev)
-    if (p) dev_free(p);
+    if (p) dev_free(p, true);
   if (P->h_sc) hipHostFree(P->h_sc);
   if (P->dws_info_h) hipHostFree(P->dws_info_h);
   for (auto& ev : P->ev)
@@ -1765,8 +1772,9 @@ int gp_csr_pcg(const int64_t* rp, const int64_t* ci, const float* v, const float
   DevScalars* h = nullptr;
   int rc = GP_OK;
   auto cleanup = [&]() {
+    sync_for_free();
     for (void* q : {(void*)w.r, (void*)w.z, (void*)w.p, (void*)w.Ap, (void*)w.sc, (void*)w.part})
-      if (q) dev_free(q);
+      if (q) dev_free(q, true);
     if (h) hipHostFree(h);
   };
   do {
@@ -2018,10 +2026,11 @@ int gp_batch_create(gp_problem* const* probs, int32_t n, int32_t tj, int32_t tk,
 int gp_batch_destroy(gp_batch* B) {
   if (!B) return GP_OK;
   hipSetDevice(B->device);
-  // dev_free waits for the work that may still use these buffers
-  if (B->d_probs) dev_free(B->d_probs);
-  if (B->d_off) dev_free(B->d_off);
-  if (B->d_status) dev_free(B->d_status);
+  // one wait for the work that may still use these buffers, then the frees
+  sync_for_free();
+  if (B->d_probs) dev_free(B->d_probs, true);
+  if (B->d_off) dev_free(B->d_off, true);
+  if (B->d_status) dev_free(B->d_status, true);
   if (B->h_status) hipHostFree(B->h_status);
   for (auto& e : B->ev)
     if (e) hipEventDestroy(e);
@@ -2306,11 +2315,11 @@ int gp_sched_destroy(gp_sched* Q) {
   hipDeviceSynchronize();
   sched_free_graph(Q);
   if (Q->cap) hipStreamDestroy(Q->cap);
-  if (Q->d_slots) dev_free(Q->d_slots);
-  if (Q->d_off) dev_free(Q->d_off);
-  if (Q->d_b2s) dev_free(Q->d_b2s);
-  if (Q->d_act) dev_free(Q->d_act);
-  if (Q->d_status) dev_free(Q->d_status);
+  if (Q->d_slots) dev_free(Q->d_slots, true);  // after the device-wide wait above
+  if (Q->d_off) dev_free(Q->d_off, true);
+  if (Q->d_b2s) dev_free(Q->d_b2s, true);
+  if (Q->d_act) dev_free(Q->d_act, true);
+  if (Q->d_status) dev_free(Q->d_status, true);
   if (Q->h_status) hipHostFree(Q->h_status);
   for (auto& e : Q->ev)
     if (e) hipEventDestroy(e);
