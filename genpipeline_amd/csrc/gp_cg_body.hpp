@@ -435,4 +435,32 @@ __device__ inline bool body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
   return false;
 }
 
+// The pending x and p updates after a solve whose launches alternate XP 1
+// (odd) and XP 2 (even): when its last working launch, K = sc->iter, is odd,
+// p_{K-1} and x_K are formed with the operations the sweep's XP 2 uses, from
+// alpha_{K-1}, beta_{K-1} (stored by launch K) and r_{K-1}, which launch K-1
+// wrote to r0; nodes lb*kBlock + t, + nb*kBlock, ... Nothing is pending after
+// an even K.
+template <typename T>
+__device__ inline void body_cg_xp_fix(int nn, const DevScalars* __restrict__ sc,
+                                      const T* __restrict__ r0, const T* __restrict__ minv,
+                                      T* __restrict__ x, T* __restrict__ p, int lb, int nb) {
+  const int K = sc->iter;
+  if ((K & 1) == 0) return;
+  const T alpha = (T)sc->cga[(K - 1) & 1], beta = (T)sc->cgb[(K - 1) & 1];
+  constexpr bool NM = sizeof(T) == 4;  // float sweeps: M^-1 per node
+  for (int n = lb * kBlock + threadIdx.x; n < nn; n += nb * kBlock) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const long long d = 3ll * n + q;
+      const T m = NM ? minv[n] : minv[d];
+      const T z = m * r0[d];
+      const T pk = z + beta * p[d];
+      const T xk = x[d] + alpha * pk;
+      p[d] = pk;
+      x[d] = xk;
+    }
+  }
+}
+
 }  // namespace gp

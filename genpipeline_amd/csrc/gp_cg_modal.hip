@@ -125,28 +125,13 @@ bool launch_cg_modal(const Grid& g, const KuTile& t, const void* kc, int prec, c
 }
 
 // The pending x and p updates of an odd final iteration of a two-step solve
-// (see launch_cg_modal), with the operations the sweep uses.
+// (see launch_cg_modal).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_cg_xp_fix(int nn, const DevScalars* __restrict__ sc,
                                                      const T* __restrict__ r0,
                                                      const T* __restrict__ minv,
                                                      T* __restrict__ x, T* __restrict__ p) {
-  const int K = sc->iter;
-  if ((K & 1) == 0) return;  // iteration K's launch stored x and p
-  const T alpha = (T)sc->cga[(K - 1) & 1], beta = (T)sc->cgb[(K - 1) & 1];
-  constexpr bool NM = sizeof(T) == 4;  // float sweeps: M^-1 per node
-  for (int n = blockIdx.x * kBlock + threadIdx.x; n < nn; n += gridDim.x * kBlock) {
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      const long long d = 3ll * n + q;
-      const T m = NM ? minv[n] : minv[d];
-      const T z = m * r0[d];
-      const T pk = z + beta * p[d];
-      const T xk = x[d] + alpha * pk;
-      p[d] = pk;
-      x[d] = xk;
-    }
-  }
+  body_cg_xp_fix<T>(nn, sc, r0, minv, x, p, blockIdx.x, gridDim.x);
 }
 
 template <typename T>

@@ -175,7 +175,10 @@ enum SchedPhase : int {
   PH_RC_GRAM,    // Gram partials v_i.(K v_j), f.v_i
   PH_RC_SOLVE,   // Gram sums and coefficients (the slot's first block)
   PH_RC_COMB,    // start = V c (refined jobs: widened into u64)
-  PH_RC_STORE    // the finished solve's solution into the ring
+  PH_RC_STORE,   // the finished solve's solution into the ring
+  // x and p two iterations at a time (see k_tick)
+  PH_ALIGN,      // one tick's wait so that PCG launch k runs on a tick of k's parity
+  PH_XPFIX       // pending x and p updates of an odd last PCG iteration
 };
 enum SolveKind : int { SK_OC = 0, SK_COMPL = 1, SK_FINAL = 2 };
 

@@ -85,8 +85,9 @@ __device__ inline void sched_after_update(SchedState& nx, const SchedCfg& c) {
 
 // Next state of a slot after tick work; run by the slot's lead thread only.
 // fin: the PCG solve (PH_CG) or the bisection (PH_OCSTEP) finished this tick.
+// par: this tick's parity.
 __device__ inline SchedState sched_next(const SchedState& cur, const SchedCfg& C,
-                                        const BatchProb& P, bool fin) {
+                                        const BatchProb& P, bool fin, int par) {
   SchedState nx = cur;
   switch (cur.phase) {
     case PH_CAST_IN:
@@ -173,23 +174,31 @@ __device__ inline SchedState sched_next(const SchedState& cur, const SchedCfg& C
       nx.phase = PH_INITFIN;
       break;
     case PH_INITFIN:
-      nx.phase = PH_CG;
+      // launch k of a solve runs on a tick of parity k & 1: launch 0 waits
+      // for an even tick
+      nx.phase = par == 1 ? PH_CG : PH_ALIGN;
       nx.k = 0;
       break;
+    case PH_ALIGN:
+      nx.phase = PH_CG;
+      break;
     case PH_CG:
-      if (fin) {
-        nx.pcg_total += P.sc->iter;
-        if (C.refine) {
-          // an inner solve: its correction is added whatever its status; a
-          // breakdown ends the refinement after one more residual
-          if (P.sc->status == GP_BREAKDOWN) nx.ir_brk = 1;
-          nx.phase = PH_CORR;
-        } else {
-          if (P.sc->status > nx.status) nx.status = P.sc->status;
-          nx.phase = sched_solved(cur, C);
-        }
-      } else {
+      // the solve's last launch: the pending x and p updates first
+      if (fin)
+        nx.phase = PH_XPFIX;
+      else
         nx.k = cur.k + 1;
+      break;
+    case PH_XPFIX:
+      nx.pcg_total += P.sc->iter;
+      if (C.refine) {
+        // an inner solve: its correction is added whatever its status; a
+        // breakdown ends the refinement after one more residual
+        if (P.sc->status == GP_BREAKDOWN) nx.ir_brk = 1;
+        nx.phase = PH_CORR;
+      } else {
+        if (P.sc->status > nx.status) nx.status = P.sc->status;
+        nx.phase = sched_solved(cur, C);
       }
       break;
     case PH_SENS:
@@ -320,7 +329,14 @@ __device__ __attribute__((noinline)) void sched_recycle_phase(const BatchProb& P
 // Occupancy floor of 4 waves per SIMD: the 128-register budget the PCG sweep
 // fits in, kept for every phase of the kernel (the FP64 sensitivity phase
 // spills a little; a fourth wave per SIMD is worth more to the sweep).
-template <int TJ, int TK>
+// PAR: the tick's parity (t & 1). The PCG phase updates x and p two
+// iterations at a time, as the single-problem solve does: launch k of a solve
+// runs on a tick of parity k & 1 (PH_ALIGN), so odd ticks leave x and p (XP 1)
+// and even ticks apply both pending updates (XP 2), one sweep instantiation
+// per kernel. Launch 0 runs XP 2 with alpha_{-2} = beta_{-2} = 0 (the body
+// reads them only from launch 1 on) and r_{-2} = 0 (cleared in PH_CGINIT): its
+// x and p are the every-iteration form's values.
+template <int TJ, int TK, int PAR>
 __global__ __launch_bounds__(kBlock, 4) void k_tick(SchedTable tab, int t) {
   static_assert(TJ * TK <= kBlock, "the scheduler runs every body with kBlock threads");
   static_assert(sizeof(CornerLds<float, TJ * TK>) >= sizeof(GramLds),
@@ -401,6 +417,11 @@ __global__ __launch_bounds__(kBlock, 4) void k_tick(SchedTable tab, int t) {
       for (int v = lb; v < S.nb_vec; v += nb)
         body_cg_init<float>(P.g.nn, ref ? P.r64 : P.f, P.wb[0], x0, P.u, P.rb[1], P.sb[0],
                             P.wb[1], P.pc, P.minv, P.fixmask, 1, P.parts, v, S.nb_vec);
+      // r_{-2} for launch 0's XP 2 form: finite (zero) whatever a former
+      // solve left in this buffer
+      for (long long d = lb * (long long)kBlock + threadIdx.x; d < P.g.ndof;
+           d += (long long)nb * kBlock)
+        P.rb[0][d] = 0.0f;
       if (lb == 0 && threadIdx.x == 0) P.sc_oc->nan_flag = 0;
       break;
     }
@@ -411,12 +432,15 @@ __global__ __launch_bounds__(kBlock, 4) void k_tick(SchedTable tab, int t) {
                           P.sc);
       break;
     case PH_CG: {
-      const int k = cur.k, c = k & 1, o = c ^ 1;
-      fin = body_cg_modal<float, TJ, TK, float, kBlock, false>(
+      const int k = cur.k, c = k & 1, o = c ^ 1;  // k & 1 == PAR (PH_ALIGN)
+      fin = body_cg_modal<float, TJ, TK, float, kBlock, false, PAR == 0 ? 2 : 1>(
           P.g, P.ni, P.ntj, P.ntk, P.mc, P.E, P.rb[o], P.wb[o], P.sb[c], P.minv, P.rb[c], P.wb[c],
           P.sb[o], P.u, P.pc, P.fixmask, P.sc, P.parts, k, lb, nb, cv);
       break;
     }
+    case PH_XPFIX:
+      body_cg_xp_fix<float>(P.g.nn, P.sc, P.rb[0], P.minv, P.u, P.pc, lb, nb);
+      break;
     case PH_SENS:
       if (S.cfg.refine)
         body_sens<double, true>(P.g, P.ke64, P.mc64, P.xPhys, P.u64, P.law, P.penal, P.E0,
@@ -475,7 +499,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_tick(SchedTable tab, int t) {
       break;
   }
   if (lb == 0 && threadIdx.x == 0) {
-    const SchedState nx = sched_next(S.st[t & 1], S.cfg, P, fin);
+    const SchedState nx = sched_next(S.st[t & 1], S.cfg, P, fin, PAR);
     S.st[(t + 1) & 1] = nx;
     tab.status[b] = nx;
   }
@@ -487,7 +511,10 @@ __global__ __launch_bounds__(kBlock, 4) void k_tick(SchedTable tab, int t) {
 bool sched_tick(const SchedTable& tab, int tj, int tk, int blocks, int t, hipStream_t st) {
 #define GP_TILE(TJ_, TK_)                                                                   \
   if (tj == TJ_ && tk == TK_) {                                                             \
-    hipLaunchKernelGGL((k_tick<TJ_, TK_>), dim3(blocks), dim3(kBlock), 0, st, tab, t);      \
+    if (t & 1)                                                                              \
+      hipLaunchKernelGGL((k_tick<TJ_, TK_, 1>), dim3(blocks), dim3(kBlock), 0, st, tab, t); \
+    else                                                                                    \
+      hipLaunchKernelGGL((k_tick<TJ_, TK_, 0>), dim3(blocks), dim3(kBlock), 0, st, tab, t); \
     return true;                                                                            \
   }
   GP_TILE(8, 32)
