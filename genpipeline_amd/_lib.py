@@ -83,6 +83,7 @@ I32, I64, F64, SZ = C.c_int32, C.c_int64, C.c_double, C.c_size_t
 # name -> argtypes (restype is int for every entry point except the two below)
 SIGNATURES = {
     "gp_footprint_bytes": [C.POINTER(GpSpec), I32, C.POINTER(SZ)],
+    "gp_recycle_bytes": [C.POINTER(GpSpec), I32, C.POINTER(SZ)],
     "gp_hex8_ke": [F64, P],
     "gp_problem_create": [C.POINTER(GpSpec), C.POINTER(P)],
     "gp_problem_destroy": [P],

@@ -439,6 +439,16 @@ int ensure_refine_ws(gp_problem* P) {
 
 // Recycled-start workspace for up to m stored solutions (kept when large
 // enough; the ring restarts empty).
+// Bytes of the ring for m solutions of nd DOFs: m solutions and their
+// products K v in float (rounded up to 256 B), then the Gram partials and the
+// coefficients in FP64.
+size_t recycle_vec_bytes(size_t nd, int m) {
+  return (2 * (size_t)m * nd * sizeof(float) + 255) & ~size_t(255);
+}
+size_t recycle_ws_bytes(size_t nd, int m) {
+  return recycle_vec_bytes(nd, m) + recycle_part_doubles() * sizeof(double);
+}
+
 int ensure_recycle_ws(gp_problem* P, int m) {
   if (P->rc_cap >= m) return GP_OK;
   if (P->rc_mem) {
@@ -446,8 +456,8 @@ int ensure_recycle_ws(gp_problem* P, int m) {
     P->rc_mem = nullptr;
   }
   const size_t nd = (size_t)P->g.ndof;
-  const size_t vb = (2 * (size_t)m * nd * sizeof(float) + 255) & ~size_t(255);
-  ALLOC(P->rc_mem, vb + recycle_part_doubles() * sizeof(double));
+  const size_t vb = recycle_vec_bytes(nd, m);
+  ALLOC(P->rc_mem, recycle_ws_bytes(nd, m));
   char* b = static_cast<char*>(P->rc_mem);
   P->rc_v = reinterpret_cast<float*>(b);
   P->rc_kv = P->rc_v + (size_t)m * nd;
@@ -909,6 +919,16 @@ int gp_set_allocator(gp_alloc_fn alloc_fn, gp_free_fn free_fn, void* ctx) {
   g_alloc_fn.store(alloc_fn);
   g_free_fn.store(free_fn);
   g_alloc_ctx.store(ctx);
+  return GP_OK;
+}
+
+int gp_recycle_bytes(const gp_spec* spec, int32_t recycle, size_t* bytes) {
+  int rc = validate_spec(spec);
+  if (rc) return rc;
+  if (!bytes) return fail(GP_ERR_ARG, "bytes is NULL");
+  if (recycle < 0 || recycle > kRecycleMax)
+    return fail(GP_ERR_ARG, "recycle must be 0..%d", kRecycleMax);
+  *bytes = recycle ? recycle_ws_bytes((size_t)make_grid(spec).ndof, recycle) : 0;
   return GP_OK;
 }
 

@@ -74,12 +74,16 @@ class ProblemSpec:
         return s, keep
 
 
-def footprint_bytes(spec: ProblemSpec, precision: str = "fp32") -> int:
-    """Device bytes one problem holds (used for admission by HBM footprint)."""
+def footprint_bytes(spec: ProblemSpec, precision: str = "fp32", recycle: int = 0) -> int:
+    """Device bytes one problem holds (used for admission by HBM footprint),
+    with the ring of `recycle` stored solutions when it recycles starts."""
     cs, _keep = spec.to_c()
     out = C.c_size_t(0)
     L.call("gp_footprint_bytes", C.byref(cs), PRECISIONS[precision], C.byref(out))
-    return int(out.value)
+    ring = C.c_size_t(0)
+    if recycle:
+        L.call("gp_recycle_bytes", C.byref(cs), int(recycle), C.byref(ring))
+    return int(out.value) + int(ring.value)
 
 
 def _ptr(t: Optional[torch.Tensor]):

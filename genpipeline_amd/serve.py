@@ -129,13 +129,9 @@ def job_args(req: ServeRequest, cfg: ServeConfig) -> dict:
 
 def problem_bytes(shape, kind, cfg: ServeConfig) -> int:
     """Device bytes a pooled problem of this grid and kind holds, with the
-    recycled-start ring of a "simp" problem (stored solutions and their
-    products in float, the Gram partials)."""
-    spec = _spec(shape, kind)
-    b = footprint_bytes(spec)
-    if kind == "simp" and cfg.simp_recycle > 0:
-        b += 2 * cfg.simp_recycle * spec.n_dof * 4 + (44 * 512 + 9) * 8 + 256
-    return b
+    recycled-start ring of a "simp" problem."""
+    return footprint_bytes(_spec(shape, kind),
+                           recycle=cfg.simp_recycle if kind == "simp" else 0)
 
 
 def make_problem(shape, kind, cfg: ServeConfig, device) -> Problem:

@@ -161,6 +161,11 @@ int gp_hex8_ke(double nu, double* ke_out);
 /* Device bytes a problem of this spec allocates (admission by footprint). */
 int gp_footprint_bytes(const gp_spec* spec, int32_t precision, size_t* bytes);
 
+/* Device bytes a problem of this spec adds for recycled starts over
+ * `recycle` stored solutions (0..8; 0: none). No counterpart in the
+ * reference, which keeps no solution history; for admission by footprint. */
+int gp_recycle_bytes(const gp_spec* spec, int32_t recycle, size_t* bytes);
+
 /* ---- problem lifecycle ----------------------------------------------- *
  * Replaces the per-solver set-up of SIMPSolver.__init__ / SIMPSolverGPU.__init__
  * (simp_solver.py:24-54, simp_solver_gpu.py:16-69): KE, edof (derived

@@ -93,3 +93,27 @@ def test_bench_refuses_more_than_one_gpu():
     assert p.returncode == 2
     assert "one GPU only" in p.stdout + p.stderr
     assert p.stdout.strip() == "" or not p.stdout.lstrip().startswith("{")
+
+
+def test_recycle_ring_bytes_without_gpu():
+    """The recycled-start ring of m solutions: 2m float vectors of n_dof (to
+    256 B) plus a fixed FP64 Gram area, whatever the grid; 0 without it;
+    counts above 8 rejected."""
+    from genpipeline_amd import ProblemSpec, footprint_bytes
+    base = {}
+    for nx, ny, nz in ((8, 4, 4), (16, 8, 8)):
+        sp = ProblemSpec(nx, ny, nz)
+        b0 = footprint_bytes(sp)
+        assert footprint_bytes(sp, recycle=0) == b0
+        ring = {m: footprint_bytes(sp, recycle=m) - b0 for m in (1, 6, 8)}
+        for m, b in ring.items():
+            vec = (2 * m * sp.n_dof * 4 + 255) // 256 * 256
+            base[(nx, m)] = b - vec
+            assert b > vec
+        assert ring[1] < ring[6] < ring[8]
+    # the Gram area does not depend on the grid or the count
+    assert len(set(base.values())) == 1
+    lib = _lib.load()
+    cs, _keep = ProblemSpec(8, 4, 4).to_c()
+    out = ctypes.c_size_t()
+    assert lib.gp_recycle_bytes(ctypes.byref(cs), 9, ctypes.byref(out)) == _lib.GP_ERR_ARG
