@@ -437,8 +437,6 @@ int ensure_refine_ws(gp_problem* P) {
   return GP_OK;
 }
 
-// Recycled-start workspace for up to m stored solutions (kept when large
-// enough; the ring restarts empty).
 // Bytes of the ring for m solutions of nd DOFs: m solutions and their
 // products K v in float (rounded up to 256 B), then the Gram partials and the
 // coefficients in FP64.
@@ -449,6 +447,8 @@ size_t recycle_ws_bytes(size_t nd, int m) {
   return recycle_vec_bytes(nd, m) + recycle_part_doubles() * sizeof(double);
 }
 
+// Recycled-start workspace for up to m stored solutions (kept when large
+// enough; the ring restarts empty).
 int ensure_recycle_ws(gp_problem* P, int m) {
   if (P->rc_cap >= m) return GP_OK;
   if (P->rc_mem) {
