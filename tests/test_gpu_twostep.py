@@ -1,8 +1,10 @@
-"""The single-problem PCG updates x and p two iterations at a time (odd
-launches leave them, even launches apply both updates, an odd last iteration
-is completed after the solve); the scheduler's sweep updates them every
+"""The single-problem PCG and the device scheduler update x and p two
+iterations at a time (odd launches leave them, even launches apply both
+updates, an odd last iteration is completed after the solve); the scheduler
+starts its launch 0 in the even form with zero history where the single
+problem uses the every-iteration form. The lockstep batch updates them every
 iteration. Stopped by the iteration cap at odd and even counts, or by
-convergence, both give the same bits."""
+convergence, all give the same bits."""
 import numpy as np
 import pytest
 import torch
@@ -36,7 +38,7 @@ def _sched_solve(gpu, P, xphys, tol, max_iter):
 
 
 @pytest.mark.parametrize("iters", [1, 2, 3, 4, 7, 8])
-def test_capped_solve_bitwise_every_iteration_form(gpu, iters):
+def test_capped_solve_bitwise_scheduler(gpu, iters):
     P, xphys = _setup(gpu)
     E = P.stiffness(xphys)
     u, r = P.pcg(E, tol=0.0, max_iter=iters, precision="fp32")
@@ -48,7 +50,7 @@ def test_capped_solve_bitwise_every_iteration_form(gpu, iters):
 
 
 @pytest.mark.parametrize("seed", [3, 4, 5, 6])
-def test_converged_solve_bitwise_every_iteration_form(gpu, seed):
+def test_converged_solve_bitwise_scheduler(gpu, seed):
     P, xphys = _setup(gpu, seed)
     E = P.stiffness(xphys)
     u, r = P.pcg(E, tol=1e-6, max_iter=5000, precision="fp32")
@@ -56,4 +58,20 @@ def test_converged_solve_bitwise_every_iteration_form(gpu, seed):
     ref = _sched_solve(gpu, P, xphys, 1e-6, 5000)
     assert ref.pcg_iters == r.iters
     assert torch.equal(ref.u, u.double())
+    P.close()
+
+
+@pytest.mark.parametrize("iters", [1, 2, 3, 7, 8])
+def test_capped_run_bitwise_lockstep_every_iteration_form(gpu, iters):
+    """Two OC iterations with every solve capped at `iters` PCG iterations:
+    the lockstep batch's every-iteration x and p update and Problem.run's
+    two-step update give the same u, density and compliance."""
+    from genpipeline_amd.batch import Batch
+    P, _ = _setup(gpu)
+    ref = P.run(2, tol=0.0, max_iter=iters, precision="fp32", warm_start=True)
+    assert ref.pcg_iters == 3 * iters  # two OC solves and the final one
+    with Batch([P], tile=(8, 32)) as B:
+        (got,) = B.run(2, tol=0.0, max_iter=iters)
+    assert torch.equal(got.u, ref.u) and torch.equal(got.xPhys, ref.xPhys)
+    assert got.compliance == ref.compliance and got.pcg_iters == ref.pcg_iters
     P.close()
