@@ -796,14 +796,19 @@ int pcg_core_lp(gp_problem* P, const float* E, const double* rhs, S* d, double t
   launch_cg_init<float, S>(g, rhs, nullptr, false, d, rb[1], sb[0], wb[1], pc, minv, P->fixmask,
                            P->parts, sc, tol, max_iter, s);
   LAUNCHCHK();
-  return pcg_drive(
+  // x and p rounded to 16 bits once per pair of iterations
+  const int rc = pcg_drive(
       P, 2, max_iter, true,
       [&](int k) {
         const int c = k & 1, o = c ^ 1;
         launch_cg_modal(g, tile, &kc, prec, E, rb[o], wb[o], sb[c], minv, rb[c], wb[c], sb[o], d,
-                        pc, P->fixmask, sc, P->parts, k, s);
+                        pc, P->fixmask, sc, P->parts, k, s, true);
       },
       st, s);
+  if (rc) return rc;
+  launch_cg_xp_fix<float, S>(g, sc, rb[0], minv, d, pc, s);
+  LAUNCHCHK();
+  return GP_OK;
 }
 
 // FP64 iterative refinement: r = f - K u in FP64, correction solved in FP32.

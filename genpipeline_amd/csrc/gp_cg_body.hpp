@@ -145,7 +145,8 @@ struct StageIn {
 // launch writes, then p_{k-1}, x_k as with XP 0; each value is formed with the
 // same operations as by the launch that would have stored it, so alternating
 // XP 1 and 2 gives bitwise the x of XP 0 at every stored launch, moving 18 B
-// per node and iteration less. Needs S == T (no storage rounding between).
+// per node and iteration less (S == T). With a 16-bit S the pair of updates is
+// formed in T and rounded to S once, every second iteration.
 template <typename T, int TJ, int TK, typename S = T, int NB = whole_waves(TJ * TK),
           bool UNROLL2 = true, int XP = 0>
 __device__ inline bool body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
@@ -163,7 +164,6 @@ __device__ inline bool body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
   constexpr bool NM = sizeof(T) == 4;
   static_assert(NB >= NT && (NB + 63) / 64 == (NT + 63) / 64, "padding within the last wave");
   static_assert(NB % 64 == 0, "whole waves: the block sums use the DPP wave sum");
-  static_assert(XP == 0 || std::is_same<S, T>::value, "two-step update needs S == T");
   __shared__ T zs[2][3][TJ + 1][TK + 1];
   const int tid = threadIdx.x;
   const bool act = NB == NT || tid < NT;
@@ -440,11 +440,11 @@ __device__ inline bool body_cg_modal(const Grid& g, int NI, int ntj, int ntk,
 // p_{K-1} and x_K are formed with the operations the sweep's XP 2 uses, from
 // alpha_{K-1}, beta_{K-1} (stored by launch K) and r_{K-1}, which launch K-1
 // wrote to r0; nodes lb*kBlock + t, + nb*kBlock, ... Nothing is pending after
-// an even K.
-template <typename T>
+// an even K. S: storage of x and p, as in the sweep.
+template <typename T, typename S = T>
 __device__ inline void body_cg_xp_fix(int nn, const DevScalars* __restrict__ sc,
                                       const T* __restrict__ r0, const T* __restrict__ minv,
-                                      T* __restrict__ x, T* __restrict__ p, int lb, int nb) {
+                                      S* __restrict__ x, S* __restrict__ p, int lb, int nb) {
   const int K = sc->iter;
   if ((K & 1) == 0) return;
   const T alpha = (T)sc->cga[(K - 1) & 1], beta = (T)sc->cgb[(K - 1) & 1];
@@ -455,10 +455,10 @@ __device__ inline void body_cg_xp_fix(int nn, const DevScalars* __restrict__ sc,
       const long long d = 3ll * n + q;
       const T m = NM ? minv[n] : minv[d];
       const T z = m * r0[d];
-      const T pk = z + beta * p[d];
-      const T xk = x[d] + alpha * pk;
-      p[d] = pk;
-      x[d] = xk;
+      const T pk = z + beta * ld_s<T>(p[d]);
+      const T xk = ld_s<T>(x[d]) + alpha * pk;
+      p[d] = st_s<S>(pk);
+      x[d] = st_s<S>(xk);
     }
   }
 }

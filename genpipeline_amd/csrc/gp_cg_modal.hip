@@ -65,7 +65,7 @@ static void launch_cg_ts(const dim3& grid, int NI, int ntj, int ntk, const Grid&
 
 // Launch k of a solve: with two_step, k = 0 updates x and p as usual, odd k
 // leaves them, even k >= 2 applies both pending updates.
-template <int TJ, int TK, typename T>
+template <int TJ, int TK, typename T, typename S = T>
 static void launch_cg_xp(bool two_step, const dim3& grid, int NI, int ntj, int ntk, const Grid& g,
                          const void* kc, const float* E, const void* rp, const void* wp,
                          const void* sp, const void* minv, void* rn, void* wn, void* sn, void* x,
@@ -73,11 +73,11 @@ static void launch_cg_xp(bool two_step, const dim3& grid, int NI, int ntj, int n
                          hipStream_t st) {
 #define GP_XARGS grid, NI, ntj, ntk, g, kc, E, rp, wp, sp, minv, rn, wn, sn, x, p, fixmask, sc, parts, k, st
   if (!two_step || k == 0)
-    launch_cg_ts<TJ, TK, T, T, 0>(GP_XARGS);
+    launch_cg_ts<TJ, TK, T, S, 0>(GP_XARGS);
   else if (k & 1)
-    launch_cg_ts<TJ, TK, T, T, 1>(GP_XARGS);
+    launch_cg_ts<TJ, TK, T, S, 1>(GP_XARGS);
   else
-    launch_cg_ts<TJ, TK, T, T, 2>(GP_XARGS);
+    launch_cg_ts<TJ, TK, T, S, 2>(GP_XARGS);
 #undef GP_XARGS
 }
 
@@ -95,8 +95,8 @@ static void launch_cg_t(const Grid& g, int NI, const void* kc, int prec, const f
   switch (prec) {
     case GP_F32: launch_cg_xp<TJ, TK, float>(two_step, GP_ARGS); break;
     case GP_F64: launch_cg_xp<TJ, TK, double>(two_step, GP_ARGS); break;
-    case GP_F16: launch_cg_ts<TJ, TK, float, _Float16>(GP_ARGS); break;
-    case GP_BF16: launch_cg_ts<TJ, TK, float, bf16>(GP_ARGS); break;
+    case GP_F16: launch_cg_xp<TJ, TK, float, _Float16>(two_step, GP_ARGS); break;
+    case GP_BF16: launch_cg_xp<TJ, TK, float, bf16>(two_step, GP_ARGS); break;
     default: break;
   }
 #undef GP_ARGS
@@ -126,24 +126,28 @@ bool launch_cg_modal(const Grid& g, const KuTile& t, const void* kc, int prec, c
 
 // The pending x and p updates of an odd final iteration of a two-step solve
 // (see launch_cg_modal).
-template <typename T>
+template <typename T, typename S>
 __global__ __launch_bounds__(kBlock) void k_cg_xp_fix(int nn, const DevScalars* __restrict__ sc,
                                                      const T* __restrict__ r0,
                                                      const T* __restrict__ minv,
-                                                     T* __restrict__ x, T* __restrict__ p) {
-  body_cg_xp_fix<T>(nn, sc, r0, minv, x, p, blockIdx.x, gridDim.x);
+                                                     S* __restrict__ x, S* __restrict__ p) {
+  body_cg_xp_fix<T, S>(nn, sc, r0, minv, x, p, blockIdx.x, gridDim.x);
 }
 
-template <typename T>
-void launch_cg_xp_fix(const Grid& g, const DevScalars* sc, const T* r0, const T* minv, T* x,
-                      T* p, hipStream_t st) {
-  hipLaunchKernelGGL((k_cg_xp_fix<T>), dim3(vec_blocks(g)), dim3(kBlock), 0, st, g.nn, sc, r0,
-                     minv, x, p);
+template <typename T, typename S>
+void launch_cg_xp_fix(const Grid& g, const DevScalars* sc, const T* r0, const T* minv, S* x,
+                      S* p, hipStream_t st) {
+  hipLaunchKernelGGL((k_cg_xp_fix<T, S>), dim3(vec_blocks(g)), dim3(kBlock), 0, st, g.nn, sc,
+                     r0, minv, x, p);
 }
-template void launch_cg_xp_fix<float>(const Grid&, const DevScalars*, const float*, const float*,
-                                      float*, float*, hipStream_t);
-template void launch_cg_xp_fix<double>(const Grid&, const DevScalars*, const double*,
-                                       const double*, double*, double*, hipStream_t);
+#define GP_INST(T, S)                                                                        \
+  template void launch_cg_xp_fix<T, S>(const Grid&, const DevScalars*, const T*, const T*, S*, \
+                                       S*, hipStream_t);
+GP_INST(float, float)
+GP_INST(double, double)
+GP_INST(float, _Float16)
+GP_INST(float, bf16)
+#undef GP_INST
 
 template <typename T, typename S>
 __global__ __launch_bounds__(kBlock) void k_cg_init(int nn, const double* __restrict__ f,

@@ -341,10 +341,11 @@ void launch_cg_init(const Grid& g, const double* f, const T* Ax, bool use_x0, S*
 // One Jacobi-PCG iteration k (Chronopoulos-Gear form) on the modal sweep
 // (prec: GP_F32 / GP_F64, or GP_F16 / GP_BF16 16-bit storage of x and p):
 // reads r_{k-1} (rp), w_{k-1} (wp), s_{k-2} (sp); writes r_k, w_k, s_{k-1}
-// (rn, wn, sn); updates x and p in place. two_step (GP_F32 / GP_F64 only):
-// odd launches leave x and p alone and even launches k >= 2 apply both
-// updates, bitwise the same x at every even launch; after the last launch
-// launch_cg_xp_fix completes an odd final iteration.
+// (rn, wn, sn); updates x and p in place. two_step: odd launches leave x and
+// p alone and even launches k >= 2 apply both updates (GP_F32 / GP_F64:
+// bitwise the same x at every even launch; 16-bit storage rounds once per
+// pair); after the last launch launch_cg_xp_fix completes an odd final
+// iteration.
 bool launch_cg_modal(const Grid& g, const KuTile& t, const void* kc, int prec, const float* E,
                      const void* rp, const void* wp, const void* sp, const void* minv, void* rn,
                      void* wn, void* sn, void* x, void* p, const uint8_t* fixmask,
@@ -352,10 +353,10 @@ bool launch_cg_modal(const Grid& g, const KuTile& t, const void* kc, int prec, c
 // After a two-step solve: when the last iteration K (sc->iter) is odd, x_K =
 // x_{K-1} + alpha_{K-1} p_{K-1} with p_{K-1} = M r_{K-1} + beta_{K-1} p_{K-2},
 // as iteration K's launch would have stored them (r0: the r buffer of even
-// iterations).
-template <typename T>
-void launch_cg_xp_fix(const Grid& g, const DevScalars* sc, const T* r0, const T* minv, T* x,
-                      T* p, hipStream_t st);
+// iterations). S: storage of x and p.
+template <typename T, typename S = T>
+void launch_cg_xp_fix(const Grid& g, const DevScalars* sc, const T* r0, const T* minv, S* x,
+                      S* p, hipStream_t st);
 template <typename T>
 void launch_pcg_b(const Grid& g, int it, int nA, T* x, T* r, T* z, const T* p, const T* Ap,
                   const T* minv, Parts parts, DevScalars* sc, hipStream_t st);
