@@ -148,7 +148,8 @@ int gp_version(void);                                 /* 100*major + minor */
  * device, ctx) takes it back. The library waits for the device's queued work
  * before each release, as hipFree does. NULL, NULL restores hipMalloc and
  * hipFree. Returns GP_ERR_STATE while any allocation made through the
- * current pair is alive (set it before creating handles).                */
+ * current pair is alive (set it before creating handles, and not while
+ * another thread may be creating one).                                   */
 typedef void* (*gp_alloc_fn)(size_t bytes, int32_t device, void* ctx);
 typedef void (*gp_free_fn)(void* ptr, int32_t device, void* ctx);
 int gp_set_allocator(gp_alloc_fn alloc, gp_free_fn release, void* ctx);
