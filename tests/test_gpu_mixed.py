@@ -59,3 +59,32 @@ def test_16bit_needs_refinement(gpu):
     with pytest.raises(_lib.GpError, match="refinement"):
         P.pcg(E, precision="bf16", refine=False)
     P.close()
+
+
+@pytest.mark.parametrize("K", [3, 4, 7])
+@pytest.mark.parametrize("prec,bound", [("fp16", 3e-3), ("bf16", 2e-2)])
+def test_capped_16bit_inner_solve_tracks_fp32(gpu, prec, bound, K):
+    """One inner solve capped at K iterations (max_outer 1, u = its
+    correction). x and p are updated two iterations at a time, and after an
+    odd K the last update is completed once the solve ends. The 16-bit
+    iterate equals the float one up to 16-bit rounding, at odd and even K. The
+    float iterate of one iteration fewer differs from it by far more, so a
+    missing last update would fail the bound."""
+    nx, ny, nz = 16, 8, 8
+    P = Problem(ProblemSpec(nx, ny, nz), clamp_x0(nx, ny, nz), traction(nx, ny, nz), device=gpu)
+    E = P.stiffness(torch.full((P.spec.n_elem,), 0.5, dtype=torch.float64, device=gpu))
+
+    def solve(p, k):
+        u, st = P.pcg(E, tol=1e-14, max_iter=k, precision=p, refine=True, inner_tol=1e-14,
+                      max_outer=1)
+        assert st.iters == k and st.outer == 1, (p, k, st)
+        return u
+
+    ref = solve("fp32", K)
+    scale = float(ref.abs().max())
+    d_lp = float((solve(prec, K) - ref).abs().max()) / scale
+    d_prev = float((solve("fp32", K - 1) - ref).abs().max()) / scale
+    print(f"[capped {prec} K={K}] 16-bit vs float {d_lp:.2e}, one iteration fewer {d_prev:.2e}")
+    assert d_lp <= bound, d_lp
+    assert d_prev >= 3 * bound, d_prev
+    P.close()
