@@ -82,7 +82,7 @@ class SIMPSolverGPU:
                  device: str = "cuda", preserved_mask: np.ndarray | None = None,
                  dtype: torch.dtype = torch.float32, *, max_iter: int = 20000,
                  pcg_tol: float | None = None, ke: np.ndarray | None = None,
-                 solver: str | None = None):
+                 solver: str | None = None, recycle: int = 0):
         if not torch.cuda.is_available():
             raise RuntimeError("SIMPSolverGPU needs a ROCm GPU (torch.cuda.is_available() is False)")
         self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
@@ -97,11 +97,20 @@ class SIMPSolverGPU:
         # every solve of run() to that relative residual (no counterpart in
         # the reference)
         self.pcg_tol = None if pcg_tol is None else float(pcg_tol)
+        # > 0: run() starts each warm-started solve from the Galerkin
+        # combination of its last `recycle` solutions (1..8) instead of the
+        # last u alone (no counterpart in the reference, which warm-starts
+        # from the previous u, simp_solver_gpu.py:593,704)
+        self.recycle = int(recycle)
+        if not 0 <= self.recycle <= 8:
+            raise ValueError("recycle must be 0..8")
         table = _lib.hex8_ke(0.3) if ke is None else np.array(ke, np.float64).reshape(24, 24)
         self._ke_dev = None if ke is None else np.ascontiguousarray(table)
         self.solver = solver or ("pcg" if ke is None else "direct")
         if self.solver not in ("pcg", "direct"):
             raise ValueError(f"solver must be 'pcg' or 'direct', not {self.solver!r}")
+        if self.recycle and self.solver != "pcg":
+            raise ValueError("recycle applies to the PCG solver only")
         self.Ke = torch.from_numpy(table).to(
             torch.float64 if dtype == torch.float64 else torch.float32)
         self.last_compliance = 0.0
@@ -191,7 +200,7 @@ class SIMPSolverGPU:
                     tol=self.pcg_tol if fixed_tol else 1e-6, max_iter=self.max_iter,
                     precision=self.precision, warm_start=True, tol_schedule=not fixed_tol,
                     early_stop=True, compliance_each_iter=True,
-                    direct=self.solver == "direct")
+                    direct=self.solver == "direct", recycle=self.recycle)
         self.last_result = res
         self._u_prev = res.u
         self.last_compliance = float(res.compliance)

@@ -92,6 +92,29 @@ def test_simp_solver_gpu_fp32_close_to_oracle(gpu):
     assert abs(s.last_compliance - ref_c) <= 1e-4 * abs(ref_c)
 
 
+def test_simp_solver_gpu_recycled_starts(gpu):
+    """recycle=6 starts each warm-started solve from the Galerkin combination
+    of the run's last six solutions: the same answer to the solver's
+    tolerance, in no more PCG iterations; the direct solver refuses it."""
+    from genpipeline_amd.topology import SIMPSolverGPU
+    nx, ny, nz = 12, 6, 6
+    P = O.reference_problem(nx, ny, nz, force_mag=1.0, volfrac=0.3, ke="hex8", law="clamp",
+                            oc="gpu", gpu_bcs=True)
+    ref_x, ref_c, ref_n = _oracle_simp_gpu_run(P, 10, 0.3)
+    plain = SIMPSolverGPU(nx, ny, nz)
+    plain.run(volfrac=0.3, n_iters=10)
+    s = SIMPSolverGPU(nx, ny, nz, recycle=6)
+    xp = s.run(volfrac=0.3, n_iters=10)
+    assert s.last_result.iters_done == plain.last_result.iters_done == ref_n
+    assert np.abs(xp.ravel() - ref_x).max() < 2e-3
+    assert abs(s.last_compliance - ref_c) <= 1e-4 * abs(ref_c)
+    assert s.last_result.pcg_iters <= plain.last_result.pcg_iters
+    with pytest.raises(ValueError):
+        SIMPSolverGPU(nx, ny, nz, recycle=6, solver="direct")
+    with pytest.raises(ValueError):
+        SIMPSolverGPU(nx, ny, nz, recycle=9)
+
+
 def test_simp_solver_gpu_semi_private_api(gpu):
     """_get_bcs / _assemble_K / _solve as the FEM evaluator calls them."""
     from genpipeline_amd.topology import SIMPSolverGPU
