@@ -210,6 +210,10 @@ std::atomic<long long> g_live{0};
 std::mutex g_alloc_mu;
 
 int alloc(void** p, size_t bytes) {
+  if (bytes == 0) {  // nothing to hold (dev_free ignores NULL)
+    *p = nullptr;
+    return GP_OK;
+  }
   const gp_alloc_fn fn = g_alloc_fn.load();
   if (fn) {
     int dev = 0;
