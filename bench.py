@@ -497,6 +497,21 @@ def bench_mixed128(a):
     the Galerkin combination of the last --recycle solutions (0: the last one
     alone). The compliance and density errors of each are reported against
     FP64."""
+    per, w = mixed128_legs(a)
+    dt = per["fp16+ir"]["seconds"]
+    return _line("SIMP solves/sec, 16-bit PCG storage + FP64 refinement (128x64x64)", 1.0 / dt,
+                 "solves/s", 1, 1, dt * 1e3, "fp16-storage/f32-arith/f64-refine",
+                 {"workload": f"simp_mixed_128x64x64_seed{a.seed}", "grid": [128, 64, 64],
+                  "oc_iters": w.n_iters, "outer_tol": 1e-6, "inner_tol": 1e-3,
+                  "recycle": a.recycle,
+                  "load": "traction, total 1, face x=max", "volfrac": 0.25},
+                 extra={"runs": per})
+
+
+def mixed128_legs(a, names=None):
+    """The C3 runs named in names (all five when None; the FP64 reference
+    always runs first): per-run seconds, PCG iterations and errors against
+    FP64, and the workload."""
     import torch
     from genpipeline_amd import Problem
     from genpipeline_amd.problems import mixed_128x64x64
@@ -511,7 +526,10 @@ def bench_mixed128(a):
             ("bf16+ir", dict(precision="bf16", refine=True, tol=1e-6, inner_tol=1e-3, **rc)),
             ("fp32+ir", dict(precision="fp32", refine=True, tol=1e-6, inner_tol=1e-3, **rc)),
             ("fp32", dict(precision="fp32", tol=1e-6, **rc))]
-    P.run(2, **base, **runs[1][1])  # warm-up of the 16-bit path
+    if names is not None:
+        runs = [r for r in runs if r[0] == "fp64" or r[0] in names]
+    # warm-up: the 16-bit path when it runs
+    P.run(2, **base, **dict(runs).get("fp16+ir", runs[0][1]))
     res = {}
     for name, kw in runs:
         torch.cuda.synchronize()
@@ -530,14 +548,7 @@ def bench_mixed128(a):
                      / abs(ref.compliance),
                      "density_max_abs_err_vs_fp64": float((r.xPhys - ref.xPhys).abs().max())}
     P.close()
-    r, dt = res["fp16+ir"]
-    return _line("SIMP solves/sec, 16-bit PCG storage + FP64 refinement (128x64x64)", 1.0 / dt,
-                 "solves/s", 1, 1, dt * 1e3, "fp16-storage/f32-arith/f64-refine",
-                 {"workload": f"simp_mixed_128x64x64_seed{a.seed}", "grid": [128, 64, 64],
-                  "oc_iters": w.n_iters, "outer_tol": 1e-6, "inner_tol": 1e-3,
-                  "recycle": a.recycle,
-                  "load": "traction, total 1, face x=max", "volfrac": 0.25},
-                 extra={"runs": per})
+    return per, w
 
 
 def _event_us(fn, reps):
@@ -924,7 +935,9 @@ def main():
             out["extra"]["reference_literal_c1"], u_lit = gpu_reference_literal_c1()
         except Exception as e:
             out["extra"]["reference_literal_c1"] = {"error": f"{type(e).__name__}: {e}"}
-        for key, fn in (("batch48", lambda: batch48_legs(a)), ("serve", lambda: serve_legs(a))):
+        for key, fn in (("batch48", lambda: batch48_legs(a)),
+                        ("mixed128", lambda: mixed128_legs(a, ("fp16+ir", "fp32+ir"))[0]),
+                        ("serve", lambda: serve_legs(a))):
             try:
                 out["extra"][key] = fn()
             except Exception as e:  # the headline line is printed whatever a side leg does
