@@ -236,8 +236,9 @@ def test_pcg_modal_and_gather_agree(gpu):
 
 
 def test_pcg_breakdown_on_indefinite_table(gpu):
-    """The reference's own element table is indefinite: PCG must report
-    breakdown instead of returning garbage (survey finding F4)."""
+    """The reference's own element table (simp_solver.py:284-1543) is
+    asymmetric and its symmetric part has two negative eigenvalues: PCG must
+    report breakdown (p^T A p <= 0) instead of returning garbage."""
     grid = (8, 4, 4)
     spec, fixed, f, x, rng = _problem(*grid, seed=7, law="clamp")
     KEr = O.ke_ref()
